@@ -1,0 +1,138 @@
+"""ctypes mirror of include/dhara_hip.h (structures, constants, field ids).
+
+Kept in one place so every C library with this ABI (the HIP product library and anything
+else that implements the same entry points) is driven by the same definitions.
+"""
+import ctypes as C
+
+ABI_VERSION = 1
+
+OK, E_ARG, E_DEVICE, E_ALLOC, E_STATE, E_NUMERIC, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
+
+PICARD_ABS_PSI, PICARD_REL_PSI, PICARD_REL_THETA = 0, 1, 2
+FP64, MIXED = 0, 1
+THETA_CARRY_CONSISTENT, THETA_CARRY_REFERENCE = 0, 1
+TOP_SURFACE, TOP_PRESCRIBED = 0, 1
+INIT_FROM_THETA, INIT_FROM_PSI = 0, 1
+BC_DIRICHLET, BC_FLUX = 0, 1
+
+_FIELDS = [
+    "PSI", "THETA", "KSAT", "ALPHA", "NVG", "K", "C",
+    "ZTOPO", "MANNING", "PONDING", "WATERELEV", "RAIN", "EVAP", "TRANSP", "QTOP", "QBOT",
+    "HPOTEN", "QCAPA",
+    "BCPSI_W", "BCPSI_E", "BCPSI_S", "BCPSI_N", "BCPSI_T", "BCPSI_B",
+    "BCQ_W", "BCQ_E", "BCQ_S", "BCQ_N", "BCQ_T", "BCQ_B",
+    "ROOT",
+    "BCTYPE_W", "BCTYPE_E", "BCTYPE_S", "BCTYPE_N", "BCTYPE_T", "BCTYPE_B",
+    "PROCMAP",
+]
+F = {name: i for i, name in enumerate(_FIELDS)}
+INT_FIELDS = {F[n] for n in ("BCTYPE_W", "BCTYPE_E", "BCTYPE_S", "BCTYPE_N", "BCTYPE_T",
+                             "BCTYPE_B", "PROCMAP")}
+CELL_FIELDS = {F[n] for n in ("PSI", "THETA", "KSAT", "ALPHA", "NVG", "K", "C")}
+SURF_FIELDS = {F[n] for n in ("ZTOPO", "MANNING", "PONDING", "WATERELEV", "RAIN", "EVAP",
+                              "TRANSP", "QTOP", "QBOT", "HPOTEN", "QCAPA", "BCPSI_T",
+                              "BCPSI_B", "BCQ_T", "BCQ_B", "BCTYPE_T", "BCTYPE_B", "PROCMAP")}
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("nroot_groups", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("theta_s", C.c_double), ("theta_r", C.c_double), ("alpha", C.c_double),
+        ("n_vg", C.c_double), ("Ss", C.c_double), ("porosity", C.c_double),
+        ("picard_maxiter", C.c_int32), ("picard_rule", C.c_int32),
+        ("picard_tol", C.c_double), ("am", C.c_double),
+        ("lin_maxiter", C.c_int32), ("precision", C.c_int32),
+        ("lin_tol", C.c_double),
+        ("refine_maxiter", C.c_int32), ("inner_maxiter", C.c_int32),
+        ("inner_tol", C.c_double),
+        ("overland", C.c_int32), ("olf_maxiter", C.c_int32),
+        ("olf_tol", C.c_double),
+        ("hmin", C.c_double), ("delta", C.c_double), ("K0", C.c_double),
+        ("theta_carry", C.c_int32), ("dirichlet_face_quirk", C.c_int32),
+        ("top_bc", C.c_int32), ("reserved0", C.c_int32),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("picard_iters", C.c_int32), ("picard_converged", C.c_int32),
+        ("lin_iters", C.c_int32), ("lin_converged", C.c_int32),
+        ("refine_sweeps", C.c_int32),
+        ("olf_iters", C.c_int32), ("olf_converged", C.c_int32),
+        ("nan_seen", C.c_int32),
+        ("picard_delta", C.c_double),
+        ("mb_ref", C.c_double), ("storage_change", C.c_double),
+        ("boundary_inflow", C.c_double), ("mb_rel", C.c_double),
+        ("outlet_discharge", C.c_double), ("ring_outflow", C.c_double),
+        ("wall_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def default_params(**overrides):
+    """Same values as dhara_params_defaults() in include/dhara_hip.h."""
+    p = Params(
+        dt=60.0, theta_s=0.43, theta_r=0.078, alpha=3.6, n_vg=1.56, Ss=1e-5, porosity=0.43,
+        picard_maxiter=20, picard_rule=PICARD_ABS_PSI, picard_tol=1e-6, am=0.0,
+        lin_maxiter=2000, precision=FP64, lin_tol=1e-12,
+        refine_maxiter=8, inner_maxiter=2000, inner_tol=1e-5,
+        overland=1, olf_maxiter=2000, olf_tol=1e-12,
+        hmin=1e-5, delta=1e-7, K0=1e-5,
+        theta_carry=THETA_CARRY_CONSISTENT, dirichlet_face_quirk=1,
+        top_bc=TOP_SURFACE, reserved0=0)
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(f"unknown parameter {k!r}")
+        setattr(p, k, v)
+    return p
+
+
+def face_size(grid, field):
+    """Element count of a field for this grid."""
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    name = _FIELDS[field]
+    if field in CELL_FIELDS:
+        return nx * ny * nz
+    if name == "ROOT":
+        return grid.nroot_groups * nz
+    if name.endswith("_W") or name.endswith("_E"):
+        return ny * nz
+    if name.endswith("_S") or name.endswith("_N"):
+        return nx * nz
+    return nx * ny
+
+
+def declare(lib, prefix):
+    """Attach argtypes/restype for the common entry points of a library with this ABI."""
+    P = C.POINTER
+    vp, dp, ip, sz = C.c_void_p, P(C.c_double), P(C.c_int32), C.c_size_t
+    sig = {
+        "destroy": [vp],
+        "set_params": [vp, P(Params)],
+        "upload_field": [vp, C.c_int32, dp, sz],
+        "upload_field_i32": [vp, C.c_int32, ip, sz],
+        "download_field": [vp, C.c_int32, dp, sz],
+        "download_field_i32": [vp, C.c_int32, ip, sz],
+        "init_state": [vp, C.c_int32],
+        "subsurface_step": [vp, P(StepStats)],
+        "overland_step": [vp, P(StepStats)],
+        "step": [vp, C.c_int32, P(StepStats)],
+        "eval_constitutive": [vp, dp, dp, dp, dp],
+        "eval_inverse": [vp, dp, dp],
+        "apply_subsurface_operator": [vp, dp, dp],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, prefix + name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    getattr(lib, prefix + "last_error").argtypes = [vp]
+    getattr(lib, prefix + "last_error").restype = C.c_char_p

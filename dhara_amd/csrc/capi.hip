@@ -1,0 +1,958 @@
+// capi.hip -- context, device memory and the C-ABI of include/dhara_hip.h.
+//
+// One context = one HIP stream + every device buffer of one model.  The time loop of the
+// reference (src/device.cu:374-475 -> src/subsurface.cu:686-847 -> src/overland.cu:387-419)
+// is driven from here; all arithmetic runs in kernels.hip.  The host synchronises once per
+// Krylov batch, once per Picard iteration and once at the end of a step (the reference
+// synchronised on every dot product, Picard max and mass-balance reduction).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dhara_hip.h"
+#include "kernels.h"
+
+using dh::FB;
+using dh::FE;
+using dh::FN;
+using dh::FS;
+using dh::FT;
+using dh::FW;
+
+struct dhara_ctx {
+    dhara_grid grid{};
+    dhara_params p{};
+    int dev = 0;
+    hipStream_t s = nullptr;
+    std::string err;
+    int N = 0, S = 0;
+    dh::Geo g{};
+    dh::Geo2 g2{};
+
+    // 3D FP64 state
+    double *psi_n = nullptr, *psi_m = nullptr, *x = nullptr;
+    double *theta_n = nullptr, *theta_m = nullptr, *theta_new = nullptr;
+    double *K = nullptr, *C = nullptr, *Ks = nullptr, *alpha_f = nullptr, *n_f = nullptr;
+    double *D = nullptr, *b = nullptr, *r = nullptr, *pv[2] = {nullptr, nullptr}, *q = nullptr;
+    // 3D FP32 solver arrays (mixed precision)
+    float *K32 = nullptr, *D32 = nullptr, *r32 = nullptr, *d32 = nullptr;
+    float *p32[2] = {nullptr, nullptr}, *q32 = nullptr;
+    uint8_t* flags = nullptr;
+    // surface
+    double *z = nullptr, *mann = nullptr, *h = nullptr, *H = nullptr;
+    double *rain = nullptr, *evap = nullptr, *transp = nullptr, *hpoten = nullptr, *qcapa = nullptr;
+    double *bcpsi[6] = {}, *bcq[6] = {};
+    int* bctype[6] = {};
+    int* procmap = nullptr;
+    double* root = nullptr;
+    // overland solver
+    double *D2 = nullptr, *b2 = nullptr, *r2 = nullptr, *p2[2] = {nullptr, nullptr}, *q2 = nullptr;
+    double *ke = nullptr, *kn = nullptr;
+    // reductions
+    double* part = nullptr;
+    unsigned* cnt = nullptr;
+    dh::PcgState* st_sub = nullptr;
+    dh::PcgState* st_olf = nullptr;
+    dh::PicardState* pst = nullptr;
+    double* sums = nullptr;
+    // pinned host mirrors
+    dh::PcgState* h_st = nullptr;
+    dh::PicardState* h_pst = nullptr;
+    double* h_sums = nullptr;
+
+    bool face_dirichlet[6] = {false, false, false, false, false, false};
+    bool has_alpha = false, has_n = false, has_transp = false;
+    bool initialized = false;
+    bool theta_m_current = true;
+    int pcg_hint = 8, olf_hint = 8;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int set_err(dhara_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err((ctx), DHARA_E_DEVICE,                                           \
+                           std::string(#call) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+#define LAUNCHCHK(ctx)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess)                                                               \
+            return set_err((ctx), DHARA_E_DEVICE,                                           \
+                           std::string("kernel launch: ") + hipGetErrorString(e_));         \
+    } while (0)
+
+template <typename T>
+int dalloc(dhara_ctx* c, T** ptr, size_t count)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return set_err(c, DHARA_E_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(v, 0, std::max<size_t>(count, 1) * sizeof(T), c->s);
+    if (e != hipSuccess) return set_err(c, DHARA_E_DEVICE, "hipMemset");
+    c->allocs.push_back(v);
+    *ptr = static_cast<T*>(v);
+    return DHARA_OK;
+}
+
+size_t face_size(const dhara_grid& g, int f)
+{
+    switch (f) {
+    case FW: case FE: return (size_t)g.ny * g.nz;
+    case FS: case FN: return (size_t)g.nx * g.nz;
+    default: return (size_t)g.nx * g.ny;
+    }
+}
+
+int alloc_mixed(dhara_ctx* c)
+{
+    if (c->K32) return DHARA_OK;
+    const size_t N = c->N;
+    int rc;
+    if ((rc = dalloc(c, &c->K32, N)) || (rc = dalloc(c, &c->D32, N)) || (rc = dalloc(c, &c->r32, N)) ||
+        (rc = dalloc(c, &c->d32, N)) || (rc = dalloc(c, &c->p32[0], N)) || (rc = dalloc(c, &c->p32[1], N)) ||
+        (rc = dalloc(c, &c->q32, N)))
+        return rc;
+    // K32 must mirror K
+    dh::launch_cast_f32(c->s, c->N, c->K, c->K32);
+    LAUNCHCHK(c);
+    return DHARA_OK;
+}
+
+int validate_params(dhara_ctx* c, const dhara_params* p)
+{
+    if (!(p->dt > 0) || !(p->theta_s > p->theta_r) || !(p->alpha > 0) || !(p->n_vg > 1.0) ||
+        !(p->porosity > 0) || p->Ss < 0)
+        return set_err(c, DHARA_E_ARG, "invalid soil/time parameters");
+    if (p->precision != DHARA_FP64 && p->precision != DHARA_MIXED)
+        return set_err(c, DHARA_E_ARG, "precision must be DHARA_FP64 or DHARA_MIXED");
+    if (p->picard_rule < 0 || p->picard_rule > 2) return set_err(c, DHARA_E_ARG, "bad picard_rule");
+    if (p->top_bc != DHARA_TOP_SURFACE && p->top_bc != DHARA_TOP_PRESCRIBED)
+        return set_err(c, DHARA_E_ARG, "bad top_bc");
+    return DHARA_OK;
+}
+
+dh::Soil soil_of(const dhara_ctx* c)
+{
+    dh::Soil s;
+    s.Ks = c->Ks;
+    s.alpha_f = c->has_alpha ? c->alpha_f : nullptr;
+    s.n_f = c->has_n ? c->n_f : nullptr;
+    s.alpha = c->p.alpha;
+    s.n = c->p.n_vg;
+    s.ts = c->p.theta_s;
+    s.tr = c->p.theta_r;
+    return s;
+}
+
+dh::Reduce red_of(const dhara_ctx* c) { return dh::Reduce{c->part, c->cnt}; }
+
+bool has_dirichlet(const dhara_ctx* c)
+{
+    for (int f = 0; f < 6; ++f) {
+        if (f == FT && c->p.top_bc == DHARA_TOP_SURFACE) continue;
+        if (c->face_dirichlet[f]) return true;
+    }
+    return false;
+}
+
+int read_state(dhara_ctx* c, dh::PcgState* dev)
+{
+    HIPCHK(c, hipMemcpyAsync(c->h_st, dev, sizeof(dh::PcgState), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+dh::VgArgs vg_args(dhara_ctx* c, const double* psi, double* theta)
+{
+    dh::VgArgs a{};
+    a.g = c->g;
+    a.soil = soil_of(c);
+    a.psi = psi;
+    a.psi_m = c->psi_m;
+    a.theta_m = c->theta_m;
+    a.theta = theta;
+    a.K = c->K;
+    a.C = c->C;
+    a.K32 = c->K32;
+    a.rule = c->p.picard_rule;
+    a.am = c->p.am;
+    a.pst = c->pst;
+    a.red = red_of(c);
+    return a;
+}
+
+dh::SurfArgs surf_args(dhara_ctx* c)
+{
+    dh::SurfArgs a{};
+    a.nxy = c->S;
+    a.dt = c->p.dt;
+    a.dz = c->grid.dz;
+    a.ts = c->p.theta_s;
+    a.K = c->K;
+    a.psi_n = c->psi_n;
+    a.theta_n = c->theta_n;
+    a.Ks = c->Ks;
+    a.h = c->h;
+    a.hpoten = c->hpoten;
+    a.qcapa = c->qcapa;
+    a.rain = c->rain;
+    a.evap = c->evap;
+    a.qtop = c->bcq[FT];
+    a.toptype = c->bctype[FT];
+    return a;
+}
+
+dh::AsmArgs asm_args(dhara_ctx* c, bool dir)
+{
+    dh::AsmArgs a{};
+    a.g = c->g;
+    a.dt = c->p.dt;
+    a.Ss = c->p.Ss;
+    a.poros = c->p.porosity;
+    a.K = c->K;
+    a.C = c->C;
+    a.theta_m = c->theta_m;
+    a.theta_n = c->theta_n;
+    a.psi_n = c->psi_n;
+    a.psi_m = c->psi_m;
+    a.x = c->x;
+    a.transp = c->transp;
+    a.root = c->root;
+    a.procmap = c->procmap;
+    a.has_transp = c->has_transp ? 1 : 0;
+    for (int f = 0; f < 6; ++f) {
+        a.bctype[f] = c->bctype[f];
+        a.bcpsi[f] = c->bcpsi[f];
+        a.bcq[f] = c->bcq[f];
+    }
+    a.flags = dir ? c->flags : nullptr;
+    a.quirk = c->p.dirichlet_face_quirk;
+    a.D = c->D;
+    a.b = c->b;
+    a.r = c->r;
+    a.D32 = c->D32;
+    a.r32 = c->r32;
+    a.d32 = c->d32;
+    a.lin_tol = c->p.lin_tol;
+    a.inner_tol = c->p.inner_tol;
+    a.maxit = c->p.lin_maxiter;
+    a.inner_maxit = c->p.inner_maxiter;
+    a.st = c->st_sub;
+    a.red = red_of(c);
+    return a;
+}
+
+dh::OlfArgs olf_args(dhara_ctx* c)
+{
+    dh::OlfArgs a{};
+    a.g = c->g2;
+    a.hmin = c->p.hmin;
+    a.delta = c->p.delta;
+    a.K0 = c->p.K0;
+    a.h = c->h;
+    a.z = c->z;
+    a.mann = c->mann;
+    a.H = c->H;
+    a.ke = c->ke;
+    a.kn = c->kn;
+    a.D = c->D2;
+    a.b = c->b2;
+    a.r = c->r2;
+    a.tol = c->p.olf_tol;
+    a.maxit = c->p.olf_maxiter;
+    a.st = c->st_olf;
+    a.red = red_of(c);
+    a.out = c->sums;
+    return a;
+}
+
+// Queue Krylov iterations in batches; the kernels stop doing work once the device-side
+// state says the solve is done, so over-queued iterations cost one launch each.
+template <class IterFn>
+int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, IterFn iter)
+{
+    int parity = 0, launched = 0;
+    int batch = std::max(4, hint);
+    for (;;) {
+        for (int b = 0; b < batch; ++b) {
+            iter(parity);
+            parity ^= 1;
+        }
+        LAUNCHCHK(c);
+        launched += batch;
+        int rc = read_state(c, st);
+        if (rc) return rc;
+        if (c->h_st->done) break;
+        if (launched > maxit + 4) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
+        batch = std::max(4, std::min(16, (hint - launched) > 0 ? (hint - launched) : 8));
+    }
+    hint = std::max(1, c->h_st->iter);
+    return DHARA_OK;
+}
+
+int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
+{
+    const bool dir = has_dirichlet(c);
+    dh::launch_assemble(c->s, asm_args(c, dir), false);
+    LAUNCHCHK(c);
+    double* const pp[2] = {c->pv[0], c->pv[1]};
+    int rc = run_krylov(c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, [&](int par) {
+        dh::launch_pcg3_iter_f64(c->s, c->g, dir ? c->flags : nullptr, c->K, c->D, c->x, c->r, pp,
+                                 c->q, par, c->st_sub, red_of(c));
+    });
+    if (rc) return rc;
+    st->lin_iters += c->h_st->iter;
+    st->lin_converged &= c->h_st->conv;
+    if (c->h_st->nan) st->nan_seen = 1;
+    return DHARA_OK;
+}
+
+int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
+{
+    const bool dir = has_dirichlet(c);
+    dh::launch_assemble(c->s, asm_args(c, dir), true);
+    LAUNCHCHK(c);
+    float* const pp[2] = {c->p32[0], c->p32[1]};
+    dh::RefineArgs ra{};
+    ra.g = c->g;
+    ra.K = c->K;
+    ra.D = c->D;
+    ra.b = c->b;
+    ra.x = c->x;
+    ra.flags = dir ? c->flags : nullptr;
+    ra.r32 = c->r32;
+    ra.D32 = c->D32;
+    ra.d32 = c->d32;
+    ra.lin_tol = c->p.lin_tol;
+    ra.inner_tol = c->p.inner_tol;
+    ra.st = c->st_sub;
+    ra.red = red_of(c);
+    int sweeps = 0;
+    int rc = read_state(c, c->st_sub);
+    if (rc) return rc;
+    bool outer_conv = c->h_st->outer_conv != 0;
+    while (!outer_conv && sweeps < c->p.refine_maxiter) {
+        rc = run_krylov(c, c->st_sub, c->p.inner_maxiter, c->pcg_hint, [&](int par) {
+            dh::launch_pcg3_iter_f32(c->s, c->g, dir ? c->flags : nullptr, c->K32, c->D32, c->d32,
+                                     c->r32, pp, c->q32, par, c->st_sub, red_of(c));
+        });
+        if (rc) return rc;
+        st->lin_iters += c->h_st->iter;
+        ++sweeps;
+        dh::launch_refine(c->s, ra);
+        LAUNCHCHK(c);
+        rc = read_state(c, c->st_sub);
+        if (rc) return rc;
+        outer_conv = c->h_st->outer_conv != 0;
+        if (c->h_st->nan) { st->nan_seen = 1; break; }
+    }
+    st->refine_sweeps += sweeps;
+    st->lin_converged &= outer_conv ? 1 : 0;
+    return DHARA_OK;
+}
+
+int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
+{
+    const bool dir = has_dirichlet(c);
+    const bool mixed = c->p.precision == DHARA_MIXED;
+    if (mixed) {
+        int rc = alloc_mixed(c);
+        if (rc) return rc;
+    }
+    if (c->p.am != 0.0)
+        HIPCHK(c, hipMemcpyAsync(c->psi_m, c->psi_n, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->s));
+    // the surface pass forces the top face to flux type, so the row flags come after it
+    dh::SurfArgs sa = surf_args(c);
+    if (c->p.top_bc == DHARA_TOP_SURFACE) dh::launch_surface_pre(c->s, sa);
+    if (dir) {
+        const int* bt[6];
+        for (int f = 0; f < 6; ++f) bt[f] = c->bctype[f];
+        dh::launch_row_flags(c->s, c->g, bt, c->flags);
+    }
+    LAUNCHCHK(c);
+    if (!c->theta_m_current) {
+        std::swap(c->theta_m, c->theta_new);
+        c->theta_m_current = true;
+    }
+    st->lin_converged = 1;
+    int niter = 0, conv = 0;
+    double delta = 0.0;
+    while (!conv && niter < c->p.picard_maxiter) {
+        int rc = mixed ? solve_mixed(c, st) : solve_fp64(c, st);
+        if (rc) return rc;
+        dh::launch_vg_picard(c->s, vg_args(c, c->x, c->theta_new));
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(c->h_pst, c->pst, sizeof(dh::PicardState), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        ++niter;
+        delta = c->h_pst->delta;
+        if (c->h_pst->nan) { st->nan_seen = 1; c->theta_m_current = false; break; }
+        if (delta < c->p.picard_tol) {
+            conv = 1;
+            c->theta_m_current = false;
+        } else if (niter < c->p.picard_maxiter) {
+            if (c->p.am == 0.0) {
+                std::swap(c->theta_m, c->theta_new);
+            } else {
+                dh::launch_vg(c->s, vg_args(c, c->psi_m, c->theta_m));
+                LAUNCHCHK(c);
+            }
+            c->theta_m_current = true;
+        } else {
+            c->theta_m_current = false;
+        }
+    }
+    st->picard_iters = niter;
+    st->picard_converged = conv;
+    st->picard_delta = st->nan_seen ? NAN : delta;
+    if (c->p.top_bc == DHARA_TOP_SURFACE) dh::launch_surface_post(c->s, sa);
+    dh::EndArgs ea{};
+    ea.g = c->g;
+    ea.dt = c->p.dt;
+    ea.Ss = c->p.Ss;
+    ea.poros = c->p.porosity;
+    ea.x = c->x;
+    ea.theta_new = c->theta_new;
+    ea.theta_m = c->theta_m;
+    ea.psi_n = c->psi_n;
+    ea.theta_n = c->theta_n;
+    ea.transp = c->transp;
+    ea.root = c->root;
+    ea.procmap = c->procmap;
+    ea.has_transp = c->has_transp ? 1 : 0;
+    ea.carry_reference = c->p.theta_carry == DHARA_THETA_CARRY_REFERENCE ? 1 : 0;
+    for (int f = 0; f < 6; ++f) {
+        ea.bctype[f] = c->bctype[f];
+        ea.bcq[f] = c->bcq[f];
+    }
+    ea.red = red_of(c);
+    ea.out = c->sums;
+    dh::launch_end_step(c->s, ea);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(c->h_sums, c->sums, 9 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    // theta carry consumed theta_m; the last vG result (theta_new) is theta(psi^n)
+    c->theta_m_current = false;
+    const double* s = c->h_sums;
+    const double dx = c->grid.dx, dy = c->grid.dy, dz = c->grid.dz, dt = c->p.dt;
+    const double sdtheta = s[0], sss = s[1], ssink = s[2];
+    const double qt = s[3], qb = s[4], qw = s[5], qe = s[6], qs = s[7], qn = s[8];
+    st->mb_ref = sdtheta * dz + ((qe - qw) + (qn - qs) + (qb - qt)) * dt + ssink + sss;
+    st->storage_change = (sdtheta * dz + sss) * dx * dy;
+    const double inflow = (qt - qb) * dx * dy * dt + (qw - qe) * dy * dz * dt + (qs - qn) * dx * dz * dt -
+                          ssink * dx * dy;
+    st->boundary_inflow = inflow;
+    const double flux_scale = (std::fabs(qt) + std::fabs(qb)) * dx * dy * dt +
+                              (std::fabs(qw) + std::fabs(qe)) * dy * dz * dt +
+                              (std::fabs(qs) + std::fabs(qn)) * dx * dz * dt + ssink * dx * dy;
+    const double scale = std::max({std::fabs(st->storage_change), std::fabs(inflow), flux_scale, 1e-300});
+    st->mb_rel = std::fabs(st->storage_change - inflow) / scale;
+    if (!std::isfinite(st->mb_rel)) st->nan_seen = 1;
+    return st->nan_seen ? set_err(c, DHARA_E_NUMERIC, "NaN/Inf in the subsurface state") : DHARA_OK;
+}
+
+int do_overland_step(dhara_ctx* c, dhara_step_stats* st)
+{
+    dh::OlfArgs a = olf_args(c);
+    dh::launch_olf_faces(c->s, a);
+    dh::launch_olf_assemble(c->s, a);
+    LAUNCHCHK(c);
+    double* const pp[2] = {c->p2[0], c->p2[1]};
+    int rc = run_krylov(c, c->st_olf, c->p.olf_maxiter, c->olf_hint, [&](int par) {
+        dh::launch_pcg2_iter(c->s, c->g2, c->ke, c->kn, c->D2, c->H, c->r2, pp, c->q2, par, c->st_olf,
+                             red_of(c));
+    });
+    if (rc) return rc;
+    st->olf_iters = c->h_st->iter;
+    st->olf_converged = c->h_st->conv;
+    if (c->h_st->nan) st->nan_seen = 1;
+    dh::launch_olf_post(c->s, a);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(c->h_sums, c->sums, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    st->outlet_discharge = c->h_sums[0];
+    st->ring_outflow = c->h_sums[1];
+    return st->nan_seen ? set_err(c, DHARA_E_NUMERIC, "NaN/Inf in the overland state") : DHARA_OK;
+}
+
+double* dfield(dhara_ctx* c, int f, size_t* n)
+{
+    const size_t N = c->N, S = c->S;
+    switch (f) {
+    case DHARA_F_PSI: *n = N; return c->psi_n;
+    case DHARA_F_THETA: *n = N; return c->theta_n;
+    case DHARA_F_KSAT: *n = N; return c->Ks;
+    case DHARA_F_ALPHA: *n = N; return c->alpha_f;
+    case DHARA_F_NVG: *n = N; return c->n_f;
+    case DHARA_F_K: *n = N; return c->K;
+    case DHARA_F_C: *n = N; return c->C;
+    case DHARA_F_ZTOPO: *n = S; return c->z;
+    case DHARA_F_MANNING: *n = S; return c->mann;
+    case DHARA_F_PONDING: *n = S; return c->h;
+    case DHARA_F_WATERELEV: *n = S; return c->H;
+    case DHARA_F_RAIN: *n = S; return c->rain;
+    case DHARA_F_EVAP: *n = S; return c->evap;
+    case DHARA_F_TRANSP: *n = S; return c->transp;
+    case DHARA_F_QTOP: *n = S; return c->bcq[FT];
+    case DHARA_F_QBOT: *n = S; return c->bcq[FB];
+    case DHARA_F_HPOTEN: *n = S; return c->hpoten;
+    case DHARA_F_QCAPA: *n = S; return c->qcapa;
+    case DHARA_F_ROOT: *n = (size_t)c->grid.nroot_groups * c->grid.nz; return c->root;
+    default: break;
+    }
+    if (f >= DHARA_F_BCPSI_W && f <= DHARA_F_BCPSI_B) {
+        *n = face_size(c->grid, f - DHARA_F_BCPSI_W);
+        return c->bcpsi[f - DHARA_F_BCPSI_W];
+    }
+    if (f >= DHARA_F_BCQ_W && f <= DHARA_F_BCQ_B) {
+        *n = face_size(c->grid, f - DHARA_F_BCQ_W);
+        return c->bcq[f - DHARA_F_BCQ_W];
+    }
+    return nullptr;
+}
+
+int* ifield(dhara_ctx* c, int f, size_t* n)
+{
+    if (f >= DHARA_F_BCTYPE_W && f <= DHARA_F_BCTYPE_B) {
+        *n = face_size(c->grid, f - DHARA_F_BCTYPE_W);
+        return c->bctype[f - DHARA_F_BCTYPE_W];
+    }
+    if (f == DHARA_F_PROCMAP) { *n = c->S; return c->procmap; }
+    return nullptr;
+}
+
+void after_upload(dhara_ctx* c, int f)
+{
+    if (f == DHARA_F_ALPHA) c->has_alpha = true;
+    if (f == DHARA_F_NVG) c->has_n = true;
+    if (f == DHARA_F_TRANSP) c->has_transp = true;
+    if (f == DHARA_F_PSI || f == DHARA_F_THETA || f == DHARA_F_KSAT || f == DHARA_F_ALPHA ||
+        f == DHARA_F_NVG)
+        c->initialized = false;
+}
+
+}  // namespace
+
+// =========================================================================================
+extern "C" {
+
+int dhara_abi_version(void) { return DHARA_ABI_VERSION; }
+
+int dhara_device_info(char* buf, size_t n)
+{
+    if (!buf || n == 0) return DHARA_E_ARG;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return DHARA_E_DEVICE;
+    std::snprintf(buf, n, "%s %s CUs=%d", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return DHARA_OK;
+}
+
+const char* dhara_last_error(const dhara_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx** out)
+{
+    if (!grid || !params || !out) return DHARA_E_ARG;
+    *out = nullptr;
+    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || grid->nroot_groups < 1 || !(grid->dx > 0) ||
+        !(grid->dy > 0) || !(grid->dz > 0))
+        return DHARA_E_ARG;
+    const double n3 = (double)grid->nx * grid->ny * grid->nz;
+    if (n3 >= 2147483647.0) return DHARA_E_ARG;
+    dhara_ctx* c = new dhara_ctx;
+    int rc = validate_params(c, params);
+    if (rc) { delete c; return rc; }
+    c->grid = *grid;
+    c->p = *params;
+    if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return DHARA_E_DEVICE;
+    }
+    c->N = grid->nx * grid->ny * grid->nz;
+    c->S = grid->nx * grid->ny;
+    dh::Geo& g = c->g;
+    g.nx = grid->nx; g.ny = grid->ny; g.nz = grid->nz; g.nxy = c->S; g.n = c->N;
+    g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
+    g.wx = 1.0 / (grid->dx * grid->dx);
+    g.wy = 1.0 / (grid->dy * grid->dy);
+    g.wz = 1.0 / (grid->dz * grid->dz);
+    dh::Geo2& g2 = c->g2;
+    g2.nx = grid->nx; g2.ny = grid->ny; g2.n = c->S; g2.dx = grid->dx; g2.dy = grid->dy;
+    g2.cx = params->dt / (grid->dx * grid->dx);
+    g2.cy = params->dt / (grid->dy * grid->dy);
+
+    const size_t N = c->N, S = c->S;
+    double** d3[] = {&c->psi_n, &c->psi_m, &c->x, &c->theta_n, &c->theta_m, &c->theta_new, &c->K,
+                     &c->C, &c->Ks, &c->alpha_f, &c->n_f, &c->D, &c->b, &c->r, &c->pv[0], &c->pv[1], &c->q};
+    for (double** pp : d3)
+        if ((rc = dalloc(c, pp, N))) { dhara_destroy(c); return rc; }
+    double** d2[] = {&c->z, &c->mann, &c->h, &c->H, &c->rain, &c->evap, &c->transp, &c->hpoten,
+                     &c->qcapa, &c->D2, &c->b2, &c->r2, &c->p2[0], &c->p2[1], &c->q2, &c->ke, &c->kn};
+    for (double** pp : d2)
+        if ((rc = dalloc(c, pp, S))) { dhara_destroy(c); return rc; }
+    for (int f = 0; f < 6; ++f) {
+        const size_t fs = face_size(*grid, f);
+        if ((rc = dalloc(c, &c->bcpsi[f], fs)) || (rc = dalloc(c, &c->bcq[f], fs)) ||
+            (rc = dalloc(c, &c->bctype[f], fs))) {
+            dhara_destroy(c);
+            return rc;
+        }
+        std::vector<int> ones(fs, DHARA_BC_FLUX);
+        if (hipMemcpy(c->bctype[f], ones.data(), fs * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            dhara_destroy(c);
+            return DHARA_E_DEVICE;
+        }
+    }
+    if ((rc = dalloc(c, &c->procmap, S)) || (rc = dalloc(c, &c->root, (size_t)grid->nroot_groups * grid->nz)) ||
+        (rc = dalloc(c, &c->flags, N)) || (rc = dalloc(c, &c->part, (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
+        (rc = dalloc(c, &c->cnt, 16)) || (rc = dalloc(c, &c->st_sub, 1)) || (rc = dalloc(c, &c->st_olf, 1)) ||
+        (rc = dalloc(c, &c->pst, 1)) || (rc = dalloc(c, &c->sums, 16))) {
+        dhara_destroy(c);
+        return rc;
+    }
+    {
+        std::vector<double> m(S, 0.03);
+        if (hipMemcpy(c->mann, m.data(), S * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            dhara_destroy(c);
+            return DHARA_E_DEVICE;
+        }
+    }
+    if (hipHostMalloc((void**)&c->h_st, sizeof(dh::PcgState)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_pst, sizeof(dh::PicardState)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_sums, 16 * sizeof(double)) != hipSuccess) {
+        dhara_destroy(c);
+        return DHARA_E_ALLOC;
+    }
+    if (c->p.precision == DHARA_MIXED && (rc = alloc_mixed(c))) { dhara_destroy(c); return rc; }
+    if (hipStreamSynchronize(c->s) != hipSuccess) { dhara_destroy(c); return DHARA_E_DEVICE; }
+    *out = c;
+    return DHARA_OK;
+}
+
+int dhara_destroy(dhara_ctx* c)
+{
+    if (!c) return DHARA_OK;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    for (void* v : c->allocs) (void)hipFree(v);
+    if (c->h_st) (void)hipHostFree(c->h_st);
+    if (c->h_pst) (void)hipHostFree(c->h_pst);
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+    return DHARA_OK;
+}
+
+int dhara_set_params(dhara_ctx* c, const dhara_params* p)
+{
+    if (!c || !p) return DHARA_E_ARG;
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    c->p = *p;
+    c->g2.cx = p->dt / (c->grid.dx * c->grid.dx);
+    c->g2.cy = p->dt / (c->grid.dy * c->grid.dy);
+    if (p->precision == DHARA_MIXED) return alloc_mixed(c);
+    return DHARA_OK;
+}
+
+int dhara_get_params(const dhara_ctx* c, dhara_params* p)
+{
+    if (!c || !p) return DHARA_E_ARG;
+    *p = c->p;
+    return DHARA_OK;
+}
+
+int dhara_set_precision(dhara_ctx* c, int32_t precision)
+{
+    if (!c) return DHARA_E_ARG;
+    dhara_params p = c->p;
+    p.precision = precision;
+    return dhara_set_params(c, &p);
+}
+
+int dhara_upload_field(dhara_ctx* c, int32_t f, const double* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    double* d = dfield(c, f, &want);
+    if (!d) return set_err(c, DHARA_E_ARG, "unknown double field");
+    if (n != want) return set_err(c, DHARA_E_ARG, "field size mismatch");
+    HIPCHK(c, hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    after_upload(c, f);
+    return DHARA_OK;
+}
+
+int dhara_upload_field_i32(dhara_ctx* c, int32_t f, const int32_t* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    int* d = ifield(c, f, &want);
+    if (!d) return set_err(c, DHARA_E_ARG, "unknown int field");
+    if (n != want) return set_err(c, DHARA_E_ARG, "field size mismatch");
+    if (f == DHARA_F_PROCMAP) {
+        for (size_t i = 0; i < n; ++i)
+            if (host[i] < 0 || host[i] >= c->grid.nroot_groups)
+                return set_err(c, DHARA_E_ARG, "procmap entry out of range");
+    }
+    if (f >= DHARA_F_BCTYPE_W && f <= DHARA_F_BCTYPE_B) {
+        bool any = false;
+        for (size_t i = 0; i < n; ++i) any |= host[i] == DHARA_BC_DIRICHLET;
+        c->face_dirichlet[f - DHARA_F_BCTYPE_W] = any;
+    }
+    HIPCHK(c, hipMemcpyAsync(d, host, n * sizeof(int32_t), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_download_field(dhara_ctx* c, int32_t f, double* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    double* d = dfield(c, f, &want);
+    if (!d) return set_err(c, DHARA_E_ARG, "unknown double field");
+    if (n != want) return set_err(c, DHARA_E_ARG, "field size mismatch");
+    if ((f == DHARA_F_ALPHA && !c->has_alpha) || (f == DHARA_F_NVG && !c->has_n)) {
+        std::fill(host, host + n, f == DHARA_F_ALPHA ? c->p.alpha : c->p.n_vg);
+        return DHARA_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_download_field_i32(dhara_ctx* c, int32_t f, int32_t* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    int* d = ifield(c, f, &want);
+    if (!d) return set_err(c, DHARA_E_ARG, "unknown int field");
+    if (n != want) return set_err(c, DHARA_E_ARG, "field size mismatch");
+    HIPCHK(c, hipMemcpyAsync(host, d, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_upload_field_device(dhara_ctx* c, int32_t f, const void* dev, size_t n)
+{
+    if (!c || !dev) return DHARA_E_ARG;
+    size_t want = 0;
+    if (double* d = dfield(c, f, &want)) {
+        if (n != want) return set_err(c, DHARA_E_ARG, "field size mismatch");
+        HIPCHK(c, hipMemcpyAsync(d, dev, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    } else if (int* di = ifield(c, f, &want)) {
+        if (n != want) return set_err(c, DHARA_E_ARG, "field size mismatch");
+        std::vector<int32_t> tmp(n);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), dev, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return dhara_upload_field_i32(c, f, tmp.data(), n);
+    } else {
+        return set_err(c, DHARA_E_ARG, "unknown field");
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    after_upload(c, f);
+    return DHARA_OK;
+}
+
+int dhara_download_field_device(dhara_ctx* c, int32_t f, void* dev, size_t n)
+{
+    if (!c || !dev) return DHARA_E_ARG;
+    size_t want = 0;
+    const void* src = nullptr;
+    size_t es = sizeof(double);
+    if (double* d = dfield(c, f, &want)) src = d;
+    else if (int* di = ifield(c, f, &want)) { src = di; es = sizeof(int32_t); }
+    else return set_err(c, DHARA_E_ARG, "unknown field");
+    if (n != want) return set_err(c, DHARA_E_ARG, "field size mismatch");
+    HIPCHK(c, hipMemcpyAsync(dev, src, n * es, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_init_state(dhara_ctx* c, int32_t mode)
+{
+    if (!c) return DHARA_E_ARG;
+    const dh::Soil soil = soil_of(c);
+    if (mode == DHARA_INIT_FROM_THETA) {
+        dh::launch_vg_inverse(c->s, c->g, soil, c->theta_n, c->psi_n);
+    } else if (mode != DHARA_INIT_FROM_PSI) {
+        return set_err(c, DHARA_E_ARG, "bad init mode");
+    }
+    dh::VgArgs a = vg_args(c, c->psi_n, c->theta_n);
+    dh::launch_vg(c->s, a);
+    LAUNCHCHK(c);
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->theta_m, c->theta_n, nb, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->theta_new, c->theta_n, nb, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->psi_m, c->psi_n, nb, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->x, c->psi_n, nb, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    c->theta_m_current = true;
+    c->initialized = true;
+    return DHARA_OK;
+}
+
+int dhara_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
+{
+    if (!c) return DHARA_E_ARG;
+    if (!c->initialized) return set_err(c, DHARA_E_STATE, "dhara_init_state not called after a state upload");
+    const auto t0 = std::chrono::steady_clock::now();
+    dhara_step_stats tmp;
+    std::memset(&tmp, 0, sizeof(tmp));
+    int rc = do_subsurface_step(c, &tmp);
+    tmp.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = tmp;
+    return rc;
+}
+
+int dhara_overland_step(dhara_ctx* c, dhara_step_stats* st)
+{
+    if (!c) return DHARA_E_ARG;
+    const auto t0 = std::chrono::steady_clock::now();
+    dhara_step_stats tmp;
+    std::memset(&tmp, 0, sizeof(tmp));
+    int rc = do_overland_step(c, &tmp);
+    tmp.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) {
+        st->olf_iters = tmp.olf_iters;
+        st->olf_converged = tmp.olf_converged;
+        st->outlet_discharge = tmp.outlet_discharge;
+        st->ring_outflow = tmp.ring_outflow;
+        st->nan_seen |= tmp.nan_seen;
+        st->wall_ms = tmp.wall_ms;
+    }
+    return rc;
+}
+
+int dhara_step(dhara_ctx* c, int32_t nsteps, dhara_step_stats* st)
+{
+    if (!c || nsteps < 0) return DHARA_E_ARG;
+    if (!c->initialized) return set_err(c, DHARA_E_STATE, "dhara_init_state not called after a state upload");
+    for (int s = 0; s < nsteps; ++s) {
+        const auto t0 = std::chrono::steady_clock::now();
+        dhara_step_stats tmp;
+        std::memset(&tmp, 0, sizeof(tmp));
+        int rc = do_subsurface_step(c, &tmp);
+        if (rc == DHARA_OK && c->p.overland) rc = do_overland_step(c, &tmp);
+        tmp.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (st) st[s] = tmp;
+        if (rc) return rc;
+    }
+    return DHARA_OK;
+}
+
+int dhara_eval_constitutive(dhara_ctx* c, const double* psi, double* theta, double* K, double* C)
+{
+    if (!c || !psi || !theta || !K || !C) return DHARA_E_ARG;
+    const size_t nb = (size_t)c->N * sizeof(double);
+    // scratch: p[0] <- psi, outputs into D, b, r
+    HIPCHK(c, hipMemcpyAsync(c->pv[0], psi, nb, hipMemcpyHostToDevice, c->s));
+    dh::VgArgs a = vg_args(c, c->pv[0], c->D);
+    a.K = c->b;
+    a.C = c->r;
+    a.K32 = nullptr;
+    dh::launch_vg(c->s, a);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(theta, c->D, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipMemcpyAsync(K, c->b, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipMemcpyAsync(C, c->r, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_eval_inverse(dhara_ctx* c, const double* theta, double* psi)
+{
+    if (!c || !theta || !psi) return DHARA_E_ARG;
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->pv[0], theta, nb, hipMemcpyHostToDevice, c->s));
+    dh::launch_vg_inverse(c->s, c->g, soil_of(c), c->pv[0], c->pv[1]);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(psi, c->pv[1], nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_assemble_subsurface(dhara_ctx* c, double* diag, double* rhs)
+{
+    if (!c) return DHARA_E_ARG;
+    if (!c->initialized) return set_err(c, DHARA_E_STATE, "dhara_init_state not called");
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->psi_m, c->psi_n, nb, hipMemcpyDeviceToDevice, c->s));
+    if (c->p.top_bc == DHARA_TOP_SURFACE) dh::launch_surface_pre(c->s, surf_args(c));
+    const bool dir = has_dirichlet(c);
+    if (dir) {
+        const int* bt[6];
+        for (int f = 0; f < 6; ++f) bt[f] = c->bctype[f];
+        dh::launch_row_flags(c->s, c->g, bt, c->flags);
+    }
+    dh::launch_vg(c->s, vg_args(c, c->psi_m, c->theta_m));
+    c->theta_m_current = true;
+    dh::launch_assemble(c->s, asm_args(c, dir), false);
+    LAUNCHCHK(c);
+    if (diag) HIPCHK(c, hipMemcpyAsync(diag, c->D, nb, hipMemcpyDeviceToHost, c->s));
+    if (rhs) HIPCHK(c, hipMemcpyAsync(rhs, c->b, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_apply_subsurface_operator(dhara_ctx* c, const double* xin, double* y)
+{
+    if (!c || !xin || !y) return DHARA_E_ARG;
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->pv[0], xin, nb, hipMemcpyHostToDevice, c->s));
+    dh::launch_apply3_f64(c->s, c->g, has_dirichlet(c) ? c->flags : nullptr, c->K, c->D, c->pv[0], c->pv[1]);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(y, c->pv[1], nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_apply_subsurface_operator_f32(dhara_ctx* c, const double* xin, double* y)
+{
+    if (!c || !xin || !y) return DHARA_E_ARG;
+    int rc = alloc_mixed(c);
+    if (rc) return rc;
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->pv[0], xin, nb, hipMemcpyHostToDevice, c->s));
+    dh::launch_cast_f32(c->s, c->N, c->K, c->K32);
+    dh::launch_cast_f32(c->s, c->N, c->D, c->D32);
+    dh::launch_cast_f32(c->s, c->N, c->pv[0], c->p32[0]);
+    dh::launch_apply3_f32(c->s, c->g, has_dirichlet(c) ? c->flags : nullptr, c->K32, c->D32, c->p32[0],
+                          c->p32[1]);
+    dh::launch_cast_f64(c->s, c->N, c->p32[1], c->pv[1]);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(y, c->pv[1], nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_overland_conductance(dhara_ctx* c, double* ke, double* kn)
+{
+    if (!c || !ke || !kn) return DHARA_E_ARG;
+    dh::launch_olf_faces(c->s, olf_args(c));
+    LAUNCHCHK(c);
+    const size_t nb = (size_t)c->S * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(ke, c->ke, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipMemcpyAsync(kn, c->kn, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// kernels.h -- argument blocks and host launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.h"
+
+namespace dh {
+
+enum Face { FW = 0, FE = 1, FS = 2, FN = 3, FT = 4, FB = 5 };
+
+// Row flags of the subsurface operator (only built when a Dirichlet face exists):
+// identity row, or an off-diagonal coupling removed (src/subsurface.cu:144-352).
+enum RowFlag : uint8_t {
+    RF_IDENTITY = 1, RF_CUT_W = 2, RF_CUT_E = 4, RF_CUT_S = 8, RF_CUT_N = 16,
+    RF_CUT_D = 32,   // coupling to k-1 (toward the top)
+    RF_CUT_U = 64,   // coupling to k+1 (toward the bottom)
+};
+
+struct Geo {
+    int nx, ny, nz, nxy, n;
+    double dx, dy, dz;
+    double wx, wy, wz;    // 1/dx^2, 1/dy^2, 1/dz^2
+};
+
+struct Geo2 {
+    int nx, ny, n;
+    double dx, dy;
+    double cx, cy;        // dt/dx^2, dt/dy^2
+};
+
+struct Reduce {
+    double* part;         // kMaxSums * kMaxBlocks partials
+    unsigned* cnt;        // arrival counter (one per concurrently live reduction)
+};
+
+struct Soil {
+    const double* Ks;
+    const double* alpha_f;   // nullptr: scalar alpha
+    const double* n_f;       // nullptr: scalar n
+    double alpha, n, ts, tr;
+};
+
+struct VgArgs {
+    Geo g;
+    Soil soil;
+    const double* psi;       // evaluated head (x for the Picard pass)
+    double* psi_m;           // Picard pass: read for the increment, then psi^m <- am psi^m + (1-am) x
+    const double* theta_m;   // REL_THETA rule
+    double* theta;           // out
+    double* K;               // out
+    double* C;               // out
+    float* K32;              // out (mixed precision), may be null
+    int rule;
+    double am;
+    PicardState* pst;
+    Reduce red;
+};
+
+struct AsmArgs {
+    Geo g;
+    double dt, Ss, poros;
+    const double *K, *C, *theta_m, *theta_n, *psi_n, *psi_m, *x;
+    const double *transp, *root;
+    const int* procmap;
+    int has_transp;
+    const int* bctype[6];
+    const double* bcpsi[6];
+    double* bcq[6];          // bcq[FB] is written (free drainage flux = K of the cell)
+    const uint8_t* flags;
+    int quirk;
+    double* D;
+    double* b;
+    double* r;               // FP64 residual (FP64 path)
+    float* D32;              // mixed path
+    float* r32;
+    float* d32;
+    double lin_tol, inner_tol;
+    int maxit, inner_maxit;
+    PcgState* st;
+    Reduce red;
+};
+
+struct RefineArgs {
+    Geo g;
+    const double* K;
+    const double* D;
+    const double* b;
+    double* x;
+    const uint8_t* flags;
+    float* r32;
+    const float* D32;
+    float* d32;
+    double lin_tol, inner_tol;
+    PcgState* st;
+    Reduce red;
+};
+
+struct SurfArgs {
+    int nxy;
+    double dt, dz, ts;
+    const double *K, *psi_n, *theta_n, *Ks;   // top layer = first nxy cells
+    double *h, *hpoten, *qcapa;
+    const double *rain, *evap;
+    double* qtop;                              // bcq[FT]
+    int* toptype;                              // bctype[FT]
+};
+
+struct EndArgs {
+    Geo g;
+    double dt, Ss, poros;
+    const double *x, *theta_new, *theta_m;
+    double *psi_n, *theta_n;
+    const double *transp, *root;
+    const int* procmap;
+    int has_transp;
+    int carry_reference;
+    const int* bctype[6];
+    const double* bcq[6];
+    Reduce red;
+    double* out;             // 9 sums
+};
+
+struct OlfArgs {
+    Geo2 g;
+    double hmin, delta, K0;
+    double* h;
+    const double *z, *mann;
+    double* H;
+    double *ke, *kn;
+    double *D, *b, *r;
+    double tol;
+    int maxit;
+    PcgState* st;
+    Reduce red;
+    double* out;             // outlet sums
+};
+
+// ---- launchers (all asynchronous on `s`) ----------------------------------------------
+void launch_vg(hipStream_t s, const VgArgs& a);                 // theta, K, C of psi
+void launch_vg_picard(hipStream_t s, const VgArgs& a);          // + increment + relax
+void launch_vg_inverse(hipStream_t s, const Geo& g, const Soil& soil, const double* theta,
+                       double* psi);
+void launch_row_flags(hipStream_t s, const Geo& g, const int* const bctype[6], uint8_t* flags);
+void launch_surface_pre(hipStream_t s, const SurfArgs& a);
+void launch_surface_post(hipStream_t s, const SurfArgs& a);
+void launch_assemble(hipStream_t s, const AsmArgs& a, bool mixed);
+void launch_refine(hipStream_t s, const RefineArgs& a);
+void launch_end_step(hipStream_t s, const EndArgs& a);
+
+// One Jacobi-PCG iteration on the 3D operator (spmv+dot, then update+dots).
+// p[0]/p[1] ping-pong; `parity` selects which one is read.
+void launch_pcg3_iter_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
+                          const double* D, double* x, double* r, double* const p[2], double* q,
+                          int parity, PcgState* st, Reduce red);
+void launch_pcg3_iter_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
+                          const float* D, float* x, float* r, float* const p[2], float* q,
+                          int parity, PcgState* st, Reduce red);
+// y = A x (tests)
+void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
+                       const double* D, const double* x, double* y);
+void launch_apply3_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
+                       const float* D, const float* x, float* y);
+void launch_cast_f32(hipStream_t s, int n, const double* in, float* out);
+void launch_cast_f64(hipStream_t s, int n, const float* in, double* out);
+
+// Overland
+void launch_olf_faces(hipStream_t s, const OlfArgs& a);
+void launch_olf_assemble(hipStream_t s, const OlfArgs& a);
+void launch_pcg2_iter(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
+                      const double* D, double* x, double* r, double* const p[2], double* q,
+                      int parity, PcgState* st, Reduce red);
+void launch_olf_post(hipStream_t s, const OlfArgs& a);
+
+}  // namespace dh

@@ -1,0 +1,839 @@
+// kernels.hip -- gfx950 kernels of the subsurface (3D Richards) and overland (2D diffusive
+// wave) solvers.  Every kernel is grid-stride over cells in the reference layout
+// (k*nx*ny + j*nx + i, k = 0 top) and keeps per-cell state in HBM as SoA arrays.
+//
+// Reductions use dh::grid_reduce (last-block-done, fixed order).  The Krylov kernels read
+// PcgState::done at entry and return at once when the solve has finished, so the host can
+// queue iterations in batches without a round trip per iteration.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace dh {
+
+namespace {
+
+inline int grid_for(int n, int per_block = kBlock, int cap = 2048)
+{
+    int g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    return g < cap ? g : cap;
+}
+
+__device__ __forceinline__ void decompose(const Geo& g, int c, int& i, int& j, int& k)
+{
+    k = c / g.nxy;
+    const int rem = c - k * g.nxy;
+    j = rem / g.nx;
+    i = rem - j * g.nx;
+}
+
+__device__ __forceinline__ double cell_alpha(const Soil& s, int c) { return s.alpha_f ? s.alpha_f[c] : s.alpha; }
+__device__ __forceinline__ double cell_n(const Soil& s, int c) { return s.n_f ? s.n_f[c] : s.n; }
+
+// y_c = (A p)_c of the subsurface operator: diagonal D plus the 7-point couplings
+// -K_f/h^2 with arithmetic-mean faces K_f = (K_c + K_nb)/2 (src/subsurface.cu:26-102);
+// domain faces carry no coupling; Dirichlet rows/cuts come from the row flags.
+template <typename T, bool DIR, class PF>
+__device__ __forceinline__ T op3_row(const Geo& g, int c, int i, int j, int k, T Dc, T pc, PF pf,
+                                     const T* __restrict__ K, unsigned fl)
+{
+    if (DIR && (fl & RF_IDENTITY)) return pc;
+    const T Kc = K[c];
+    const T h = T(0.5);
+    const T wx = (T)g.wx, wy = (T)g.wy, wz = (T)g.wz;
+    T acc = Dc * pc;
+    if (k > 0 && !(DIR && (fl & RF_CUT_D))) acc -= wz * (h * (Kc + K[c - g.nxy])) * pf(c - g.nxy);
+    if (j > 0 && !(DIR && (fl & RF_CUT_S))) acc -= wy * (h * (Kc + K[c - g.nx])) * pf(c - g.nx);
+    if (i > 0 && !(DIR && (fl & RF_CUT_W))) acc -= wx * (h * (Kc + K[c - 1])) * pf(c - 1);
+    if (i < g.nx - 1 && !(DIR && (fl & RF_CUT_E))) acc -= wx * (h * (Kc + K[c + 1])) * pf(c + 1);
+    if (j < g.ny - 1 && !(DIR && (fl & RF_CUT_N))) acc -= wy * (h * (Kc + K[c + g.nx])) * pf(c + g.nx);
+    if (k < g.nz - 1 && !(DIR && (fl & RF_CUT_U))) acc -= wz * (h * (Kc + K[c + g.nxy])) * pf(c + g.nxy);
+    return acc;
+}
+
+// Overland operator after elimination of the Dirichlet ring: ring rows are identity,
+// interior rows couple only to interior neighbours through face conductances
+// ke (east face of a cell) and kn (north face) scaled by dt/dx^2, dt/dy^2
+// (src/overland.cu:65-189).
+template <class PF>
+__device__ __forceinline__ double op2_row(const Geo2& g, int c, double Dc, double pc, PF pf,
+                                          const double* __restrict__ ke,
+                                          const double* __restrict__ kn)
+{
+    const int j = c / g.nx, i = c - j * g.nx;
+    if (i == 0 || j == 0 || i == g.nx - 1 || j == g.ny - 1) return pc;
+    double acc = Dc * pc;
+    if (i > 1) acc -= g.cx * ke[c - 1] * pf(c - 1);
+    if (i < g.nx - 2) acc -= g.cx * ke[c] * pf(c + 1);
+    if (j > 1) acc -= g.cy * kn[c - g.nx] * pf(c - g.nx);
+    if (j < g.ny - 2) acc -= g.cy * kn[c] * pf(c + g.nx);
+    return acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// Constitutive kernels
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_vg(VgArgs a)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.g.n; c += gridDim.x * blockDim.x) {
+        double th, K, C;
+        vg_eval(a.psi[c], a.soil.Ks[c], cell_alpha(a.soil, c), cell_n(a.soil, c), a.soil.ts,
+                a.soil.tr, th, K, C);
+        a.theta[c] = th;
+        a.K[c] = K;
+        a.C[c] = C;
+        if (a.K32) a.K32[c] = (float)K;
+    }
+}
+
+// vanGenuchten on psi^{m+1} (src/subsurface.cu:754) fused with GetIterationDifference
+// (432-446), the max reduction (777) and ModifiedPicardUpdate (450-461).
+__global__ __launch_bounds__(kBlock) void k_vg_picard(VgArgs a)
+{
+    double mx[2] = {0.0, 0.0};   // increment, NaN flag
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.g.n; c += gridDim.x * blockDim.x) {
+        const double xc = a.psi[c];
+        const double pm = a.psi_m[c];
+        double th, K, C;
+        vg_eval(xc, a.soil.Ks[c], cell_alpha(a.soil, c), cell_n(a.soil, c), a.soil.ts, a.soil.tr,
+                th, K, C);
+        double d;
+        if (a.rule == 1) d = fabs((xc - pm) / xc);
+        else if (a.rule == 2) d = fabs((th - a.theta_m[c]) / th);
+        else d = fabs(xc - pm);
+        if (!(d == d) || !isfinite(xc)) mx[1] = 1.0;
+        else mx[0] = fmax(mx[0], d);
+        a.theta[c] = th;
+        a.K[c] = K;
+        a.C[c] = C;
+        if (a.K32) a.K32[c] = (float)K;
+        a.psi_m[c] = a.am * pm + (1.0 - a.am) * xc;
+    }
+    double out[2];
+    if (grid_reduce<2, true>(mx, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
+        a.pst->delta = out[0];
+        a.pst->nan = out[1] > 0.0 ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_vg_inverse(Geo g, Soil s, const double* theta, double* psi)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x)
+        psi[c] = vg_inverse(theta[c], cell_alpha(s, c), cell_n(s, c), s.ts, s.tr);
+}
+
+// ---------------------------------------------------------------------------------------
+// Boundary handling
+// ---------------------------------------------------------------------------------------
+struct BcTypes { const int* t[6]; };
+
+__global__ __launch_bounds__(kBlock) void k_row_flags(Geo g, BcTypes bt, uint8_t* flags)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        decompose(g, c, i, j, k);
+        const int bi = k * g.ny + j, bj = k * g.nx + i, bk = j * g.nx + i;
+        unsigned f = 0;
+        if (bt.t[FT][bk] == 0) { if (k == 1) f |= RF_CUT_D; if (k == 0) f |= RF_IDENTITY; }
+        if (bt.t[FB][bk] == 0) { if (k == g.nz - 2) f |= RF_CUT_U; if (k == g.nz - 1) f |= RF_IDENTITY; }
+        if (bt.t[FS][bj] == 0) { if (j == 1) f |= RF_CUT_S; if (j == 0) f |= RF_IDENTITY; }
+        if (bt.t[FN][bj] == 0) { if (j == g.ny - 2) f |= RF_CUT_N; if (j == g.ny - 1) f |= RF_IDENTITY; }
+        if (bt.t[FW][bi] == 0) { if (i == 1) f |= RF_CUT_W; if (i == 0) f |= RF_IDENTITY; }
+        if (bt.t[FE][bi] == 0) { if (i == g.nx - 2) f |= RF_CUT_E; if (i == g.nx - 1) f |= RF_IDENTITY; }
+        flags[c] = (uint8_t)f;
+    }
+}
+
+// EstimateFluxes (src/global.cu:172-191) + IdentifyTopBoundary (206-257).  Both only use
+// quantities fixed within a step, so the top flux is computed once per step.
+__global__ __launch_bounds__(kBlock) void k_surface_pre(SurfArgs a)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nxy; t += gridDim.x * blockDim.x) {
+        const double ph = a.h[t];
+        const double hp = ph + a.rain[t] * a.dt - a.evap[t] * a.dt;
+        double qc = -a.K[t] * ((a.psi_n[t] - hp - 0.5 * a.dz) / (0.5 * a.dz));
+        if (ph / a.dt > a.Ks[t]) qc = fmax(qc, a.Ks[t]);
+        double q;
+        if (hp > 0.0) {
+            q = (hp / a.dt > qc) ? qc : hp / a.dt;
+            q = fmin(q, (a.ts - a.theta_n[t]) * a.dz / a.dt);
+            q = fmin(q, a.Ks[t]);
+        } else {
+            q = hp / a.dt;
+        }
+        a.hpoten[t] = hp;
+        a.qcapa[t] = qc;
+        a.qtop[t] = q;
+        a.toptype[t] = 1;
+    }
+}
+
+// SubsurfaceEstimateInfiltrationPonding, flux branch (src/subsurface.cu:474-481).
+__global__ __launch_bounds__(kBlock) void k_surface_post(SurfArgs a)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nxy; t += gridDim.x * blockDim.x)
+        a.h[t] = fmax(a.hpoten[t] - a.qtop[t] * a.dt, 0.0);
+}
+
+// ---------------------------------------------------------------------------------------
+// Matrix-free assembly: diagonal D and right-hand side b of the mixed-form modified Picard
+// system (src/subsurface.cu:381-429 with 26-353), fused with the initial residual of the
+// warm-started solve r0 = b - A x0 and its dot products.
+// ---------------------------------------------------------------------------------------
+template <bool DIR, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
+{
+    const Geo g = a.g;
+    const double dx = g.dx, dy = g.dy, dz = g.dz, dt = a.dt;
+    const double dx2inv = -1.0 / (dx * dx), dy2inv = -1.0 / (dy * dy), dz2inv = -1.0 / (dz * dz);
+    const double* __restrict__ K = a.K;
+    const double* __restrict__ x = a.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};   // bb, rr(64), rz(solver precision), rr(solver)
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        decompose(g, c, i, j, k);
+        const double Kc = K[c];
+        const double kd = (k == 0) ? 0.0 : 0.5 * (Kc + K[c - g.nxy]);
+        const double ku = (k == g.nz - 1) ? 0.0 : 0.5 * (Kc + K[c + g.nxy]);
+        const double ks = (j == 0) ? 0.0 : 0.5 * (Kc + K[c - g.nx]);
+        const double kn = (j == g.ny - 1) ? 0.0 : 0.5 * (Kc + K[c + g.nx]);
+        const double kw = (i == 0) ? 0.0 : 0.5 * (Kc + K[c - 1]);
+        const double ke = (i == g.nx - 1) ? 0.0 : 0.5 * (Kc + K[c + 1]);
+        const double thm = a.theta_m[c];
+        const double Cc = a.C[c];
+        double D = a.Ss / dt * thm / a.poros + Cc / dt - (ke + kw) * dx2inv - (kn + ks) * dy2inv -
+                   (ku + kd) * dz2inv;
+        const int id2d = j * g.nx + i;
+        double sink = 0.0;
+        if (a.has_transp) {
+            const double tr = a.transp[id2d];
+            if (tr > 0.0) sink = -tr * a.root[a.procmap[id2d] * g.nz + k] / dz;
+        }
+        double b = a.Ss / dt * thm / a.poros * a.psi_n[c] + Cc / dt * a.psi_m[c] - (ku - kd) / dz +
+                   (a.theta_n[c] - thm) / dt + sink;
+        const int bi = k * g.ny + j, bj = k * g.nx + i, bk = j * g.nx + i;
+        bool ident = false;
+        // top
+        if (DIR && a.bctype[FT][bk] == 0) {
+            if (k == 1) b -= dz2inv * kd * a.bcpsi[FT][bk];
+            if (k == 0) { ident = true; b = a.bcpsi[FT][bk]; }
+        } else if (k == 0) {
+            b += (-a.bcq[FT][bk] * dz) * dz2inv;
+        }
+        // bottom: flux type means free drainage, qb = K of the cell
+        if (DIR && a.bctype[FB][bk] == 0) {
+            if (k == g.nz - 2) b -= dz2inv * ku * a.bcpsi[FB][bk];
+            if (k == g.nz - 1) { ident = true; b = a.bcpsi[FB][bk]; }
+        } else if (k == g.nz - 1) {
+            a.bcq[FB][bk] = Kc;
+            b += (Kc * dz) * dz2inv;
+        }
+        // south
+        if (DIR && a.bctype[FS][bj] == 0) {
+            if (j == 1) b -= dy2inv * (a.quirk ? kn : ks) * a.bcpsi[FS][bj];
+            if (j == 0) { ident = true; b = a.bcpsi[FS][bj]; }
+        } else if (j == 0) {
+            b += (-a.bcq[FS][bj] * dy) * dy2inv;
+        }
+        // north
+        if (DIR && a.bctype[FN][bj] == 0) {
+            if (j == g.ny - 2) b -= dy2inv * (a.quirk ? ks : kn) * a.bcpsi[FN][bj];
+            if (j == g.ny - 1) { ident = true; b = a.bcpsi[FN][bj]; }
+        } else if (j == g.ny - 1) {
+            b += (a.bcq[FN][bj] * dy) * dy2inv;
+        }
+        // west
+        if (DIR && a.bctype[FW][bi] == 0) {
+            if (i == 1) b -= dx2inv * (a.quirk ? ke : kw) * a.bcpsi[FW][bi];
+            if (i == 0) { ident = true; b = a.bcpsi[FW][bi]; }
+        } else if (i == 0) {
+            b += (-a.bcq[FW][bi] * dx) * dx2inv;
+        }
+        // east
+        if (DIR && a.bctype[FE][bi] == 0) {
+            if (i == g.nx - 2) b -= dx2inv * (a.quirk ? kw : ke) * a.bcpsi[FE][bi];
+            if (i == g.nx - 1) { ident = true; b = a.bcpsi[FE][bi]; }
+        } else if (i == g.nx - 1) {
+            b += (a.bcq[FE][bi] * dx) * dx2inv;
+        }
+        if (ident) D = 1.0;
+        a.D[c] = D;
+        a.b[c] = b;
+        // r0 = b - A x0
+        const unsigned fl = DIR ? a.flags[c] : 0u;
+        double Ax;
+        if (DIR && (fl & RF_IDENTITY)) {
+            Ax = x[c];
+        } else {
+            Ax = D * x[c];
+            if (k > 0 && !(DIR && (fl & RF_CUT_D))) Ax += dz2inv * kd * x[c - g.nxy];
+            if (j > 0 && !(DIR && (fl & RF_CUT_S))) Ax += dy2inv * ks * x[c - g.nx];
+            if (i > 0 && !(DIR && (fl & RF_CUT_W))) Ax += dx2inv * kw * x[c - 1];
+            if (i < g.nx - 1 && !(DIR && (fl & RF_CUT_E))) Ax += dx2inv * ke * x[c + 1];
+            if (j < g.ny - 1 && !(DIR && (fl & RF_CUT_N))) Ax += dy2inv * kn * x[c + g.nx];
+            if (k < g.nz - 1 && !(DIR && (fl & RF_CUT_U))) Ax += dz2inv * ku * x[c + g.nxy];
+        }
+        const double r = b - Ax;
+        acc[0] += b * b;
+        acc[1] += r * r;
+        if (MIXED) {
+            const float r32 = (float)r, D32 = (float)D;
+            a.D32[c] = D32;
+            a.r32[c] = r32;
+            a.d32[c] = 0.0f;
+            const float z32 = r32 / D32;
+            acc[2] += (double)r32 * (double)z32;
+            acc[3] += (double)r32 * (double)r32;
+        } else {
+            a.r[c] = r;
+            acc[2] += r * (r / D);
+            acc[3] += r * r;
+        }
+    }
+    double out[4];
+    if (grid_reduce<4, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
+        PcgState* st = a.st;
+        st->bb = out[0];
+        st->rr_outer = out[1];
+        st->rz = out[2];
+        st->rr = out[3];
+        st->iter = 0;
+        st->total = 0;
+        st->beta = 0.0;
+        st->alpha = 0.0;
+        st->nan = (isfinite(out[0]) && isfinite(out[1])) ? 0 : 1;
+        const double tol2_outer = a.lin_tol * a.lin_tol * out[0];
+        st->outer_conv = (out[1] <= tol2_outer) ? 1 : 0;
+        if (MIXED) {
+            st->tol2 = a.inner_tol * a.inner_tol * out[3];
+            st->maxit = a.inner_maxit;
+            st->conv = (out[3] <= st->tol2) ? 1 : 0;
+            st->done = (st->outer_conv || st->conv || st->nan) ? 1 : 0;
+        } else {
+            st->tol2 = tol2_outer;
+            st->maxit = a.maxit;
+            st->conv = st->outer_conv;
+            st->done = (st->conv || st->nan || a.maxit <= 0) ? 1 : 0;
+        }
+    }
+}
+
+// FP64 iterative refinement: x <- x + d (pointwise), then r = b - A x in FP64 with the
+// FP32 copy of r as the next inner right-hand side.
+__global__ __launch_bounds__(kBlock) void k_refine_add(int n, double* x, const float* d)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        x[c] += (double)d[c];
+}
+
+template <bool DIR>
+__global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
+{
+    const Geo g = a.g;
+    const double* __restrict__ x = a.x;
+    double acc[3] = {0.0, 0.0, 0.0};   // rr64, rz32, rr32
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        decompose(g, c, i, j, k);
+        const unsigned fl = DIR ? a.flags[c] : 0u;
+        const double Dc = a.D[c];
+        auto pf = [&](int cc) { return x[cc]; };
+        const double Ax = op3_row<double, DIR>(g, c, i, j, k, Dc, x[c], pf, a.K, fl);
+        const double r = a.b[c] - Ax;
+        const float r32 = (float)r;
+        a.r32[c] = r32;
+        a.d32[c] = 0.0f;
+        const float z32 = r32 / a.D32[c];
+        acc[0] += r * r;
+        acc[1] += (double)r32 * (double)z32;
+        acc[2] += (double)r32 * (double)r32;
+    }
+    double out[3];
+    if (grid_reduce<3, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
+        PcgState* st = a.st;
+        st->rr_outer = out[0];
+        st->outer_conv = (out[0] <= a.lin_tol * a.lin_tol * st->bb) ? 1 : 0;
+        st->rz = out[1];
+        st->rr = out[2];
+        st->iter = 0;
+        st->beta = 0.0;
+        st->tol2 = a.inner_tol * a.inner_tol * out[2];
+        st->conv = (out[2] <= st->tol2) ? 1 : 0;
+        st->nan = isfinite(out[0]) ? 0 : 1;
+        st->done = (st->outer_conv || st->conv || st->nan) ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Jacobi-PCG.  Iteration i:
+//   (1) p = M^-1 r + beta p (beta = 0 on the first iteration), q = A p, pq = (p, q)
+//   (2) alpha = rz / pq, x += alpha p, r -= alpha q, rz' = (r, M^-1 r), rr = (r, r)
+// Stopping: ||r||_2 <= tol ||b||_2, checked before the first and after every iteration
+// (the order of cusp::monitor, cusp/detail/monitor.inl:178-200).
+// ---------------------------------------------------------------------------------------
+template <typename T, bool DIR>
+__global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __restrict__ flags,
+                                                      const T* __restrict__ K, const T* __restrict__ D,
+                                                      const T* __restrict__ r, const T* __restrict__ pold,
+                                                      T* __restrict__ pnew, T* __restrict__ q,
+                                                      PcgState* st, Reduce red)
+{
+    if (st->done) return;
+    const bool first = (st->iter == 0);
+    const T beta = (T)st->beta;
+    auto pf = [&](int c) -> T {
+        const T z = r[c] / D[c];
+        return first ? z : z + beta * pold[c];
+    };
+    double acc[1] = {0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        decompose(g, c, i, j, k);
+        const T pc = pf(c);
+        pnew[c] = pc;
+        const unsigned fl = DIR ? flags[c] : 0u;
+        const T qc = op3_row<T, DIR>(g, c, i, j, k, D[c], pc, pf, K, fl);
+        q[c] = qc;
+        acc[0] += (double)pc * (double)qc;
+    }
+    double out[1];
+    if (grid_reduce<1, false>(acc, red.part, red.cnt, out) && lin_tid() == 0) {
+        st->pq = out[0];
+        const double al = st->rz / out[0];
+        if (!(out[0] != 0.0) || !isfinite(al)) {
+            st->done = 1;
+            st->nan = isfinite(out[0]) ? st->nan : 1;
+            st->alpha = 0.0;
+        } else {
+            st->alpha = al;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x, T* __restrict__ r,
+                                                       const T* __restrict__ D, const T* __restrict__ p,
+                                                       const T* __restrict__ q, PcgState* st, Reduce red)
+{
+    if (st->done) return;
+    const T alpha = (T)st->alpha;
+    double acc[2] = {0.0, 0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const T xc = x[c] + alpha * p[c];
+        const T rc = r[c] - alpha * q[c];
+        x[c] = xc;
+        r[c] = rc;
+        const T zc = rc / D[c];
+        acc[0] += (double)rc * (double)zc;
+        acc[1] += (double)rc * (double)rc;
+    }
+    double out[2];
+    if (grid_reduce<2, false>(acc, red.part, red.cnt, out) && lin_tid() == 0) {
+        const int it = st->iter + 1;
+        const double rz_old = st->rz;
+        st->rz = out[0];
+        st->rr = out[1];
+        st->beta = out[0] / rz_old;
+        st->iter = it;
+        st->total = st->total + 1;
+        const int conv = (out[1] <= st->tol2) ? 1 : 0;
+        const int nan = isfinite(out[1]) ? 0 : 1;
+        st->conv = conv;
+        st->nan = st->nan | nan;
+        st->done = (conv || nan || it >= st->maxit) ? 1 : 0;
+    }
+}
+
+template <typename T, bool DIR>
+__global__ __launch_bounds__(kBlock) void k_apply3(Geo g, const uint8_t* flags, const T* K,
+                                                   const T* D, const T* x, T* y)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        decompose(g, c, i, j, k);
+        auto pf = [&](int cc) { return x[cc]; };
+        y[c] = op3_row<T, DIR>(g, c, i, j, k, D[c], x[c], pf, K, DIR ? flags[c] : 0u);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cast_f32(int n, const double* in, float* out)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        out[c] = (float)in[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cast_f64(int n, const float* in, double* out)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        out[c] = (double)in[c];
+}
+
+// ---------------------------------------------------------------------------------------
+// End of step: mass-balance sums (src/subsurface.cu:494-629, 808-826) and the state copies
+// psi^n <- psi^{n+1}, theta^n <- theta carry (836-840).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_end_step(EndArgs a)
+{
+    const Geo g = a.g;
+    // 0 sum dtheta, 1 sum Ss th/phi dpsi dz, 2 sink volume/area, 3 qt, 4 qb, 5 qw, 6 qe, 7 qs, 8 qn
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        decompose(g, c, i, j, k);
+        const double thn = a.theta_new[c];
+        const double xc = a.x[c];
+        acc[0] += thn - a.theta_n[c];
+        acc[1] += (xc - a.psi_n[c]) * (a.Ss / a.dt * thn / a.poros) * g.dz * a.dt;
+        const int id2d = j * g.nx + i;
+        if (a.has_transp) {
+            const double tr = a.transp[id2d];
+            if (tr > 0.0) acc[2] += tr * a.dt * a.root[a.procmap[id2d] * g.nz + k];
+        }
+        const int bi = k * g.ny + j, bj = k * g.nx + i, bk = id2d;
+        if (k == 0 && a.bctype[FT][bk] != 0) acc[3] += a.bcq[FT][bk];
+        if (k == g.nz - 1 && a.bctype[FB][bk] != 0) acc[4] += a.bcq[FB][bk];
+        if (i == 0 && a.bctype[FW][bi] != 0) acc[5] += a.bcq[FW][bi];
+        if (i == g.nx - 1 && a.bctype[FE][bi] != 0) acc[6] += a.bcq[FE][bi];
+        if (j == 0 && a.bctype[FS][bj] != 0) acc[7] += a.bcq[FS][bj];
+        if (j == g.ny - 1 && a.bctype[FN][bj] != 0) acc[8] += a.bcq[FN][bj];
+        a.psi_n[c] = xc;
+        a.theta_n[c] = a.carry_reference ? a.theta_m[c] : thn;
+    }
+    double out[9];
+    if (grid_reduce<9, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
+        for (int v = 0; v < 9; ++v) a.out[v] = out[v];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Overland
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ bool is_ring(const Geo2& g, int i, int j)
+{
+    return i == 0 || j == 0 || i == g.nx - 1 || j == g.ny - 1;
+}
+
+// Face conductance of the diffusive-wave model (UpdateKOverland, src/overland.cu:193-368):
+// S = sqrt(gx^2 + gy^2), K = h_avg^(5/3) / (n_avg sqrt(S)) when the upwind depth, the mean
+// depth exceed hmin and |S| > delta, K0 otherwise.  `up_nb` selects the neighbour as the
+// upwind cell.
+__device__ __forceinline__ double olf_face_k(double gdir, double gcross, double h_nb, double h_c,
+                                             double m_nb, double m_c, bool up_nb, double hmin,
+                                             double delta, double K0)
+{
+    const double Ssl = sqrt(gdir * gdir + gcross * gcross);
+    const double hup = up_nb ? h_nb : h_c;
+    const double hav = (h_nb + h_c) / 2.;
+    if (hup > hmin && hav > hmin && fabs(Ssl) > delta)
+        return pow(hav, 5. / 3.) / (0.5 * (m_nb + m_c) * sqrt(Ssl));
+    return K0;
+}
+
+// H = h + z, then the east and north face conductances of every face the eliminated
+// system uses, each evaluated from its non-ring cell (the row that owns the coupling in
+// the reference matrix).
+__global__ __launch_bounds__(kBlock) void k_olf_H(OlfArgs a)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.g.n; c += gridDim.x * blockDim.x)
+        a.H[c] = a.h[c] + a.z[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_olf_faces(OlfArgs a)
+{
+    const Geo2 g = a.g;
+    const int nx = g.nx;
+    const double* __restrict__ H = a.H;
+    const double* __restrict__ h = a.h;
+    const double* __restrict__ m = a.mann;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const int j = c / nx, i = c - j * nx;
+        double ke = 0.0, kn = 0.0;
+        // east face (i,j)-(i+1,j): used when row j is interior
+        if (i < nx - 1 && j > 0 && j < g.ny - 1) {
+            const double gx = (H[c + 1] - H[c]) / g.dx;
+            const double gy = (H[c + nx + 1] + H[c + nx] - H[c - nx + 1] - H[c - nx]) / (4 * g.dy);
+            const bool west_owner = !is_ring(g, i, j);
+            // owner's view: neighbour upwind if its H is strictly higher
+            const bool up_east = west_owner ? (H[c + 1] > H[c]) : !(H[c] > H[c + 1]);
+            ke = olf_face_k(gx, gy, h[c + 1], h[c], m[c + 1], m[c], up_east, a.hmin, a.delta, a.K0);
+            // olf_face_k's "neighbour" is the east cell here
+        }
+        // north face (i,j)-(i,j+1): used when column i is interior
+        if (j < g.ny - 1 && i > 0 && i < nx - 1) {
+            const double gyy = (H[c + nx] - H[c]) / g.dy;
+            const double gxx = (H[c + nx + 1] + H[c + 1] - H[c + nx - 1] - H[c - 1]) / (4 * g.dx);
+            const bool south_owner = !is_ring(g, i, j);
+            const bool up_north = south_owner ? (H[c + nx] > H[c]) : !(H[c] > H[c + nx]);
+            kn = olf_face_k(gyy, gxx, h[c + nx], h[c], m[c + nx], m[c], up_north, a.hmin, a.delta, a.K0);
+        }
+        a.ke[c] = ke;
+        a.kn[c] = kn;
+    }
+}
+
+// SetUpLinearSystemsOverland (src/overland.cu:65-189) with the ring eliminated, fused with
+// r0 = b - A H.
+__global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
+{
+    const Geo2 g = a.g;
+    const int nx = g.nx;
+    const double dtodx2 = -g.cx, dtody2 = -g.cy;
+    const double* __restrict__ H = a.H;
+    const double* __restrict__ ke = a.ke;
+    const double* __restrict__ kn = a.kn;
+    double acc[3] = {0.0, 0.0, 0.0};   // bb, rz, rr
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const int j = c / nx, i = c - j * nx;
+        double D, b, r;
+        if (is_ring(g, i, j)) {
+            D = 1.0;
+            b = H[c] + 0.0;
+            r = 0.0;
+        } else {
+            const double kE = ke[c], kW = ke[c - 1], kN = kn[c], kS = kn[c - nx];
+            D = 1.0 - dtodx2 * (kE + kW) - dtody2 * (kN + kS);
+            b = H[c] + 0.0 + 0.0 + 0.0;
+            double Ax = D * H[c];
+            // couplings to ring cells move to the right-hand side
+            if (i == 1) b -= dtodx2 * kW * H[c - 1]; else Ax += dtodx2 * kW * H[c - 1];
+            if (i == nx - 2) b -= dtodx2 * kE * H[c + 1]; else Ax += dtodx2 * kE * H[c + 1];
+            if (j == 1) b -= dtody2 * kS * H[c - nx]; else Ax += dtody2 * kS * H[c - nx];
+            if (j == g.ny - 2) b -= dtody2 * kN * H[c + nx]; else Ax += dtody2 * kN * H[c + nx];
+            r = b - Ax;
+        }
+        a.D[c] = D;
+        a.b[c] = b;
+        a.r[c] = r;
+        acc[0] += b * b;
+        acc[1] += r * (r / D);
+        acc[2] += r * r;
+    }
+    double out[3];
+    if (grid_reduce<3, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
+        PcgState* st = a.st;
+        st->bb = out[0];
+        st->rz = out[1];
+        st->rr = out[2];
+        st->iter = 0;
+        st->total = 0;
+        st->beta = 0.0;
+        st->alpha = 0.0;
+        st->tol2 = a.tol * a.tol * out[0];
+        st->maxit = a.maxit;
+        st->conv = (out[2] <= st->tol2) ? 1 : 0;
+        st->nan = isfinite(out[0]) ? 0 : 1;
+        st->done = (st->conv || st->nan || a.maxit <= 0) ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg2_spmv(Geo2 g, const double* __restrict__ ke,
+                                                      const double* __restrict__ kn,
+                                                      const double* __restrict__ D,
+                                                      const double* __restrict__ r,
+                                                      const double* __restrict__ pold,
+                                                      double* __restrict__ pnew, double* __restrict__ q,
+                                                      PcgState* st, Reduce red)
+{
+    if (st->done) return;
+    const bool first = (st->iter == 0);
+    const double beta = st->beta;
+    auto pf = [&](int c) -> double {
+        const double z = r[c] / D[c];
+        return first ? z : z + beta * pold[c];
+    };
+    double acc[1] = {0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const double pc = pf(c);
+        pnew[c] = pc;
+        const double qc = op2_row(g, c, D[c], pc, pf, ke, kn);
+        q[c] = qc;
+        acc[0] += pc * qc;
+    }
+    double out[1];
+    if (grid_reduce<1, false>(acc, red.part, red.cnt, out) && lin_tid() == 0) {
+        st->pq = out[0];
+        const double al = st->rz / out[0];
+        if (!(out[0] != 0.0) || !isfinite(al)) {
+            st->done = 1;
+            st->nan = isfinite(out[0]) ? st->nan : 1;
+            st->alpha = 0.0;
+        } else {
+            st->alpha = al;
+        }
+    }
+}
+
+// WaterDepthOverland (src/overland.cu:371-383) and the discharge into the Dirichlet ring.
+__global__ __launch_bounds__(kBlock) void k_olf_post(OlfArgs a)
+{
+    const Geo2 g = a.g;
+    const int nx = g.nx;
+    const double* __restrict__ H = a.H;
+    double acc[2] = {0.0, 0.0};   // x = 0 edge, whole ring
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const int j = c / nx, i = c - j * nx;
+        if (!is_ring(g, i, j)) {
+            const double Hc = H[c];
+            if (i == 1) {
+                const double q = a.ke[c - 1] * (Hc - H[c - 1]) / g.dx * g.dy;
+                acc[0] += q;
+                acc[1] += q;
+            }
+            if (i == nx - 2) acc[1] += a.ke[c] * (Hc - H[c + 1]) / g.dx * g.dy;
+            if (j == 1) acc[1] += a.kn[c - nx] * (Hc - H[c - nx]) / g.dy * g.dx;
+            if (j == g.ny - 2) acc[1] += a.kn[c] * (Hc - H[c + nx]) / g.dy * g.dx;
+        }
+        a.h[c] = fmax(H[c] - a.z[c], 0.0);
+    }
+    double out[2];
+    if (grid_reduce<2, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
+        a.out[0] = out[0];
+        a.out[1] = out[1];
+    }
+}
+
+}  // namespace
+
+// =========================================================================================
+// Launchers
+// =========================================================================================
+void launch_vg(hipStream_t s, const VgArgs& a)
+{
+    hipLaunchKernelGGL(k_vg, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_vg_picard(hipStream_t s, const VgArgs& a)
+{
+    hipLaunchKernelGGL(k_vg_picard, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_vg_inverse(hipStream_t s, const Geo& g, const Soil& soil, const double* theta, double* psi)
+{
+    hipLaunchKernelGGL(k_vg_inverse, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, soil, theta, psi);
+}
+
+void launch_row_flags(hipStream_t s, const Geo& g, const int* const bctype[6], uint8_t* flags)
+{
+    BcTypes bt;
+    for (int f = 0; f < 6; ++f) bt.t[f] = bctype[f];
+    hipLaunchKernelGGL(k_row_flags, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, bt, flags);
+}
+
+void launch_surface_pre(hipStream_t s, const SurfArgs& a)
+{
+    hipLaunchKernelGGL(k_surface_pre, dim3(grid_for(a.nxy)), dim3(kBlock), 0, s, a);
+}
+
+void launch_surface_post(hipStream_t s, const SurfArgs& a)
+{
+    hipLaunchKernelGGL(k_surface_post, dim3(grid_for(a.nxy)), dim3(kBlock), 0, s, a);
+}
+
+void launch_assemble(hipStream_t s, const AsmArgs& a, bool mixed)
+{
+    const dim3 gr(grid_for(a.g.n)), bl(kBlock);
+    const bool dir = a.flags != nullptr;
+    if (dir) {
+        if (mixed) hipLaunchKernelGGL((k_assemble<true, true>), gr, bl, 0, s, a);
+        else hipLaunchKernelGGL((k_assemble<true, false>), gr, bl, 0, s, a);
+    } else {
+        if (mixed) hipLaunchKernelGGL((k_assemble<false, true>), gr, bl, 0, s, a);
+        else hipLaunchKernelGGL((k_assemble<false, false>), gr, bl, 0, s, a);
+    }
+}
+
+void launch_refine(hipStream_t s, const RefineArgs& a)
+{
+    const dim3 gr(grid_for(a.g.n)), bl(kBlock);
+    hipLaunchKernelGGL(k_refine_add, gr, bl, 0, s, a.g.n, a.x, (const float*)a.d32);
+    if (a.flags) hipLaunchKernelGGL((k_refine_residual<true>), gr, bl, 0, s, a);
+    else hipLaunchKernelGGL((k_refine_residual<false>), gr, bl, 0, s, a);
+}
+
+void launch_end_step(hipStream_t s, const EndArgs& a)
+{
+    hipLaunchKernelGGL(k_end_step, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+template <typename T>
+static void pcg3_iter(hipStream_t s, const Geo& g, const uint8_t* flags, const T* K, const T* D,
+                      T* x, T* r, T* const p[2], T* q, int parity, PcgState* st, Reduce red)
+{
+    const dim3 gr(grid_for(g.n)), bl(kBlock);
+    const T* pold = p[parity & 1];
+    T* pnew = p[(parity + 1) & 1];
+    if (flags) hipLaunchKernelGGL((k_pcg3_spmv<T, true>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, st, red);
+    else hipLaunchKernelGGL((k_pcg3_spmv<T, false>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, st, red);
+    hipLaunchKernelGGL((k_pcg_update<T>), gr, bl, 0, s, g.n, x, r, D, (const T*)pnew, (const T*)q, st, red);
+}
+
+void launch_pcg3_iter_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
+                          const double* D, double* x, double* r, double* const p[2], double* q,
+                          int parity, PcgState* st, Reduce red)
+{
+    pcg3_iter<double>(s, g, flags, K, D, x, r, p, q, parity, st, red);
+}
+
+void launch_pcg3_iter_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
+                          const float* D, float* x, float* r, float* const p[2], float* q,
+                          int parity, PcgState* st, Reduce red)
+{
+    pcg3_iter<float>(s, g, flags, K, D, x, r, p, q, parity, st, red);
+}
+
+void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
+                       const double* D, const double* x, double* y)
+{
+    const dim3 gr(grid_for(g.n)), bl(kBlock);
+    if (flags) hipLaunchKernelGGL((k_apply3<double, true>), gr, bl, 0, s, g, flags, K, D, x, y);
+    else hipLaunchKernelGGL((k_apply3<double, false>), gr, bl, 0, s, g, flags, K, D, x, y);
+}
+
+void launch_apply3_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
+                       const float* D, const float* x, float* y)
+{
+    const dim3 gr(grid_for(g.n)), bl(kBlock);
+    if (flags) hipLaunchKernelGGL((k_apply3<float, true>), gr, bl, 0, s, g, flags, K, D, x, y);
+    else hipLaunchKernelGGL((k_apply3<float, false>), gr, bl, 0, s, g, flags, K, D, x, y);
+}
+
+void launch_cast_f32(hipStream_t s, int n, const double* in, float* out)
+{
+    hipLaunchKernelGGL(k_cast_f32, dim3(grid_for(n)), dim3(kBlock), 0, s, n, in, out);
+}
+
+void launch_cast_f64(hipStream_t s, int n, const float* in, double* out)
+{
+    hipLaunchKernelGGL(k_cast_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, n, in, out);
+}
+
+void launch_olf_faces(hipStream_t s, const OlfArgs& a)
+{
+    hipLaunchKernelGGL(k_olf_H, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_olf_faces, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_olf_assemble(hipStream_t s, const OlfArgs& a)
+{
+    hipLaunchKernelGGL(k_olf_assemble, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_pcg2_iter(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
+                      const double* D, double* x, double* r, double* const p[2], double* q,
+                      int parity, PcgState* st, Reduce red)
+{
+    const dim3 gr(grid_for(g.n)), bl(kBlock);
+    const double* pold = p[parity & 1];
+    double* pnew = p[(parity + 1) & 1];
+    hipLaunchKernelGGL(k_pcg2_spmv, gr, bl, 0, s, g, ke, kn, D, r, pold, pnew, q, st, red);
+    hipLaunchKernelGGL((k_pcg_update<double>), gr, bl, 0, s, g.n, x, r, D, (const double*)pnew,
+                       (const double*)q, st, red);
+}
+
+void launch_olf_post(hipStream_t s, const OlfArgs& a)
+{
+    hipLaunchKernelGGL(k_olf_post, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+}  // namespace dh

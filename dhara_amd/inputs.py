@@ -1,0 +1,210 @@
+"""Synthetic workloads of the benchmark configurations (deterministic from fixed seeds).
+
+Gaussian random fields use circulant-embedding spectral synthesis of a stationary field with
+exponential covariance sigma^2 exp(-r/lambda) on a 2x padded periodic grid, cropped to the
+domain (Dietrich & Newsam 1997).  All generation is host-side numpy with
+numpy.random.default_rng(seed); the same arrays are uploaded to every solver instance.
+
+Each builder returns a `Workload`: grid, params, the fields to upload, the number of steps,
+and a rain schedule (rate per step index).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+LOAM = dict(theta_s=0.43, theta_r=0.078, alpha=3.6, n_vg=1.56, Ss=1e-5, porosity=0.43)
+KS_LOAM = 2.89e-6          # m/s
+MM_PER_H = 1e-3 / 3600.0   # (mm/h) -> (m/s)
+
+
+def grf(shape, dxs, sigma, corr_len, seed):
+    """Stationary Gaussian field with covariance sigma^2 exp(-|r|/corr_len).
+
+    shape: grid shape (ny, nx) or (n,); dxs: spacing per axis (same order as shape).
+    """
+    shape = tuple(int(s) for s in shape)
+    dxs = tuple(float(d) for d in dxs)
+    pad = tuple(2 * s for s in shape)
+    axes = []
+    for n, d in zip(pad, dxs):
+        idx = np.arange(n)
+        idx = np.minimum(idx, n - idx)          # periodic distance on the padded torus
+        axes.append(idx * d)
+    grids = np.meshgrid(*axes, indexing="ij")
+    r = np.sqrt(sum(g * g for g in grids))
+    cov = sigma * sigma * np.exp(-r / corr_len)
+    lam = np.real(np.fft.fftn(cov))
+    lam = np.maximum(lam, 0.0)
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(pad) + 1j * rng.standard_normal(pad)
+    f = np.fft.ifftn(np.sqrt(lam) * noise) * np.sqrt(np.prod(pad))
+    out = np.real(f)
+    sl = tuple(slice(0, s) for s in shape)
+    return np.ascontiguousarray(out[sl])
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: _abi.Grid
+    params: _abi.Params
+    fields: dict
+    nsteps: int
+    rain_schedule: list = field(default_factory=list)   # (first_step, rate array or scalar)
+    init_mode: int = _abi.INIT_FROM_PSI
+
+    @property
+    def ncells(self):
+        g = self.grid
+        return g.nx * g.ny * g.nz
+
+    def rain_at(self, step):
+        """Rain field (scalar or array) in force at a step index, None if unchanged."""
+        cur = None
+        for first, rate in self.rain_schedule:
+            if step >= first:
+                cur = rate
+        return cur
+
+    def apply(self, model):
+        model.upload_many(self.fields)
+        r0 = self.rain_at(0)
+        if r0 is not None:
+            model.upload("RAIN", r0)
+        model.init_state(self.init_mode)
+
+    def run(self, model, nsteps=None, start=0):
+        """Advance `nsteps` steps from step index `start`, applying the rain schedule."""
+        nsteps = self.nsteps if nsteps is None else nsteps
+        t, end = start, start + nsteps
+        stats = []
+        r = self.rain_at(t)
+        if r is not None:
+            model.upload("RAIN", r)
+        while t < end:
+            nxt = min([f for f, _ in self.rain_schedule if f > t] + [end])
+            stats += model.step(nxt - t)
+            t = nxt
+            r = self.rain_at(t)
+            if t < end and r is not None:
+                model.upload("RAIN", r)
+        return stats
+
+
+def hydrostatic_psi(nx, ny, nz, dz, wt_depth):
+    zc = (np.arange(nz) + 0.5) * dz                      # cell-centre depth, z down
+    col = zc - wt_depth
+    return np.ascontiguousarray(np.repeat(col, nx * ny))
+
+
+def c1_column(nx=32, ny=32, nz=20):
+    """Plain FP64 3D Richards: loam, hydrostatic IC (water table 1.0 m), top flux 1e-6 m/s,
+    free drainage, no-flux sides, dt = 60 s, 100 steps, Picard 1e-6 m, PCG 1e-12."""
+    g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
+    p = _abi.default_params(dt=60.0, overland=0, top_bc=_abi.TOP_PRESCRIBED,
+                            picard_tol=1e-6, lin_tol=1e-12, **LOAM)
+    s = nx * ny
+    fields = {
+        "KSAT": np.full(nx * ny * nz, KS_LOAM),
+        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 1.0),
+        "BCTYPE_T": np.full(s, _abi.BC_FLUX, np.int32),
+        "BCQ_T": np.full(s, 1e-6),
+    }
+    return Workload("c1_32x32x20_fp64", g, p, fields, nsteps=100)
+
+
+def _surface(nx, ny, dx, seed_topo, slope=0.01, sigma=0.05, corr=5.0):
+    x = (np.arange(nx) + 0.5) * dx
+    z = slope * x[None, :] + grf((ny, nx), (dx, dx), sigma, corr, seed_topo)
+    return np.ascontiguousarray(z.ravel())
+
+
+def c2_coupled(nx=400, ny=400, nz=30, precision=_abi.FP64, dt=30.0, rain_mm_h=20.0,
+               rain_s=3600.0, total_s=7200.0):
+    """Plain FP64 coupled run: 1% slope + GRF(0.05 m, 5 m) topography, Manning 0.03,
+    400x400x30 subsurface dz = 0.05 m, log-normal Ks (gm 2.89e-6, sigma_lnK 1, 10 m,
+    constant with depth), rain 20 mm/h for 3600 s then 3600 s recession, dt = 30 s."""
+    g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
+    p = _abi.default_params(dt=dt, overland=1, top_bc=_abi.TOP_SURFACE, precision=precision,
+                            **LOAM)
+    lnk = np.log(KS_LOAM) + grf((ny, nx), (1.0, 1.0), 1.0, 10.0, 2)
+    ks2 = np.exp(lnk).ravel()
+    fields = {
+        "KSAT": np.ascontiguousarray(np.tile(ks2, nz)),
+        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 1.0),
+        "ZTOPO": _surface(nx, ny, 1.0, 1),
+        "MANNING": np.full(nx * ny, 0.03),
+        "PONDING": np.zeros(nx * ny),
+    }
+    nsteps = int(round(total_s / dt))
+    rain_steps = int(round(rain_s / dt))
+    sched = [(0, rain_mm_h * MM_PER_H), (rain_steps, 0.0)]
+    name = f"c{'3' if precision == _abi.MIXED else '2'}_coupled_{nx}x{ny}x{nz}"
+    return Workload(name, g, p, fields, nsteps=nsteps, rain_schedule=sched)
+
+
+def c3_coupled_mixed(**kw):
+    return c2_coupled(precision=_abi.MIXED, **kw)
+
+
+def c4_adjoint(nx=64, ny=64, nz=30, perturbed=False):
+    """Adjoint workload: 64x64x30, dz = 0.05, lnKs GRF (sigma 1, 10 m), per-cell
+    alpha ~ U[2,5] 1/m and n ~ U[1.3,1.8], 10 mm/h rain, dt = 60 s, 50 steps."""
+    g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
+    p = _abi.default_params(dt=60.0, overland=1, top_bc=_abi.TOP_SURFACE, **LOAM)
+    n3 = nx * ny * nz
+    lnk = np.log(KS_LOAM) + grf((ny, nx), (1.0, 1.0), 1.0, 10.0, 3)
+    lnks = np.tile(lnk.ravel(), nz)
+    alpha = np.random.default_rng(4).uniform(2.0, 5.0, n3)
+    nvg = np.random.default_rng(5).uniform(1.3, 1.8, n3)
+    if perturbed:
+        rng = np.random.default_rng(6)
+        lnks = lnks + 0.05 * rng.standard_normal(n3)
+        alpha = alpha * (1.0 + 0.05 * rng.standard_normal(n3))
+        nvg = 1.0 + (nvg - 1.0) * (1.0 + 0.05 * rng.standard_normal(n3))
+    fields = {
+        "KSAT": np.exp(lnks), "ALPHA": alpha, "NVG": nvg,
+        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 1.0),
+        "ZTOPO": _surface(nx, ny, 1.0, 1),
+        "MANNING": np.full(nx * ny, 0.03),
+        "PONDING": np.zeros(nx * ny),
+    }
+    return Workload(f"c4_adjoint_{nx}x{ny}x{nz}", g, p, fields, nsteps=50,
+                    rain_schedule=[(0, 10.0 * MM_PER_H)])
+
+
+def c5_instances(count=256, seed=100):
+    """Ensemble instance shapes and parameters: nx, ny ~ U{32..128}, nz ~ U{10..40},
+    sigma_lnK ~ U[0.5, 1.5], rain ~ U[5, 50] mm/h; instance i uses seed 1000 + i."""
+    rng = np.random.default_rng(seed)
+    nx = rng.integers(32, 129, count)
+    ny = rng.integers(32, 129, count)
+    nz = rng.integers(10, 41, count)
+    sig = rng.uniform(0.5, 1.5, count)
+    rain = rng.uniform(5.0, 50.0, count)
+    return [dict(nx=int(nx[i]), ny=int(ny[i]), nz=int(nz[i]), sigma_lnk=float(sig[i]),
+                 rain_mm_h=float(rain[i]), seed=1000 + i) for i in range(count)]
+
+
+def c5_member(spec, dt=30.0, precision=_abi.MIXED):
+    nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
+    g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
+    p = _abi.default_params(dt=dt, overland=1, top_bc=_abi.TOP_SURFACE, precision=precision,
+                            **LOAM)
+    seed = spec["seed"]
+    n3 = nx * ny * nz
+    lnk = np.log(KS_LOAM) + grf((ny, nx), (1.0, 1.0), spec["sigma_lnk"], 10.0, seed)
+    rng = np.random.default_rng(seed + 7)
+    fields = {
+        "KSAT": np.tile(np.exp(lnk).ravel(), nz),
+        "ALPHA": rng.uniform(2.0, 5.0, n3),
+        "NVG": rng.uniform(1.3, 1.8, n3),
+        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 1.0),
+        "ZTOPO": _surface(nx, ny, 1.0, seed + 13),
+        "MANNING": np.full(nx * ny, 0.03),
+        "PONDING": np.zeros(nx * ny),
+    }
+    return Workload(f"c5_member_{seed}", g, p, fields, nsteps=200,
+                    rain_schedule=[(0, spec["rain_mm_h"] * MM_PER_H)])

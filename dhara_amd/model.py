@@ -1,0 +1,233 @@
+"""Host-side mirror of the flow-model entry points, on top of any library with the dhara C-ABI.
+
+`FlowModel` drives the HIP library (libdhara_hip.so).  The class is written against the
+ABI, not the library, so one driver loop serves every implementation of the same entry
+points (the parity tests use this to run identical call sequences).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import F, INT_FIELDS
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB_PATH = os.path.join(_HERE, "libdhara_hip.so")
+
+
+class DharaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dhara error {code}: {msg}")
+        self.code = code
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class BaseModel:
+    """One solver context.  `lib`/`prefix` select the implementation."""
+
+    def __init__(self, lib, prefix, handle, grid, params):
+        self._lib = lib
+        self._p = prefix
+        self._h = handle
+        self.grid = grid
+        self._params = params
+
+    # ---- plumbing ---------------------------------------------------------------------
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._fn("last_error")(self._h)
+            raise DharaError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._h:
+            self._fn("destroy")(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- parameters -------------------------------------------------------------------
+    @property
+    def params(self):
+        return self._params
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if not hasattr(self._params, k):
+                raise KeyError(k)
+            setattr(self._params, k, v)
+        self._check(self._fn("set_params")(self._h, C.byref(self._params)), "set_params")
+
+    # ---- fields -----------------------------------------------------------------------
+    def size(self, name):
+        return _abi.face_size(self.grid, F[name])
+
+    def upload(self, name, values):
+        fid = F[name]
+        n = self.size(name)
+        if fid in INT_FIELDS:
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(values), (n,)), dtype=np.int32)
+            rc = self._fn("upload_field_i32")(self._h, fid, _iptr(a), a.size)
+        else:
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (n,)))
+            rc = self._fn("upload_field")(self._h, fid, _dptr(a), a.size)
+        self._check(rc, f"upload {name}")
+
+    def download(self, name):
+        fid = F[name]
+        n = self.size(name)
+        if fid in INT_FIELDS:
+            a = np.empty(n, dtype=np.int32)
+            rc = self._fn("download_field_i32")(self._h, fid, _iptr(a), n)
+        else:
+            a = np.empty(n, dtype=np.float64)
+            rc = self._fn("download_field")(self._h, fid, _dptr(a), n)
+        self._check(rc, f"download {name}")
+        return a
+
+    def upload_many(self, fields):
+        for k, v in fields.items():
+            self.upload(k, v)
+
+    # ---- stepping ---------------------------------------------------------------------
+    def init_state(self, mode=_abi.INIT_FROM_PSI):
+        self._check(self._fn("init_state")(self._h, mode), "init_state")
+
+    def subsurface_step(self):
+        st = _abi.StepStats()
+        self._check(self._fn("subsurface_step")(self._h, C.byref(st)), "subsurface_step")
+        return st.as_dict()
+
+    def overland_step(self):
+        st = _abi.StepStats()
+        self._check(self._fn("overland_step")(self._h, C.byref(st)), "overland_step")
+        return st.as_dict()
+
+    def step(self, nsteps=1):
+        arr = (_abi.StepStats * max(nsteps, 1))()
+        self._check(self._fn("step")(self._h, nsteps, arr), "step")
+        return [arr[i].as_dict() for i in range(nsteps)]
+
+    # ---- kernel-level ---------------------------------------------------------------------
+    def eval_constitutive(self, psi):
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        n = self.size("PSI")
+        if psi.size != n:
+            raise ValueError("psi must have nx*ny*nz elements")
+        th, k, c = (np.empty(n) for _ in range(3))
+        self._check(self._fn("eval_constitutive")(self._h, _dptr(psi), _dptr(th), _dptr(k), _dptr(c)),
+                    "eval_constitutive")
+        return th, k, c
+
+    def eval_inverse(self, theta):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        out = np.empty_like(theta)
+        self._check(self._fn("eval_inverse")(self._h, _dptr(theta), _dptr(out)), "eval_inverse")
+        return out
+
+    def apply_operator(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._check(self._fn("apply_subsurface_operator")(self._h, _dptr(x), _dptr(y)),
+                    "apply_subsurface_operator")
+        return y
+
+
+_hip_lib = None
+
+
+def load_hip_library(path=HIP_LIB_PATH):
+    """Load libdhara_hip.so.  There is no fallback: without the HIP build this raises."""
+    global _hip_lib
+    if _hip_lib is None:
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{path} is missing: build it with `python -m dhara_amd.build` "
+                "(hipcc --offload-arch=gfx950)")
+        lib = C.CDLL(path)
+        _abi.declare(lib, "dhara_")
+        P = C.POINTER
+        lib.dhara_create.argtypes = [P(_abi.Grid), P(_abi.Params), P(C.c_void_p)]
+        lib.dhara_create.restype = C.c_int
+        lib.dhara_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        lib.dhara_device_info.restype = C.c_int
+        lib.dhara_abi_version.argtypes = []
+        lib.dhara_abi_version.restype = C.c_int
+        lib.dhara_assemble_subsurface.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        lib.dhara_assemble_subsurface.restype = C.c_int
+        lib.dhara_apply_subsurface_operator_f32.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        lib.dhara_apply_subsurface_operator_f32.restype = C.c_int
+        lib.dhara_overland_conductance.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        lib.dhara_overland_conductance.restype = C.c_int
+        lib.dhara_set_precision.argtypes = [C.c_void_p, C.c_int32]
+        lib.dhara_set_precision.restype = C.c_int
+        if lib.dhara_abi_version() != _abi.ABI_VERSION:
+            raise ImportError("libdhara_hip.so ABI version mismatch; rebuild")
+        _hip_lib = lib
+    return _hip_lib
+
+
+def device_info():
+    lib = load_hip_library()
+    buf = C.create_string_buffer(256)
+    rc = lib.dhara_device_info(buf, 256)
+    if rc != 0:
+        raise DharaError(rc, "no HIP device")
+    return buf.value.decode()
+
+
+class FlowModel(BaseModel):
+    """The HIP solver (one GPU context, one stream)."""
+
+    def __init__(self, grid, params=None):
+        lib = load_hip_library()
+        params = params if params is not None else _abi.default_params()
+        h = C.c_void_p()
+        rc = lib.dhara_create(C.byref(grid), C.byref(params), C.byref(h))
+        if rc != 0:
+            raise DharaError(rc, "dhara_create failed (no GPU or out of memory)")
+        super().__init__(lib, "dhara_", h, grid, params)
+
+    def assemble(self):
+        n = self.size("PSI")
+        d, b = np.empty(n), np.empty(n)
+        self._check(self._lib.dhara_assemble_subsurface(self._h, _dptr(d), _dptr(b)), "assemble")
+        return d, b
+
+    def apply_operator_f32(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._check(self._lib.dhara_apply_subsurface_operator_f32(self._h, _dptr(x), _dptr(y)),
+                    "apply_subsurface_operator_f32")
+        return y
+
+    def overland_conductance(self):
+        s = self.grid.nx * self.grid.ny
+        ke, kn = np.empty(s), np.empty(s)
+        self._check(self._lib.dhara_overland_conductance(self._h, _dptr(ke), _dptr(kn)),
+                    "overland_conductance")
+        return ke, kn
+
+    def set_precision(self, prec):
+        self._check(self._lib.dhara_set_precision(self._h, prec), "set_precision")
+        self._params.precision = prec

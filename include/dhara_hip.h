@@ -1,0 +1,253 @@
+/*
+ * dhara_hip.h -- C-ABI of the MI355X (gfx950) integrated surface/subsurface flow solver.
+ *
+ * Plain C: fixed-width integers, doubles, opaque handles.  No HIP, torch or C++ types cross
+ * this boundary.  Every function returns 0 on success or a negative DHARA_E* code; the text
+ * of the last error of a context is returned by dhara_last_error().  Nothing here calls
+ * exit(); the reference terminated the process on any device error
+ * (src/device.cu:162-171 CheckCudaCall, include/main.h:58-68 cudaCheckError).
+ *
+ * Units are SI throughout: psi [m], alpha [1/m], K [m/s], dt [s], rain/evaporation/
+ * transpiration rates [m/s].  The reference kept psi in m but alpha in 1/cm, dt in hours
+ * and K in m per time step (include/devconst.h:6-21, src/global.cu:79,96, src/host.cc:53);
+ * the reference-named shims in dhara_shim.h do that conversion.
+ *
+ * Layout (identical to the reference, so uploads are plain copies):
+ *   3D cell (i,j,k):   k*nx*ny + j*nx + i, k = 0 is the TOP layer, z positive downward
+ *                      (src/subsurface.cu:401-410)
+ *   surface (i,j):     j*nx + i
+ *   W/E face arrays:   k*ny + j;  S/N face arrays: k*nx + i;  top/bottom: j*nx + i
+ *                      (src/subsurface.cu:139-142)
+ *
+ * Threading: one context is used by one host thread at a time; it owns one HIP stream and
+ * all device memory it needs.  Separate contexts are independent.  Calls are synchronous
+ * on return.
+ */
+#ifndef DHARA_HIP_H
+#define DHARA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DHARA_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define DHARA_OK            0
+#define DHARA_E_ARG        -1   /* bad argument / size mismatch                        */
+#define DHARA_E_DEVICE     -2   /* HIP runtime error                                    */
+#define DHARA_E_ALLOC      -3   /* device or host allocation failed                     */
+#define DHARA_E_STATE      -4   /* call made in the wrong state (e.g. step before init) */
+#define DHARA_E_NUMERIC    -5   /* NaN/Inf detected in the state                        */
+#define DHARA_E_UNSUPPORTED -6  /* option not supported by this build                   */
+
+/* ---- enumerations ---------------------------------------------------------------- */
+/* Picard stopping rule.  src/subsurface.cu:762-780: PICARD_METHOD 0 uses
+ * max|(psi^{m+1}-psi^m)/psi^{m+1}|, PICARD_METHOD 1 the same on theta. */
+#define DHARA_PICARD_ABS_PSI   0   /* max|psi^{m+1}-psi^m| [m]  (default)               */
+#define DHARA_PICARD_REL_PSI   1   /* reference PICARD_METHOD = 0                        */
+#define DHARA_PICARD_REL_THETA 2   /* reference PICARD_METHOD = 1                        */
+
+/* Linear-solver precision of the subsurface solve. */
+#define DHARA_FP64   0   /* Jacobi-PCG in FP64 end to end                                */
+#define DHARA_MIXED  1   /* FP32 Jacobi-PCG inner solves + FP64 residual refinement      */
+
+/* theta carried to the next step.  The reference copies theta^{n+1,m} (the iterate
+ * BEFORE the last solve) into theta^n (src/subsurface.cu:839). */
+#define DHARA_THETA_CARRY_CONSISTENT 0   /* theta^n <- theta(psi^{n+1})  (default)       */
+#define DHARA_THETA_CARRY_REFERENCE  1   /* theta^n <- theta^{n+1,m}     (reference)      */
+
+/* Top boundary of the subsurface.  The reference always derives it from the surface
+ * (EstimateFluxes + IdentifyTopBoundary, src/global.cu:172-257, which force type 1 on every
+ * Picard iteration) and updates the ponding depth from it (src/subsurface.cu:465-491). */
+#define DHARA_TOP_SURFACE    0   /* reference coupling (default)                          */
+#define DHARA_TOP_PRESCRIBED 1   /* use the uploaded DHARA_F_BCTYPE_T/BCPSI_T/BCQ_T; the
+                                    ponding depth is left unchanged                       */
+
+/* State initialisation. */
+#define DHARA_INIT_FROM_THETA 0   /* psi <- vG^-1(theta), then theta,K,C <- vG(psi)
+                                     (src/device.cu:323-330, src/global.cu:289)          */
+#define DHARA_INIT_FROM_PSI   1   /* theta,K,C <- vG(psi)                                 */
+
+/* Boundary types (src/subsurface.cu:126-353): 0 = Dirichlet (pressure head), else flux. */
+#define DHARA_BC_DIRICHLET 0
+#define DHARA_BC_FLUX      1
+
+/* ---- field identifiers ------------------------------------------------------------ */
+enum dhara_field {
+    /* 3D cell fields, n = nx*ny*nz */
+    DHARA_F_PSI = 0,        /* pressure head psi^n [m]                      (in/out)   */
+    DHARA_F_THETA,          /* water content theta^n [-]                    (in/out)   */
+    DHARA_F_KSAT,           /* saturated conductivity [m/s]                 (in)       */
+    DHARA_F_ALPHA,          /* van Genuchten alpha per cell [1/m]  (optional, in)      */
+    DHARA_F_NVG,            /* van Genuchten n per cell [-]        (optional, in)      */
+    DHARA_F_K,              /* unsaturated conductivity K(psi) [m/s]        (out)      */
+    DHARA_F_C,              /* specific moisture capacity dtheta/dpsi [1/m] (out)      */
+    /* surface fields, n = nx*ny */
+    DHARA_F_ZTOPO,          /* ground elevation z [m]                       (in)       */
+    DHARA_F_MANNING,        /* Manning n [s m^-1/3]                         (in)       */
+    DHARA_F_PONDING,        /* ponding / overland water depth h [m]         (in/out)   */
+    DHARA_F_WATERELEV,      /* water surface elevation H = h + z [m]        (out)      */
+    DHARA_F_RAIN,           /* rain (throughfall) rate [m/s]                (in)       */
+    DHARA_F_EVAP,           /* soil evaporation rate [m/s]                  (in)       */
+    DHARA_F_TRANSP,         /* transpiration rate [m/s]                     (in)       */
+    DHARA_F_QTOP,           /* infiltration flux into layer 0 [m/s], +down  (out)      */
+    DHARA_F_QBOT,           /* free-drainage flux out of the bottom [m/s]   (out)      */
+    DHARA_F_HPOTEN,         /* potential surface head [m]                   (out)      */
+    DHARA_F_QCAPA,          /* infiltration capacity [m/s]                  (out)      */
+    /* boundary values, n = size of the face (W/E: ny*nz, S/N: nx*nz, T/B: nx*ny) */
+    DHARA_F_BCPSI_W, DHARA_F_BCPSI_E, DHARA_F_BCPSI_S, DHARA_F_BCPSI_N,
+    DHARA_F_BCPSI_T, DHARA_F_BCPSI_B,
+    DHARA_F_BCQ_W, DHARA_F_BCQ_E, DHARA_F_BCQ_S, DHARA_F_BCQ_N,
+    DHARA_F_BCQ_T, DHARA_F_BCQ_B,
+    /* root-uptake distribution: n = nroot_groups*nz, row g is the profile of group g */
+    DHARA_F_ROOT,
+    /* integer fields (dhara_upload_field_i32 / dhara_download_field_i32) */
+    DHARA_F_BCTYPE_W, DHARA_F_BCTYPE_E, DHARA_F_BCTYPE_S, DHARA_F_BCTYPE_N,
+    DHARA_F_BCTYPE_T, DHARA_F_BCTYPE_B,
+    DHARA_F_PROCMAP,        /* surface cell -> root group (reference: MPI rank map)    */
+    DHARA_F__COUNT
+};
+
+/* ---- structures -------------------------------------------------------------------- */
+typedef struct dhara_grid {
+    int32_t nx, ny, nz;
+    int32_t nroot_groups;   /* rows of DHARA_F_ROOT (reference: MPI process count)     */
+    double  dx, dy, dz;     /* [m]                                                      */
+} dhara_grid;
+
+typedef struct dhara_params {
+    double  dt;                                   /* time step [s]                      */
+    /* Mualem-van Genuchten scalars (per-cell alpha/n override when uploaded)           */
+    double  theta_s, theta_r, alpha, n_vg, Ss, porosity;
+    /* Picard loop (src/subsurface.cu:715-790)                                          */
+    int32_t picard_maxiter, picard_rule;
+    double  picard_tol, am;
+    /* subsurface linear solve                                                          */
+    int32_t lin_maxiter, precision;
+    double  lin_tol;                              /* ||r||_2 <= lin_tol * ||b||_2         */
+    int32_t refine_maxiter, inner_maxiter;        /* DHARA_MIXED only                     */
+    double  inner_tol;                            /* FP32 inner relative tolerance        */
+    /* overland flow (src/overland.cu:387-419, include/devconst.h:25-28)                 */
+    int32_t overland, olf_maxiter;
+    double  olf_tol;
+    double  hmin, delta, K0;
+    /* compatibility switches                                                           */
+    int32_t theta_carry;                          /* DHARA_THETA_CARRY_*                  */
+    int32_t dirichlet_face_quirk;                 /* 1: reference's opposite-face K in
+                                                     inner Dirichlet rows
+                                                     (src/subsurface.cu:223,259,294,328) */
+    int32_t top_bc;                               /* DHARA_TOP_*                          */
+    int32_t reserved0;
+} dhara_params;
+
+typedef struct dhara_step_stats {
+    int32_t picard_iters, picard_converged;
+    int32_t lin_iters;          /* Krylov iterations of the subsurface solves (all sweeps) */
+    int32_t lin_converged;      /* 1 if every linear solve of the step converged         */
+    int32_t refine_sweeps;      /* FP64 residual sweeps (DHARA_MIXED)                      */
+    int32_t olf_iters, olf_converged;
+    int32_t nan_seen;
+    double  picard_delta;       /* last Picard increment (rule units)                      */
+    double  mb_ref;             /* reference mass-balance formula, src/subsurface.cu:825 [m] */
+    double  storage_change;     /* sum of storage change over the domain [m^3]            */
+    double  boundary_inflow;    /* net inflow through all faces minus sinks, times dt [m^3] */
+    double  mb_rel;             /* |storage_change - boundary_inflow| / scale              */
+    double  outlet_discharge;   /* overland discharge into the x = 0 edge [m^3/s]          */
+    double  ring_outflow;       /* overland discharge into the whole Dirichlet ring [m^3/s] */
+    double  wall_ms;            /* wall time of the call                                   */
+} dhara_step_stats;
+
+typedef struct dhara_ctx dhara_ctx;
+
+/* ---- defaults ------------------------------------------------------------------------ */
+/* Parameter defaults: homogeneous loam (Carsel & Parrish), Picard 1e-6 m absolute,
+ * Jacobi-PCG to 1e-12, overland constants of include/devconst.h:25-28 in SI. */
+static inline void dhara_params_defaults(dhara_params* p)
+{
+    p->dt = 60.0;
+    p->theta_s = 0.43; p->theta_r = 0.078; p->alpha = 3.6; p->n_vg = 1.56;
+    p->Ss = 1e-5; p->porosity = 0.43;
+    p->picard_maxiter = 20; p->picard_rule = DHARA_PICARD_ABS_PSI;
+    p->picard_tol = 1e-6; p->am = 0.0;
+    p->lin_maxiter = 2000; p->precision = DHARA_FP64; p->lin_tol = 1e-12;
+    p->refine_maxiter = 8; p->inner_maxiter = 2000; p->inner_tol = 1e-5;
+    p->overland = 1; p->olf_maxiter = 2000; p->olf_tol = 1e-12;
+    p->hmin = 1e-5; p->delta = 1e-7; p->K0 = 1e-5;
+    p->theta_carry = DHARA_THETA_CARRY_CONSISTENT;
+    p->dirichlet_face_quirk = 1;
+    p->top_bc = DHARA_TOP_SURFACE;
+    p->reserved0 = 0;
+}
+
+/* ---- lifecycle ------------------------------------------------------------------------ */
+/* Replaces AllocateMemoryFlowModel's device half (src/numerical.cc:127-188) and the CUSP
+ * containers of ResizeCuspMemory (src/device.cu:27-72). */
+int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx** out);
+/* Replaces FreeDeviceMemory (src/numerical.cc:279-336). */
+int dhara_destroy(dhara_ctx* ctx);
+const char* dhara_last_error(const dhara_ctx* ctx);
+int dhara_abi_version(void);
+/* Writes "<device name> <gcnArchName> CUs=<n>" into buf. */
+int dhara_device_info(char* buf, size_t n);
+
+int dhara_set_params(dhara_ctx* ctx, const dhara_params* params);
+int dhara_get_params(const dhara_ctx* ctx, dhara_params* params);
+int dhara_set_precision(dhara_ctx* ctx, int32_t precision);
+
+/* ---- data movement ------------------------------------------------------------------- */
+/* Replace the cudaMemcpy H2D block of CopyDataToDevice (src/device.cu:223-321) and the
+ * root-density H2D of GatherRootDensityToMasterAndFilter (src/host.cc:256-257). */
+int dhara_upload_field(dhara_ctx* ctx, int32_t field, const double* host, size_t n);
+int dhara_upload_field_i32(dhara_ctx* ctx, int32_t field, const int32_t* host, size_t n);
+/* Replace SaveModelResults' D2H (src/io.cc:866-901) and SubsurfaceFlowModel's per-step
+ * psiout/thetaout copies (src/subsurface.cu:842-846; here only on request). */
+int dhara_download_field(dhara_ctx* ctx, int32_t field, double* host, size_t n);
+int dhara_download_field_i32(dhara_ctx* ctx, int32_t field, int32_t* host, size_t n);
+/* Device-to-device variants: `dev` is a device pointer of the same element type. */
+int dhara_upload_field_device(dhara_ctx* ctx, int32_t field, const void* dev, size_t n);
+int dhara_download_field_device(dhara_ctx* ctx, int32_t field, void* dev, size_t n);
+
+/* psi <-> theta initialisation: CopyDataToDevice's vanGenuchtenInverse
+ * (src/device.cu:323-330) followed by PreRunningFlowModel (src/global.cu:270-296). */
+int dhara_init_state(dhara_ctx* ctx, int32_t mode);
+
+/* ---- time stepping -------------------------------------------------------------------- */
+/* One subsurface step: SubsurfaceFlowModel (src/subsurface.cu:686-847). */
+int dhara_subsurface_step(dhara_ctx* ctx, dhara_step_stats* stats);
+/* One overland step: OverlandFlowModel (src/overland.cu:387-419). */
+int dhara_overland_step(dhara_ctx* ctx, dhara_step_stats* stats);
+/* nsteps coupled steps (subsurface then overland when params.overland != 0): the body of
+ * RunCoupledFlowModel's time loop (src/device.cu:426-469) without the canopy model.
+ * `stats` (may be NULL) receives nsteps records. */
+int dhara_step(dhara_ctx* ctx, int32_t nsteps, dhara_step_stats* stats);
+
+/* ---- kernel-level entry points used by the parity tests ------------------------------ */
+/* theta, K, C of the context's soil at the given psi (vanGenuchten, src/global.cu:66-105).
+ * All arrays have nx*ny*nz host doubles. */
+int dhara_eval_constitutive(dhara_ctx* ctx, const double* psi, double* theta, double* K,
+                            double* C);
+/* psi of the given theta (vanGenuchtenInverse, src/global.cu:32-52). */
+int dhara_eval_inverse(dhara_ctx* ctx, const double* theta, double* psi);
+/* Assembles the subsurface system at the current state the way the first Picard iteration
+ * of a step does (src/subsurface.cu:704-732) and returns its diagonal D and right-hand
+ * side b (either may be NULL). */
+int dhara_assemble_subsurface(dhara_ctx* ctx, double* diag, double* rhs);
+/* y = A x with the operator of the last dhara_assemble_subsurface / Picard iteration
+ * (the matrix-free equivalent of CUSP's DIA SpMV on a3d,
+ * cusp/system/cuda/detail/multiply/dia_spmv.h:60-111). */
+int dhara_apply_subsurface_operator(dhara_ctx* ctx, const double* x, double* y);
+/* Same operator evaluated in FP32 (K and D rounded to float, x rounded to float). */
+int dhara_apply_subsurface_operator_f32(dhara_ctx* ctx, const double* x, double* y);
+/* Overland face conductivities at the current ponding depth: ke[nx*ny] is the east face
+ * of each cell, kn[nx*ny] the north face (UpdateKOverland, src/overland.cu:193-368, seen
+ * from the non-ring cell of the face). */
+int dhara_overland_conductance(dhara_ctx* ctx, double* ke, double* kn);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DHARA_HIP_H */

@@ -1,0 +1,1220 @@
+/*
+ * dhara_oracle.cpp -- CPU FP64 restatement of Dhara's integrated flow path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see dhara_oracle.h).  Written from the reference's behaviour;
+ * every function cites the reference file:line it restates.  Units are SI (see
+ * include/dhara_hip.h); the reference's unit conversions (psi*100 with alpha in 1/cm,
+ * dt in hours, sec_p_mm2dt_p_m = 1.8) are folded into SI rates.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   constitutive (vG, inverse)   Decimal(50-digit) evaluation of the Mualem-van Genuchten
+ *                                closed form, theta(psi(theta)) round trip, C vs central
+ *                                finite difference of theta, psi >= 0 limits, monotonicity;
+ *                                the reference pow-form is checked against the same.
+ *   DIA layout / SpMV            CUSP testing/poisson.cu:7-60 (exact 6x6 matrix of
+ *                                poisson5pt(2,3)); dense matrix built column by column.
+ *   CG / BiCGStab / monitor      CUSP testing/cg.cu, bicgstab.cu (poisson5pt(10,10), b=1,
+ *                                20 iterations, |r| < 1e-4|b|; zero-residual start = 0
+ *                                iterations), testing/monitor.cu (tolerance = atol+rtol|b|).
+ *   Jacobi-PCG                   dense LU solve on <= 300 unknowns.
+ *   subsurface assembly + step   symmetry of the assembled matrix, row sums, laterally
+ *                                uniform 3D run == independent 1D Thomas column run,
+ *                                saturated linear steady state (closed form), hydrostatic
+ *                                stationarity, discrete mass conservation.
+ *   overland                     flat lake stays flat, dry stays dry, volume conservation
+ *                                against ring outflow, hand-computed K on a 3x3 stencil.
+ * Functions without such a check: none.
+ */
+#include "dhara_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+enum Face { FW = 0, FE, FS, FN, FT, FB };
+
+struct Soil {
+    double ts, tr, alpha, n, Ss, phi;
+};
+
+}  // namespace
+
+struct dor_ctx {
+    dhara_grid g;
+    dhara_params p;
+    int lin_solver, olf_solver;
+    size_t N, S;
+    bool initialized = false;
+    std::string err;
+
+    std::vector<double> psi_n, theta_n, ksat, alpha_f, n_f;
+    bool has_alpha = false, has_n = false;
+    std::vector<double> K, C, theta_m, theta_new, psi_m, x;
+    std::vector<double> ztopo, mann, h, H, rain, evap, transp, qtop, qbot, hpoten, qcapa;
+    std::vector<double> bcpsi[6], bcq[6];
+    std::vector<int> bctype[6];
+    std::vector<double> root;
+    std::vector<int> procmap;
+    std::vector<double> a3d, rhs3d;
+    std::vector<double> a2d, rhs2d;
+    std::vector<double> kw2, ke2, ks2, kn2;
+};
+
+namespace {
+
+size_t face_size(const dhara_grid& g, int f)
+{
+    switch (f) {
+    case FW: case FE: return (size_t)g.ny * g.nz;
+    case FS: case FN: return (size_t)g.nx * g.nz;
+    default: return (size_t)g.nx * g.ny;
+    }
+}
+
+int fail(dor_ctx* c, int code, const char* msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------------------
+// Mualem-van Genuchten.  src/global.cu:66-105 (vanGenuchten), SI: psi [m], alpha [1/m].
+// Evaluated in the numerically stable form x = alpha|psi|, t = x^n:
+//   Se = (1+t)^-m,  1 - Se^(1/m) = t/(1+t),  1 - (t/(1+t))^m = -expm1(-m log1p(1/t)),
+//   C = (ts-tr) (n-1) alpha x^(n-1) (1+t)^(1/n-2)
+// which is the same function as the reference's pow form without its cancellation near
+// saturation; dor_eval_constitutive_refform keeps the reference's exact expression.
+// ---------------------------------------------------------------------------------------
+inline void vg_stable(double psi, double Ks, double alpha, double n, double ts, double tr,
+                      double* theta, double* K, double* C)
+{
+    if (!(psi < 0.0)) {
+        if (std::isnan(psi)) { *theta = psi; *K = psi; *C = psi; return; }
+        *theta = ts; *K = Ks; *C = 0.0;
+        return;
+    }
+    const double m = (n - 1.0) / n;
+    const double x = alpha * (-psi);
+    const double lx = std::log(x);
+    const double t = std::exp(n * lx);
+    const double l1t = std::log1p(t);
+    const double Se = std::exp(-m * l1t);
+    *theta = (ts - tr) * Se + tr;
+    // 1 - (t/(1+t))^m ; for t == 0 this is 1.
+    const double w = (t > 0.0) ? -std::expm1(-m * std::log1p(1.0 / t)) : 1.0;
+    *K = Ks * std::sqrt(Se) * w * w;
+    // x^(n-1) (1+t)^(1/n-2) = exp((n-1) lx + (1/n - 2) l1t)
+    *C = (ts - tr) * (n - 1.0) * alpha * std::exp((n - 1.0) * lx + (1.0 / n - 2.0) * l1t);
+}
+
+// Reference expression, src/global.cu:75-99, with psi*100/alpha[1/cm] -> psi/alpha[1/m].
+inline void vg_refform(double psi, double Ks, double alpha, double n, double ts, double tr,
+                       double* theta, double* K, double* C)
+{
+    const double lambda = n - 1.0;
+    const double m = lambda / n;
+    double th;
+    if (psi < 0) th = (ts - tr) / std::pow(1.0 + std::pow(alpha * (-psi), n), m) + tr;
+    else th = ts;
+    *theta = th;
+    const double Se = (th - tr) / (ts - tr);
+    *K = Ks * std::sqrt(Se) * (1.0 - std::pow(1.0 - std::pow(Se, 1.0 / m), m)) *
+         (1.0 - std::pow(1.0 - std::pow(Se, 1.0 / m), m));
+    if (psi < 0)
+        *C = alpha * n * (1.0 / n - 1.0) * std::pow(alpha * std::fabs(psi), n - 1) * (tr - ts) *
+             std::pow(std::pow(alpha * std::fabs(psi), n) + 1, 1.0 / n - 2.0);
+    else
+        *C = 0.0;
+}
+
+// vanGenuchtenInverse, src/global.cu:32-52 (SI: no 0.01 factor).
+inline double vg_inverse(double theta, double alpha, double n, double ts, double tr)
+{
+    const double m = (n - 1.0) / n;
+    if (theta < ts)
+        return -(1.0 / alpha) * std::pow(std::pow((ts - tr) / (theta - tr), 1.0 / m) - 1.0, 1.0 / n);
+    return 0.0;
+}
+
+inline double cell_alpha(const dor_ctx& c, size_t i) { return c.has_alpha ? c.alpha_f[i] : c.p.alpha; }
+inline double cell_n(const dor_ctx& c, size_t i) { return c.has_n ? c.n_f[i] : c.p.n_vg; }
+
+void vg_field(dor_ctx& c, const double* psi, double* theta, double* K, double* C)
+{
+    const long N = (long)c.N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        double th, k, cc;
+        vg_stable(psi[i], c.ksat[i], cell_alpha(c, i), cell_n(c, i), c.p.theta_s, c.p.theta_r,
+                  &th, &k, &cc);
+        if (theta) theta[i] = th;
+        if (K) K[i] = k;
+        if (C) C[i] = cc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Generic DIA kernels (CUSP semantics).
+// ---------------------------------------------------------------------------------------
+void dia_spmv(long n, int nd, const int* off, const double* a, const double* x, double* y)
+{
+    // cusp/system/cuda/detail/multiply/dia_spmv.h:60-111: pitch = num_rows, a column
+    // index outside [0, n) is skipped.
+#pragma omp parallel for schedule(static)
+    for (long row = 0; row < n; ++row) {
+        double sum = 0.0;
+        for (int d = 0; d < nd; ++d) {
+            const long col = row + off[d];
+            if (col >= 0 && col < n) sum += a[(size_t)d * n + row] * x[col];
+        }
+        y[row] = sum;
+    }
+}
+
+double dot(long n, const double* a, const double* b)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+double nrm2(long n, const double* a) { return std::sqrt(dot(n, a, a)); }
+
+// CUSP monitor (cusp/detail/monitor.inl:28-200): tolerance = atol + rtol*|b|; finished()
+// recomputes |r| and is true when converged or iteration_count >= limit.
+struct Monitor {
+    double b_norm, r_norm, rtol, atol;
+    long limit, count = 0;
+    Monitor(double bn, long lim, double rt, double at) : b_norm(bn), r_norm(1e308), rtol(rt), atol(at), limit(lim) {}
+    double tol() const { return atol + rtol * b_norm; }
+    bool converged() const { return r_norm <= tol(); }
+    bool finished(long n, const double* r)
+    {
+        r_norm = nrm2(n, r);
+        return converged() || count >= limit;
+    }
+};
+
+// cusp::krylov::cg with identity preconditioner (cusp/krylov/detail/cg.inl:64-128).
+void cusp_cg(long n, int nd, const int* off, const double* a, const double* b, double* x,
+             Monitor& mon)
+{
+    std::vector<double> y(n), z(n), r(n), p(n);
+    dia_spmv(n, nd, off, a, x, y.data());
+    for (long i = 0; i < n; ++i) r[i] = b[i] - y[i];       // axpby(b, y, r, 1, -1)
+    z = r;                                                  // M = I
+    p = z;
+    double rz = dot(n, r.data(), z.data());
+    while (!mon.finished(n, r.data())) {
+        dia_spmv(n, nd, off, a, p.data(), y.data());
+        const double alpha = rz / dot(n, y.data(), p.data());
+        for (long i = 0; i < n; ++i) x[i] += alpha * p[i];
+        for (long i = 0; i < n; ++i) r[i] -= alpha * y[i];
+        z = r;
+        const double rz_old = rz;
+        rz = dot(n, r.data(), z.data());
+        const double beta = rz / rz_old;
+        for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        ++mon.count;
+    }
+}
+
+// cusp::krylov::bicgstab with identity preconditioner (bicgstab.inl:64-146).
+void cusp_bicgstab(long n, int nd, const int* off, const double* a, const double* b, double* x,
+                   Monitor& mon)
+{
+    std::vector<double> p(n), r(n), rs(n), s(n), AMp(n), AMs(n);
+    dia_spmv(n, nd, off, a, x, r.data());
+    for (long i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    p = r;
+    rs = r;
+    double rr_old = dot(n, rs.data(), r.data());
+    while (!mon.finished(n, r.data())) {
+        dia_spmv(n, nd, off, a, p.data(), AMp.data());
+        const double alpha = rr_old / dot(n, rs.data(), AMp.data());
+        for (long i = 0; i < n; ++i) s[i] = r[i] - alpha * AMp[i];
+        if (mon.finished(n, s.data())) {
+            for (long i = 0; i < n; ++i) x[i] += alpha * p[i];
+            break;
+        }
+        dia_spmv(n, nd, off, a, s.data(), AMs.data());
+        const double omega = dot(n, AMs.data(), s.data()) / dot(n, AMs.data(), AMs.data());
+        for (long i = 0; i < n; ++i) x[i] = x[i] + alpha * p[i] + omega * s[i];
+        for (long i = 0; i < n; ++i) r[i] = s[i] - omega * AMs[i];
+        const double rr_new = dot(n, rs.data(), r.data());
+        const double beta = (rr_new / rr_old) * (alpha / omega);
+        rr_old = rr_new;
+        for (long i = 0; i < n; ++i) p[i] = r[i] + beta * p[i] - beta * omega * AMp[i];
+        ++mon.count;
+    }
+}
+
+// Jacobi-preconditioned CG, the tight-mode solver.  Same iteration and stopping rule as the
+// device solver: r0 = b - A x0; stop when |r|_2 <= rtol |b|_2 (checked before each
+// iteration, CUSP monitor order) or after maxit iterations.
+void pcg_jacobi(long n, int nd, const int* off, const double* a, const double* b, double* x,
+                Monitor& mon, int diag_index)
+{
+    std::vector<double> r(n), z(n), p(n), q(n), dinv(n);
+    const double* D = a + (size_t)diag_index * n;
+    for (long i = 0; i < n; ++i) dinv[i] = 1.0 / D[i];
+    dia_spmv(n, nd, off, a, x, q.data());
+    for (long i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    for (long i = 0; i < n; ++i) z[i] = r[i] * dinv[i];
+    p = z;
+    double rz = dot(n, r.data(), z.data());
+    while (!mon.finished(n, r.data())) {
+        dia_spmv(n, nd, off, a, p.data(), q.data());
+        const double alpha = rz / dot(n, p.data(), q.data());
+        for (long i = 0; i < n; ++i) x[i] += alpha * p[i];
+        for (long i = 0; i < n; ++i) r[i] -= alpha * q[i];
+        for (long i = 0; i < n; ++i) z[i] = r[i] * dinv[i];
+        const double rz_new = dot(n, r.data(), z.data());
+        const double beta = rz_new / rz;
+        rz = rz_new;
+        ++mon.count;
+        if (mon.finished(n, r.data())) break;
+        for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+}
+
+int dense_solve(int n, double* A, const double* b, double* x)
+{
+    std::vector<double> rhs(b, b + n);
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; ++k) {
+        int pk = k;
+        double best = std::fabs(A[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[(size_t)i * n + k]) > best) { best = std::fabs(A[(size_t)i * n + k]); pk = i; }
+        if (best == 0.0) return DHARA_E_NUMERIC;
+        if (pk != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)pk * n + j]);
+            std::swap(rhs[k], rhs[pk]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+            if (f == 0.0) continue;
+            for (int j = k; j < n; ++j) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+            rhs[i] -= f * rhs[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = rhs[i];
+        for (int j = i + 1; j < n; ++j) s -= A[(size_t)i * n + j] * x[j];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+    return DHARA_OK;
+}
+
+int dia_dense_solve(long n, int nd, const int* off, const double* a, const double* b, double* x)
+{
+    if (n > 4000) return DHARA_E_UNSUPPORTED;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (long row = 0; row < n; ++row)
+        for (int d = 0; d < nd; ++d) {
+            const long col = row + off[d];
+            if (col >= 0 && col < n) A[(size_t)row * n + col] += a[(size_t)d * n + row];
+        }
+    return dense_solve((int)n, A.data(), b, x);
+}
+
+// ---------------------------------------------------------------------------------------
+// Subsurface.
+// ---------------------------------------------------------------------------------------
+
+// EstimateFluxes (src/global.cu:172-191) and IdentifyTopBoundary (src/global.cu:206-257).
+// Both depend only on quantities fixed within a step, so the top flux is the same in every
+// Picard iteration.  SI: hpoten = ph + rain*dt - evap*dt (the reference's
+// ppt/1000 - et*sec_p_mm2dt_p_m*dt with rates in mm).
+void surface_pre(dor_ctx& c)
+{
+    if (c.p.top_bc == DHARA_TOP_PRESCRIBED) return;
+    const long S = (long)c.S;
+    const double dt = c.p.dt, dz = c.g.dz, ts = c.p.theta_s;
+    for (long t = 0; t < S; ++t) {
+        const double hp = c.h[t] + c.rain[t] * dt - c.evap[t] * dt;
+        double qc = -c.K[t] * ((c.psi_n[t] - hp - 0.5 * dz) / (0.5 * dz));
+        if (c.h[t] / dt > c.ksat[t]) qc = std::max(qc, c.ksat[t]);
+        c.hpoten[t] = hp;
+        c.qcapa[t] = qc;
+        double q;
+        if (hp > 0.0) {
+            q = (hp / dt > qc) ? qc : hp / dt;
+            q = std::min(q, (ts - c.theta_n[t]) * dz / dt);
+            q = std::min(q, c.ksat[t]);
+        } else {
+            q = hp / dt;
+        }
+        c.bctype[FT][t] = DHARA_BC_FLUX;   // IdentifyTopBoundary always sets type 1
+        c.bcq[FT][t] = q;
+    }
+}
+
+// SetUpLinearSystemsSubsurface (src/subsurface.cu:381-429) = HydraulicConductivity-
+// AtInterface (26-79) + SetUpBlockMatrixSubsurface (83-102) + SetUpRightHandSide-
+// Subsurface (106-122) + SetUpBoundaryConditionsSubsurface (126-353), into DIA a3d[7][N]
+// with offsets [-nxny, -nx, -1, 0, +1, +nx, +nxny].
+void assemble_subsurface(dor_ctx& c)
+{
+    const int nx = c.g.nx, ny = c.g.ny, nz = c.g.nz;
+    const long nxy = (long)nx * ny, N = (long)c.N;
+    const double dx = c.g.dx, dy = c.g.dy, dz = c.g.dz, dt = c.p.dt;
+    const double Ss = c.p.Ss, poros = c.p.porosity;
+    const double dx2inv = -1.0 / (dx * dx), dy2inv = -1.0 / (dy * dy), dz2inv = -1.0 / (dz * dz);
+    const bool quirk = c.p.dirichlet_face_quirk != 0;
+    double* a = c.a3d.data();
+    double* rhs = c.rhs3d.data();
+    const double* K = c.K.data();
+#pragma omp parallel for schedule(static)
+    for (long tid = 0; tid < N; ++tid) {
+        const int k = (int)(tid / nxy);
+        const int j = (int)((tid % nxy) / nx);
+        const int i = (int)((tid % nxy) % nx);
+        const double kd = (k == 0) ? 0.0 : 0.5 * (K[tid] + K[tid - nxy]);
+        const double ku = (k == nz - 1) ? 0.0 : 0.5 * (K[tid] + K[tid + nxy]);
+        const double ks = (j == 0) ? 0.0 : 0.5 * (K[tid] + K[tid - nx]);
+        const double kn = (j == ny - 1) ? 0.0 : 0.5 * (K[tid] + K[tid + nx]);
+        const double kw = (i == 0) ? 0.0 : 0.5 * (K[tid] + K[tid - 1]);
+        const double ke = (i == nx - 1) ? 0.0 : 0.5 * (K[tid] + K[tid + 1]);
+        const double thm = c.theta_m[tid];
+        a[0 * N + tid] = dz2inv * kd;
+        a[1 * N + tid] = dy2inv * ks;
+        a[2 * N + tid] = dx2inv * kw;
+        a[3 * N + tid] = Ss / dt * thm / poros + c.C[tid] / dt - (ke + kw) * dx2inv -
+                         (kn + ks) * dy2inv - (ku + kd) * dz2inv;
+        a[4 * N + tid] = dx2inv * ke;
+        a[5 * N + tid] = dy2inv * kn;
+        a[6 * N + tid] = dz2inv * ku;
+
+        const long id2d = (long)j * nx + i;
+        const int grp = c.procmap[id2d];
+        const double tr = c.transp[id2d];
+        const double sink = (tr > 0.0) ? -tr * c.root[(size_t)grp * nz + k] / dz : 0.0;
+        double b = Ss / dt * thm / poros * c.psi_n[tid] + c.C[tid] / dt * c.psi_m[tid] -
+                   (ku - kd) / dz + (c.theta_n[tid] - thm) / dt + sink;
+
+        const long bi = (long)k * ny + j, bj = (long)k * nx + i, bk = (long)j * nx + i;
+        auto identity = [&](double v) {
+            for (int r = 0; r < 7; ++r) a[(size_t)r * N + tid] = (r == 3) ? 1.0 : 0.0;
+            b = v;
+        };
+        // top
+        if (c.bctype[FT][bk] == 0) {
+            if (k == 1) { a[0 * N + tid] = 0.0; b -= dz2inv * kd * c.bcpsi[FT][bk]; }
+            if (k == 0) identity(c.bcpsi[FT][bk]);
+        } else if (k == 0) {
+            b += (-c.bcq[FT][bk] * dz) * dz2inv;
+        }
+        // bottom (Neumann = free drainage, qb := K of the cell; src/subsurface.cu:204-211)
+        if (c.bctype[FB][bk] == 0) {
+            if (k == nz - 2) { a[6 * N + tid] = 0.0; b -= dz2inv * ku * c.bcpsi[FB][bk]; }
+            if (k == nz - 1) identity(c.bcpsi[FB][bk]);
+        } else if (k == nz - 1) {
+            c.bcq[FB][bk] = K[tid];
+            b += (c.bcq[FB][bk] * dz) * dz2inv;
+        }
+        // south
+        if (c.bctype[FS][bj] == 0) {
+            if (j == 1) { a[1 * N + tid] = 0.0; b -= dy2inv * (quirk ? kn : ks) * c.bcpsi[FS][bj]; }
+            if (j == 0) identity(c.bcpsi[FS][bj]);
+        } else if (j == 0) {
+            b += (-c.bcq[FS][bj] * dy) * dy2inv;
+        }
+        // north
+        if (c.bctype[FN][bj] == 0) {
+            if (j == ny - 2) { a[5 * N + tid] = 0.0; b -= dy2inv * (quirk ? ks : kn) * c.bcpsi[FN][bj]; }
+            if (j == ny - 1) identity(c.bcpsi[FN][bj]);
+        } else if (j == ny - 1) {
+            b += (c.bcq[FN][bj] * dy) * dy2inv;
+        }
+        // west
+        if (c.bctype[FW][bi] == 0) {
+            if (i == 1) { a[2 * N + tid] = 0.0; b -= dx2inv * (quirk ? ke : kw) * c.bcpsi[FW][bi]; }
+            if (i == 0) identity(c.bcpsi[FW][bi]);
+        } else if (i == 0) {
+            b += (-c.bcq[FW][bi] * dx) * dx2inv;
+        }
+        // east
+        if (c.bctype[FE][bi] == 0) {
+            if (i == nx - 2) { a[4 * N + tid] = 0.0; b -= dx2inv * (quirk ? kw : ke) * c.bcpsi[FE][bi]; }
+            if (i == nx - 1) identity(c.bcpsi[FE][bi]);
+        } else if (i == nx - 1) {
+            b += (c.bcq[FE][bi] * dx) * dx2inv;
+        }
+        rhs[tid] = b;
+    }
+}
+
+const int* offsets3d(const dor_ctx& c, int* off)
+{
+    const int nxy = c.g.nx * c.g.ny;
+    const int o[7] = {-nxy, -c.g.nx, -1, 0, 1, c.g.nx, nxy};
+    std::memcpy(off, o, sizeof(o));
+    return off;
+}
+
+int solve_subsurface(dor_ctx& c, int* iters, int* conv)
+{
+    int off[7];
+    offsets3d(c, off);
+    const long N = (long)c.N;
+    const double bn = nrm2(N, c.rhs3d.data());
+    Monitor mon(bn, c.p.lin_maxiter, c.p.lin_tol, 0.0);
+    switch (c.lin_solver) {
+    case DOR_LIN_PCG_JACOBI:
+        pcg_jacobi(N, 7, off, c.a3d.data(), c.rhs3d.data(), c.x.data(), mon, 3);
+        break;
+    case DOR_LIN_BICGSTAB_REF:
+        cusp_bicgstab(N, 7, off, c.a3d.data(), c.rhs3d.data(), c.x.data(), mon);
+        break;
+    case DOR_LIN_DENSE: {
+        int rc = dia_dense_solve(N, 7, off, c.a3d.data(), c.rhs3d.data(), c.x.data());
+        if (rc) return rc;
+        mon.count = 1; mon.r_norm = 0.0;
+        break;
+    }
+    default: return DHARA_E_ARG;
+    }
+    *iters = (int)mon.count;
+    *conv = mon.converged() ? 1 : 0;
+    return DHARA_OK;
+}
+
+// Mass balance.  mb_ref restates src/subsurface.cu:494-629 + 808-826 (WaterFluxEstimate and
+// the host formula) with interior faces telescoped; storage_change / boundary_inflow are
+// the same balance in m^3 over flux faces.
+void mass_balance(dor_ctx& c, dhara_step_stats* st)
+{
+    const int nx = c.g.nx, ny = c.g.ny, nz = c.g.nz;
+    const long nxy = (long)nx * ny, N = (long)c.N;
+    const double dx = c.g.dx, dy = c.g.dy, dz = c.g.dz, dt = c.p.dt;
+    const double Ss = c.p.Ss, poros = c.p.porosity;
+    double sdtheta = 0.0, sss = 0.0, ssink = 0.0;
+#pragma omp parallel for reduction(+ : sdtheta, sss, ssink) schedule(static)
+    for (long tid = 0; tid < N; ++tid) {
+        const int k = (int)(tid / nxy);
+        const long id2d = tid % nxy;
+        sdtheta += c.theta_new[tid] - c.theta_n[tid];
+        sss += (c.x[tid] - c.psi_n[tid]) * (Ss / dt * c.theta_new[tid] / poros) * dz * dt;
+        const double tr = c.transp[id2d];
+        if (tr > 0.0) ssink += tr * dt * c.root[(size_t)c.procmap[id2d] * nz + k];
+    }
+    double qt = 0.0, qb = 0.0, qw = 0.0, qe = 0.0, qs = 0.0, qn = 0.0;
+    for (long t = 0; t < nxy; ++t) {
+        if (c.bctype[FT][t] != 0) qt += c.bcq[FT][t];
+        if (c.bctype[FB][t] != 0) qb += c.bcq[FB][t];
+    }
+    for (long t = 0; t < (long)ny * nz; ++t) {
+        if (c.bctype[FW][t] != 0) qw += c.bcq[FW][t];
+        if (c.bctype[FE][t] != 0) qe += c.bcq[FE][t];
+    }
+    for (long t = 0; t < (long)nx * nz; ++t) {
+        if (c.bctype[FS][t] != 0) qs += c.bcq[FS][t];
+        if (c.bctype[FN][t] != 0) qn += c.bcq[FN][t];
+    }
+    // reference formula [m]: sum(dtheta)*dz + ((qe-qw)+(qn-qs)+(qu-qd))*dt - sum(transp)
+    //                        + sum(ssflux)*dt, transp = -TR*dt*root (negative)
+    st->mb_ref = sdtheta * dz + ((qe - qw) + (qn - qs) + (qb - qt)) * dt + ssink + sss;
+    st->storage_change = (sdtheta * dz + sss) * dx * dy;
+    const double inflow = (qt - qb) * dx * dy * dt + (qw - qe) * dy * dz * dt +
+                          (qs - qn) * dx * dz * dt - ssink * dx * dy;
+    st->boundary_inflow = inflow;
+    const double flux_scale = (std::fabs(qt) + std::fabs(qb)) * dx * dy * dt +
+                              (std::fabs(qw) + std::fabs(qe)) * dy * dz * dt +
+                              (std::fabs(qs) + std::fabs(qn)) * dx * dz * dt + ssink * dx * dy;
+    const double scale = std::max({std::fabs(st->storage_change), std::fabs(inflow), flux_scale, 1e-300});
+    st->mb_rel = std::fabs(st->storage_change - inflow) / scale;
+}
+
+int subsurface_step(dor_ctx& c, dhara_step_stats* st)
+{
+    // SubsurfaceFlowModel, src/subsurface.cu:686-847.
+    const long N = (long)c.N;
+    c.psi_m = c.psi_n;                                        // 704
+    surface_pre(c);                                           // 710, 724
+    int niter = 0, converged = 0, lin_total = 0, lin_ok = 1;
+    double delta = 0.0;
+    while (!converged && niter < c.p.picard_maxiter) {        // 715
+        vg_field(c, c.psi_m.data(), c.theta_m.data(), c.K.data(), c.C.data());   // 718
+        assemble_subsurface(c);                               // 732
+        int it = 0, cv = 0;
+        int rc = solve_subsurface(c, &it, &cv);               // 750-751
+        if (rc) return fail(&c, rc, "subsurface linear solve failed");
+        lin_total += it;
+        lin_ok &= cv;
+        vg_field(c, c.x.data(), c.theta_new.data(), c.K.data(), c.C.data());     // 754
+        ++niter;
+        // 762-780 (rule selectable; DHARA_PICARD_ABS_PSI is this build's default)
+        double mx = 0.0;
+        bool nan = false;
+        for (long i = 0; i < N; ++i) {
+            double d;
+            switch (c.p.picard_rule) {
+            case DHARA_PICARD_REL_PSI: d = std::fabs((c.x[i] - c.psi_m[i]) / c.x[i]); break;
+            case DHARA_PICARD_REL_THETA:
+                d = std::fabs((c.theta_new[i] - c.theta_m[i]) / c.theta_new[i]); break;
+            default: d = std::fabs(c.x[i] - c.psi_m[i]); break;
+            }
+            if (std::isnan(d)) nan = true;
+            mx = std::max(mx, d);   // thrust::max_element
+        }
+        if (nan) { st->nan_seen = 1; mx = NAN; }
+        delta = mx;
+        if (mx < (c.p.picard_rule == DHARA_PICARD_REL_THETA ? c.p.picard_tol : c.p.picard_tol)) {
+            converged = 1;
+        } else {
+            const double am = c.p.am;                          // ModifiedPicardUpdate 450-461
+            for (long i = 0; i < N; ++i) c.psi_m[i] = am * c.psi_m[i] + (1 - am) * c.x[i];
+        }
+    }
+    st->picard_iters = niter;
+    st->picard_converged = converged;
+    st->lin_iters = lin_total;
+    st->lin_converged = lin_ok;
+    st->picard_delta = delta;
+    // SubsurfaceEstimateInfiltrationPonding, src/subsurface.cu:465-491 (top is always a flux
+    // boundary after IdentifyTopBoundary): qss = qt, ph = max(hpoten - qss*dt, 0).
+    for (long t = 0; t < (long)c.S; ++t) {
+        c.qtop[t] = (c.bctype[FT][t] != 0) ? c.bcq[FT][t] : 0.0;
+        if (c.p.top_bc != DHARA_TOP_PRESCRIBED)
+            c.h[t] = std::max(c.hpoten[t] - c.bcq[FT][t] * c.p.dt, 0.0);
+    }
+    mass_balance(c, st);
+    for (long t = 0; t < (long)c.S; ++t) c.qbot[t] = c.bcq[FB][t];
+    // 836-840: psi^n <- psi^{n+1,m+1}; theta^n <- theta^{n+1,m} (reference) or theta(psi^n+1)
+    c.psi_n = c.x;
+    if (c.p.theta_carry == DHARA_THETA_CARRY_REFERENCE) c.theta_n = c.theta_m;
+    else c.theta_n = c.theta_new;
+    for (long i = 0; i < N; ++i)
+        if (!std::isfinite(c.psi_n[i])) { st->nan_seen = 1; break; }
+    return DHARA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Overland.
+// ---------------------------------------------------------------------------------------
+
+// UpdateKOverland, src/overland.cu:193-368, per cell (the reference's own view of each of
+// its four faces).
+void overland_k(dor_ctx& c)
+{
+    const int N = c.g.nx, M = c.g.ny;
+    const double dx = c.g.dx, dy = c.g.dy;
+    const double hmin = c.p.hmin, delta = c.p.delta, K0 = c.p.K0;
+    const double* Hs = c.H.data();
+    const double* h = c.h.data();
+    const double* mann = c.mann.data();
+    auto face = [&](double Hnb, double Hc, double hnb, double hc, double mnb, double mc, double Ssl) {
+        const double hup = (Hnb > Hc) ? hnb : hc;
+        if (hup > hmin && (hnb + hc) / 2. > hmin && std::fabs(Ssl) > delta)
+            return std::pow((hnb + hc) / 2., 5. / 3.) / (0.5 * (mnb + mc) * std::sqrt(Ssl));
+        return K0;
+    };
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < N; ++i) {
+            const long I = (long)j * N + i;
+            const double h_ = h[I], Hs_ = Hs[I], m_ = mann[I];
+            double h_p1, Hs_p1, m_p1, h_m1, Hs_m1, m_m1, h_pN, Hs_pN, m_pN, h_mN, Hs_mN, m_mN;
+            if (i == N - 1) { h_p1 = h_; Hs_p1 = Hs_; m_p1 = m_; }
+            else { h_p1 = h[I + 1]; Hs_p1 = Hs[I + 1]; m_p1 = mann[I + 1]; }
+            if (i == 0) { h_m1 = h_; Hs_m1 = Hs_; m_m1 = m_; }
+            else { h_m1 = h[I - 1]; Hs_m1 = Hs[I - 1]; m_m1 = mann[I - 1]; }
+            if (j == M - 1) { h_pN = h_; Hs_pN = Hs_; m_pN = m_; }
+            else { h_pN = h[I + N]; Hs_pN = Hs[I + N]; m_pN = mann[I + N]; }
+            if (j == 0) { h_mN = h_; Hs_mN = Hs_; m_mN = m_; }
+            else { h_mN = h[I - N]; Hs_mN = Hs[I - N]; m_mN = mann[I - N]; }
+            double Hs_pNp1 = Hs[I], Hs_pNm1 = Hs[I], Hs_mNp1 = Hs[I], Hs_mNm1 = Hs[I];
+            if (i > 0 && i < N - 1 && j > 0 && j < M - 1) {
+                Hs_pNp1 = Hs[I + N + 1]; Hs_pNm1 = Hs[I + N - 1];
+                Hs_mNp1 = Hs[I - N + 1]; Hs_mNm1 = Hs[I - N - 1];
+            }
+            const double Sse = std::sqrt(std::pow((Hs_p1 - Hs_) / dx, 2.) +
+                                         std::pow((Hs_pNp1 + Hs_pN - Hs_mNp1 - Hs_mN) / (4 * dy), 2.));
+            c.ke2[I] = face(Hs_p1, Hs_, h_p1, h_, m_p1, m_, Sse);
+            const double Ssw = std::sqrt(std::pow((Hs_ - Hs_m1) / dx, 2.) +
+                                         std::pow((Hs_pN + Hs_pNm1 - Hs_mN - Hs_mNm1) / (4 * dy), 2.));
+            c.kw2[I] = face(Hs_m1, Hs_, h_m1, h_, m_m1, m_, Ssw);
+            const double Ssn = std::sqrt(std::pow((Hs_pN - Hs_) / dy, 2.) +
+                                         std::pow((Hs_pNp1 + Hs_p1 - Hs_pNm1 - Hs_m1) / (4 * dx), 2.));
+            c.kn2[I] = face(Hs_pN, Hs_, h_pN, h_, m_pN, m_, Ssn);
+            const double Sss = std::sqrt(std::pow((Hs_ - Hs_mN) / dy, 2.) +
+                                         std::pow((Hs_p1 + Hs_mNp1 - Hs_m1 - Hs_mNm1) / (4 * dx), 2.));
+            c.ks2[I] = face(Hs_mN, Hs_, h_mN, h_, m_mN, m_, Sss);
+        }
+}
+
+// SetUpLinearSystemsOverland, src/overland.cu:65-189 (ppt = et = infil = 0, line 403-406).
+// DIA offsets [-nx, -1, 0, +1, +nx]; outer ring rows are identity with rhs = H.
+void assemble_overland(dor_ctx& c)
+{
+    const int nx = c.g.nx, ny = c.g.ny;
+    const long S = (long)c.S;
+    const double dt = c.p.dt;
+    const double dtodx2 = -dt / (c.g.dx * c.g.dx), dtody2 = -dt / (c.g.dy * c.g.dy);
+    double* a = c.a2d.data();
+    for (long tid = 0; tid < S; ++tid) {
+        const int i = (int)(tid % nx), j = (int)(tid / nx);
+        c.rhs2d[tid] = c.H[tid] + 0.0 + 0.0 + 0.0;
+        if (i == nx - 1) c.ke2[tid] = 0;
+        if (i == 0) c.kw2[tid] = 0;
+        if (j == ny - 1) c.kn2[tid] = 0;
+        if (j == 0) c.ks2[tid] = 0;
+        a[0 * S + tid] = dtody2 * c.ks2[tid];
+        a[1 * S + tid] = dtodx2 * c.kw2[tid];
+        a[2 * S + tid] = 1.0 - dtodx2 * (c.ke2[tid] + c.kw2[tid]) - dtody2 * (c.kn2[tid] + c.ks2[tid]);
+        a[3 * S + tid] = dtodx2 * c.ke2[tid];
+        a[4 * S + tid] = dtody2 * c.kn2[tid];
+        if (j == 0 || j == ny - 1 || i == 0 || i == nx - 1) {
+            for (int d = 0; d < 5; ++d) a[(size_t)d * S + tid] = 0.0;
+            a[2 * S + tid] = 1.0;
+            c.rhs2d[tid] = c.H[tid] + 0.0;
+        }
+    }
+}
+
+int overland_step(dor_ctx& c, dhara_step_stats* st)
+{
+    // OverlandFlowModel, src/overland.cu:387-419.
+    const int nx = c.g.nx, ny = c.g.ny;
+    const long S = (long)c.S;
+    for (long t = 0; t < S; ++t) c.H[t] = c.h[t] + c.ztopo[t];        // 394 (33-45)
+    overland_k(c);                                                      // 398
+    assemble_overland(c);                                               // 403
+    const int off[5] = {-nx, -1, 0, 1, nx};
+    std::vector<double> a = c.a2d, b = c.rhs2d;
+    const double bn = nrm2(S, b.data());
+    Monitor mon(bn, c.p.olf_maxiter, c.p.olf_tol, 0.0);
+    if (c.olf_solver == DOR_OLF_CG_REF) {
+        cusp_cg(S, 5, off, a.data(), b.data(), c.H.data(), mon);          // 410-411
+    } else {
+        // Eliminate the Dirichlet ring from the interior rows: same solution, and the
+        // remaining matrix is symmetric (up to exact ties of H in the upwind choice).
+        for (long tid = 0; tid < S; ++tid) {
+            const int i = (int)(tid % nx), j = (int)(tid / nx);
+            if (j == 0 || j == ny - 1 || i == 0 || i == nx - 1) continue;
+            for (int d = 0; d < 5; ++d) {
+                if (d == 2) continue;
+                const long col = tid + off[d];
+                const int ci = (int)(col % nx), cj = (int)(col / nx);
+                if (cj == 0 || cj == ny - 1 || ci == 0 || ci == nx - 1) {
+                    b[tid] -= a[(size_t)d * S + tid] * c.H[col];
+                    a[(size_t)d * S + tid] = 0.0;
+                }
+            }
+        }
+        if (c.olf_solver == DOR_OLF_DENSE) {
+            int rc = dia_dense_solve(S, 5, off, a.data(), b.data(), c.H.data());
+            if (rc) return fail(&c, rc, "overland dense solve failed");
+            mon.count = 1; mon.r_norm = 0.0;
+        } else {
+            pcg_jacobi(S, 5, off, a.data(), b.data(), c.H.data(), mon, 2);
+        }
+    }
+    st->olf_iters = (int)mon.count;
+    st->olf_converged = mon.converged() ? 1 : 0;
+    // Discharge into the Dirichlet ring (new H, the row's own face conductance).
+    const double dx = c.g.dx, dy = c.g.dy;
+    double qx0 = 0.0, qring = 0.0;
+    for (int j = 1; j < ny - 1; ++j)
+        for (int i = 1; i < nx - 1; ++i) {
+            const long I = (long)j * nx + i;
+            const double Hc = c.H[I];
+            if (i == 1) { const double q = c.kw2[I] * (Hc - c.H[I - 1]) / dx * dy; qx0 += q; qring += q; }
+            if (i == nx - 2) qring += c.ke2[I] * (Hc - c.H[I + 1]) / dx * dy;
+            if (j == 1) qring += c.ks2[I] * (Hc - c.H[I - nx]) / dy * dx;
+            if (j == ny - 2) qring += c.kn2[I] * (Hc - c.H[I + nx]) / dy * dx;
+        }
+    st->outlet_discharge = qx0;
+    st->ring_outflow = qring;
+    for (long t = 0; t < S; ++t) c.h[t] = std::max(c.H[t] - c.ztopo[t], 0.0);   // 371-383
+    return DHARA_OK;
+}
+
+std::vector<double>* dfield(dor_ctx& c, int f, size_t* n)
+{
+    const size_t N = c.N, S = c.S;
+    switch (f) {
+    case DHARA_F_PSI: *n = N; return &c.psi_n;
+    case DHARA_F_THETA: *n = N; return &c.theta_n;
+    case DHARA_F_KSAT: *n = N; return &c.ksat;
+    case DHARA_F_ALPHA: *n = N; return &c.alpha_f;
+    case DHARA_F_NVG: *n = N; return &c.n_f;
+    case DHARA_F_K: *n = N; return &c.K;
+    case DHARA_F_C: *n = N; return &c.C;
+    case DHARA_F_ZTOPO: *n = S; return &c.ztopo;
+    case DHARA_F_MANNING: *n = S; return &c.mann;
+    case DHARA_F_PONDING: *n = S; return &c.h;
+    case DHARA_F_WATERELEV: *n = S; return &c.H;
+    case DHARA_F_RAIN: *n = S; return &c.rain;
+    case DHARA_F_EVAP: *n = S; return &c.evap;
+    case DHARA_F_TRANSP: *n = S; return &c.transp;
+    case DHARA_F_QTOP: *n = S; return &c.qtop;
+    case DHARA_F_QBOT: *n = S; return &c.qbot;
+    case DHARA_F_HPOTEN: *n = S; return &c.hpoten;
+    case DHARA_F_QCAPA: *n = S; return &c.qcapa;
+    case DHARA_F_ROOT: *n = (size_t)c.g.nroot_groups * c.g.nz; return &c.root;
+    default: break;
+    }
+    if (f >= DHARA_F_BCPSI_W && f <= DHARA_F_BCPSI_B) {
+        *n = face_size(c.g, f - DHARA_F_BCPSI_W);
+        return &c.bcpsi[f - DHARA_F_BCPSI_W];
+    }
+    if (f >= DHARA_F_BCQ_W && f <= DHARA_F_BCQ_B) {
+        *n = face_size(c.g, f - DHARA_F_BCQ_W);
+        return &c.bcq[f - DHARA_F_BCQ_W];
+    }
+    return nullptr;
+}
+
+std::vector<int>* ifield(dor_ctx& c, int f, size_t* n)
+{
+    if (f >= DHARA_F_BCTYPE_W && f <= DHARA_F_BCTYPE_B) {
+        *n = face_size(c.g, f - DHARA_F_BCTYPE_W);
+        return &c.bctype[f - DHARA_F_BCTYPE_W];
+    }
+    if (f == DHARA_F_PROCMAP) { *n = c.S; return &c.procmap; }
+    return nullptr;
+}
+
+}  // namespace
+
+// =========================================================================================
+// C-ABI
+// =========================================================================================
+extern "C" {
+
+int dor_set_threads(int32_t nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    return omp_get_max_threads();
+#else
+    (void)nthreads;
+    return 1;
+#endif
+}
+
+int dor_create(const dhara_grid* grid, const dhara_params* params, int32_t lin_solver,
+               int32_t olf_solver, dor_ctx** out)
+{
+    if (!grid || !params || !out) return DHARA_E_ARG;
+    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || grid->nroot_groups < 1) return DHARA_E_ARG;
+    dor_ctx* c = new dor_ctx;
+    c->g = *grid;
+    c->p = *params;
+    c->lin_solver = lin_solver;
+    c->olf_solver = olf_solver;
+    c->N = (size_t)grid->nx * grid->ny * grid->nz;
+    c->S = (size_t)grid->nx * grid->ny;
+    const size_t N = c->N, S = c->S;
+    for (auto* v : {&c->psi_n, &c->theta_n, &c->ksat, &c->K, &c->C, &c->theta_m, &c->theta_new,
+                    &c->psi_m, &c->x, &c->rhs3d})
+        v->assign(N, 0.0);
+    c->a3d.assign(7 * N, 0.0);
+    for (auto* v : {&c->ztopo, &c->h, &c->H, &c->rain, &c->evap, &c->transp, &c->qtop,
+                    &c->qbot, &c->hpoten, &c->qcapa, &c->rhs2d, &c->kw2, &c->ke2, &c->ks2, &c->kn2})
+        v->assign(S, 0.0);
+    c->mann.assign(S, 0.03);
+    c->a2d.assign(5 * S, 0.0);
+    for (int f = 0; f < 6; ++f) {
+        const size_t fs = face_size(*grid, f);
+        c->bcpsi[f].assign(fs, 0.0);
+        c->bcq[f].assign(fs, 0.0);
+        c->bctype[f].assign(fs, DHARA_BC_FLUX);
+    }
+    c->root.assign((size_t)grid->nroot_groups * grid->nz, 0.0);
+    c->procmap.assign(S, 0);
+    *out = c;
+    return DHARA_OK;
+}
+
+int dor_destroy(dor_ctx* c)
+{
+    delete c;
+    return DHARA_OK;
+}
+
+const char* dor_last_error(const dor_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int dor_set_params(dor_ctx* c, const dhara_params* p)
+{
+    if (!c || !p) return DHARA_E_ARG;
+    c->p = *p;
+    return DHARA_OK;
+}
+
+int dor_upload_field(dor_ctx* c, int32_t f, const double* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    std::vector<double>* v = dfield(*c, f, &want);
+    if (!v) return fail(c, DHARA_E_ARG, "unknown double field");
+    if (n != want) return fail(c, DHARA_E_ARG, "field size mismatch");
+    v->assign(host, host + n);
+    if (f == DHARA_F_ALPHA) c->has_alpha = true;
+    if (f == DHARA_F_NVG) c->has_n = true;
+    return DHARA_OK;
+}
+
+int dor_upload_field_i32(dor_ctx* c, int32_t f, const int32_t* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    std::vector<int>* v = ifield(*c, f, &want);
+    if (!v) return fail(c, DHARA_E_ARG, "unknown int field");
+    if (n != want) return fail(c, DHARA_E_ARG, "field size mismatch");
+    if (f == DHARA_F_PROCMAP)
+        for (size_t i = 0; i < n; ++i)
+            if (host[i] < 0 || host[i] >= c->g.nroot_groups) return fail(c, DHARA_E_ARG, "procmap out of range");
+    v->assign(host, host + n);
+    return DHARA_OK;
+}
+
+int dor_download_field(dor_ctx* c, int32_t f, double* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    std::vector<double>* v = dfield(*c, f, &want);
+    if (!v) return fail(c, DHARA_E_ARG, "unknown double field");
+    if (n != want) return fail(c, DHARA_E_ARG, "field size mismatch");
+    if (f == DHARA_F_ALPHA && !c->has_alpha) std::fill(host, host + n, c->p.alpha);
+    else if (f == DHARA_F_NVG && !c->has_n) std::fill(host, host + n, c->p.n_vg);
+    else std::memcpy(host, v->data(), n * sizeof(double));
+    return DHARA_OK;
+}
+
+int dor_download_field_i32(dor_ctx* c, int32_t f, int32_t* host, size_t n)
+{
+    if (!c || !host) return DHARA_E_ARG;
+    size_t want = 0;
+    std::vector<int>* v = ifield(*c, f, &want);
+    if (!v) return fail(c, DHARA_E_ARG, "unknown int field");
+    if (n != want) return fail(c, DHARA_E_ARG, "field size mismatch");
+    std::memcpy(host, v->data(), n * sizeof(int32_t));
+    return DHARA_OK;
+}
+
+int dor_init_state(dor_ctx* c, int32_t mode)
+{
+    if (!c) return DHARA_E_ARG;
+    const long N = (long)c->N;
+    if (mode == DHARA_INIT_FROM_THETA) {
+        // CopyDataToDevice: vanGenuchtenInverse(thetan -> psin), src/device.cu:323-330
+        for (long i = 0; i < N; ++i)
+            c->psi_n[i] = vg_inverse(c->theta_n[i], cell_alpha(*c, i), cell_n(*c, i), c->p.theta_s, c->p.theta_r);
+    } else if (mode != DHARA_INIT_FROM_PSI) {
+        return fail(c, DHARA_E_ARG, "bad init mode");
+    }
+    // PreRunningFlowModel: vanGenuchten(psin -> thetan, K, C), src/global.cu:289-291
+    vg_field(*c, c->psi_n.data(), c->theta_n.data(), c->K.data(), c->C.data());
+    c->psi_m = c->psi_n;
+    c->x = c->psi_n;
+    c->theta_m = c->theta_n;
+    c->theta_new = c->theta_n;
+    c->initialized = true;
+    return DHARA_OK;
+}
+
+int dor_subsurface_step(dor_ctx* c, dhara_step_stats* st)
+{
+    if (!c) return DHARA_E_ARG;
+    if (!c->initialized) return fail(c, DHARA_E_STATE, "dor_init_state not called");
+    dhara_step_stats tmp;
+    std::memset(&tmp, 0, sizeof(tmp));
+    int rc = subsurface_step(*c, &tmp);
+    if (st) {
+        st->picard_iters = tmp.picard_iters; st->picard_converged = tmp.picard_converged;
+        st->lin_iters = tmp.lin_iters; st->lin_converged = tmp.lin_converged;
+        st->refine_sweeps = 0; st->nan_seen = tmp.nan_seen; st->picard_delta = tmp.picard_delta;
+        st->mb_ref = tmp.mb_ref; st->storage_change = tmp.storage_change;
+        st->boundary_inflow = tmp.boundary_inflow; st->mb_rel = tmp.mb_rel;
+    }
+    return rc;
+}
+
+int dor_overland_step(dor_ctx* c, dhara_step_stats* st)
+{
+    if (!c) return DHARA_E_ARG;
+    dhara_step_stats tmp;
+    std::memset(&tmp, 0, sizeof(tmp));
+    int rc = overland_step(*c, &tmp);
+    if (st) {
+        st->olf_iters = tmp.olf_iters; st->olf_converged = tmp.olf_converged;
+        st->outlet_discharge = tmp.outlet_discharge; st->ring_outflow = tmp.ring_outflow;
+    }
+    return rc;
+}
+
+int dor_step(dor_ctx* c, int32_t nsteps, dhara_step_stats* st)
+{
+    if (!c) return DHARA_E_ARG;
+    if (!c->initialized) return fail(c, DHARA_E_STATE, "dor_init_state not called");
+    for (int s = 0; s < nsteps; ++s) {
+        dhara_step_stats tmp;
+        std::memset(&tmp, 0, sizeof(tmp));
+        int rc = dor_subsurface_step(c, &tmp);
+        if (rc) return rc;
+        if (c->p.overland) {
+            rc = dor_overland_step(c, &tmp);
+            if (rc) return rc;
+        }
+        if (st) st[s] = tmp;
+    }
+    return DHARA_OK;
+}
+
+int dor_eval_constitutive(dor_ctx* c, const double* psi, double* theta, double* K, double* C)
+{
+    if (!c || !psi) return DHARA_E_ARG;
+    vg_field(*c, psi, theta, K, C);
+    return DHARA_OK;
+}
+
+int dor_eval_constitutive_refform(dor_ctx* c, const double* psi, double* theta, double* K,
+                                  double* C)
+{
+    if (!c || !psi) return DHARA_E_ARG;
+    for (size_t i = 0; i < c->N; ++i) {
+        double th, k, cc;
+        vg_refform(psi[i], c->ksat[i], cell_alpha(*c, i), cell_n(*c, i), c->p.theta_s,
+                   c->p.theta_r, &th, &k, &cc);
+        if (theta) theta[i] = th;
+        if (K) K[i] = k;
+        if (C) C[i] = cc;
+    }
+    return DHARA_OK;
+}
+
+int dor_eval_inverse(dor_ctx* c, const double* theta, double* psi)
+{
+    if (!c || !theta || !psi) return DHARA_E_ARG;
+    for (size_t i = 0; i < c->N; ++i)
+        psi[i] = vg_inverse(theta[i], cell_alpha(*c, i), cell_n(*c, i), c->p.theta_s, c->p.theta_r);
+    return DHARA_OK;
+}
+
+int dor_assemble_subsurface(dor_ctx* c, double* a3d, double* diag, double* rhs)
+{
+    if (!c) return DHARA_E_ARG;
+    if (!c->initialized) return fail(c, DHARA_E_STATE, "dor_init_state not called");
+    c->psi_m = c->psi_n;
+    surface_pre(*c);
+    vg_field(*c, c->psi_m.data(), c->theta_m.data(), c->K.data(), c->C.data());
+    assemble_subsurface(*c);
+    if (a3d) std::memcpy(a3d, c->a3d.data(), 7 * c->N * sizeof(double));
+    if (diag) std::memcpy(diag, c->a3d.data() + 3 * c->N, c->N * sizeof(double));
+    if (rhs) std::memcpy(rhs, c->rhs3d.data(), c->N * sizeof(double));
+    return DHARA_OK;
+}
+
+int dor_apply_subsurface_operator(dor_ctx* c, const double* x, double* y)
+{
+    if (!c || !x || !y) return DHARA_E_ARG;
+    int off[7];
+    offsets3d(*c, off);
+    dia_spmv((long)c->N, 7, off, c->a3d.data(), x, y);
+    return DHARA_OK;
+}
+
+int dor_overland_conductance(dor_ctx* c, double* ke, double* kn, double* kw, double* ks)
+{
+    if (!c) return DHARA_E_ARG;
+    for (size_t t = 0; t < c->S; ++t) c->H[t] = c->h[t] + c->ztopo[t];
+    overland_k(*c);
+    if (ke) std::memcpy(ke, c->ke2.data(), c->S * sizeof(double));
+    if (kn) std::memcpy(kn, c->kn2.data(), c->S * sizeof(double));
+    if (kw) std::memcpy(kw, c->kw2.data(), c->S * sizeof(double));
+    if (ks) std::memcpy(ks, c->ks2.data(), c->S * sizeof(double));
+    return DHARA_OK;
+}
+
+int dor_assemble_overland(dor_ctx* c, double* a2d, double* rhs)
+{
+    if (!c) return DHARA_E_ARG;
+    for (size_t t = 0; t < c->S; ++t) c->H[t] = c->h[t] + c->ztopo[t];
+    overland_k(*c);
+    assemble_overland(*c);
+    if (a2d) std::memcpy(a2d, c->a2d.data(), 5 * c->S * sizeof(double));
+    if (rhs) std::memcpy(rhs, c->rhs2d.data(), c->S * sizeof(double));
+    return DHARA_OK;
+}
+
+// ---- CUSP-semantics building blocks ----------------------------------------------------
+
+int dor_poisson5pt(int32_t m, int32_t n, int32_t* offsets, double* values)
+{
+    // stencil (0,-1),(-1,0),(0,0),(1,0),(0,1) with values -1,-1,4,-1,-1 on an m x n grid,
+    // strides (1, m): cusp/gallery/detail/poisson.inl:28-43, stencil.inl:143-186.
+    if (m < 1 || n < 1 || !offsets || !values) return DHARA_E_ARG;
+    const int st[5][2] = {{0, -1}, {-1, 0}, {0, 0}, {1, 0}, {0, 1}};
+    const double v[5] = {-1, -1, 4, -1, -1};
+    const long rows = (long)m * n;
+    for (int d = 0; d < 5; ++d) {
+        offsets[d] = st[d][0] + st[d][1] * m;
+        for (long r = 0; r < rows; ++r) {
+            const int i = (int)(r % m), j = (int)(r / m);
+            const int ii = i + st[d][0], jj = j + st[d][1];
+            values[(size_t)d * rows + r] = (ii >= 0 && ii < m && jj >= 0 && jj < n) ? v[d] : 0.0;
+        }
+    }
+    return DHARA_OK;
+}
+
+int dor_poisson7pt(int32_t m, int32_t n, int32_t k, int32_t* offsets, double* values)
+{
+    if (m < 1 || n < 1 || k < 1 || !offsets || !values) return DHARA_E_ARG;
+    const int st[7][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const double v[7] = {-1, -1, -1, 6, -1, -1, -1};
+    const long rows = (long)m * n * k;
+    for (int d = 0; d < 7; ++d) {
+        offsets[d] = st[d][0] + st[d][1] * m + st[d][2] * m * n;
+        for (long r = 0; r < rows; ++r) {
+            const int i = (int)(r % m), j = (int)((r / m) % n), l = (int)(r / ((long)m * n));
+            const int ii = i + st[d][0], jj = j + st[d][1], ll = l + st[d][2];
+            values[(size_t)d * rows + r] =
+                (ii >= 0 && ii < m && jj >= 0 && jj < n && ll >= 0 && ll < k) ? v[d] : 0.0;
+        }
+    }
+    return DHARA_OK;
+}
+
+int dor_dia_spmv(int32_t nrows, int32_t ndiag, const int32_t* offsets, const double* values,
+                 const double* x, double* y)
+{
+    if (nrows < 0 || ndiag < 1 || !offsets || !values || !x || !y) return DHARA_E_ARG;
+    dia_spmv(nrows, ndiag, offsets, values, x, y);
+    return DHARA_OK;
+}
+
+int dor_dia_cg(int32_t nrows, int32_t ndiag, const int32_t* offsets, const double* values,
+               const double* b, double* x, int32_t maxit, double rtol, double atol,
+               int32_t* iters, double* rnorm, int32_t* converged)
+{
+    if (nrows < 1 || !offsets || !values || !b || !x) return DHARA_E_ARG;
+    Monitor mon(nrm2(nrows, b), maxit, rtol, atol);
+    cusp_cg(nrows, ndiag, offsets, values, b, x, mon);
+    if (iters) *iters = (int)mon.count;
+    if (rnorm) *rnorm = mon.r_norm;
+    if (converged) *converged = mon.converged();
+    return DHARA_OK;
+}
+
+int dor_dia_bicgstab(int32_t nrows, int32_t ndiag, const int32_t* offsets,
+                     const double* values, const double* b, double* x, int32_t maxit,
+                     double rtol, double atol, int32_t* iters, double* rnorm,
+                     int32_t* converged)
+{
+    if (nrows < 1 || !offsets || !values || !b || !x) return DHARA_E_ARG;
+    Monitor mon(nrm2(nrows, b), maxit, rtol, atol);
+    cusp_bicgstab(nrows, ndiag, offsets, values, b, x, mon);
+    if (iters) *iters = (int)mon.count;
+    if (rnorm) *rnorm = mon.r_norm;
+    if (converged) *converged = mon.converged();
+    return DHARA_OK;
+}
+
+int dor_dia_pcg_jacobi(int32_t nrows, int32_t ndiag, const int32_t* offsets,
+                       const double* values, const double* b, double* x, int32_t maxit,
+                       double rtol, int32_t* iters, double* rnorm, int32_t* converged)
+{
+    if (nrows < 1 || !offsets || !values || !b || !x) return DHARA_E_ARG;
+    int di = -1;
+    for (int d = 0; d < ndiag; ++d)
+        if (offsets[d] == 0) di = d;
+    if (di < 0) return DHARA_E_ARG;
+    Monitor mon(nrm2(nrows, b), maxit, rtol, 0.0);
+    pcg_jacobi(nrows, ndiag, offsets, values, b, x, mon, di);
+    if (iters) *iters = (int)mon.count;
+    if (rnorm) *rnorm = mon.r_norm;
+    if (converged) *converged = mon.converged();
+    return DHARA_OK;
+}
+
+int dor_dense_solve(int32_t n, double* A, const double* b, double* x)
+{
+    if (n < 1 || !A || !b || !x) return DHARA_E_ARG;
+    return dense_solve(n, A, b, x);
+}
+
+int dor_thomas(int32_t n, const double* a, const double* b, const double* c, const double* d,
+               double* x)
+{
+    if (n < 1 || !a || !b || !c || !d || !x) return DHARA_E_ARG;
+    std::vector<double> cp(n), dp(n);
+    cp[0] = c[0] / b[0];
+    dp[0] = d[0] / b[0];
+    for (int i = 1; i < n; ++i) {
+        const double m = b[i] - a[i] * cp[i - 1];
+        cp[i] = c[i] / m;
+        dp[i] = (d[i] - a[i] * dp[i - 1]) / m;
+    }
+    x[n - 1] = dp[n - 1];
+    for (int i = n - 2; i >= 0; --i) x[i] = dp[i] - cp[i] * x[i + 1];
+    return DHARA_OK;
+}
+
+int dor_column_run(int32_t nz, double dz, const dhara_params* p, double Ks, double qtop,
+                   int32_t nsteps, double* psi, double* theta_out, int32_t* picard_iters)
+{
+    // Independent restatement for the column-reduction check: the mixed-form modified
+    // Picard step of one soil column written directly as a tridiagonal system
+    //   [Ss th^m/(phi dt) + C^m/dt] psi^{m+1}_k - d/dz(K_f d psi^{m+1}/dz)
+    //      = Ss th^m/(phi dt) psi^n + C^m/dt psi^m - dK/dz + (th^n - th^m)/dt
+    // with flux top (+qtop/dz) and free-drainage bottom (-K_bottom/dz), z down.
+    if (nz < 2 || !p || !psi) return DHARA_E_ARG;
+    const double dt = p->dt, Ss = p->Ss, phi = p->porosity;
+    std::vector<double> thn(nz), thm(nz), thnew(nz), Km(nz), Cm(nz), pm(nz), pn(nz), xn(nz);
+    std::vector<double> lo(nz), di(nz), up(nz), rh(nz);
+    for (int k = 0; k < nz; ++k) {
+        double K, C;
+        vg_stable(psi[k], Ks, p->alpha, p->n_vg, p->theta_s, p->theta_r, &thn[k], &K, &C);
+    }
+    int total = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        for (int k = 0; k < nz; ++k) { pn[k] = psi[k]; pm[k] = psi[k]; }
+        for (int it = 0; it < p->picard_maxiter; ++it) {
+            for (int k = 0; k < nz; ++k)
+                vg_stable(pm[k], Ks, p->alpha, p->n_vg, p->theta_s, p->theta_r, &thm[k], &Km[k], &Cm[k]);
+            for (int k = 0; k < nz; ++k) {
+                const double kup = (k > 0) ? 0.5 * (Km[k] + Km[k - 1]) : 0.0;        // face toward top
+                const double kdn = (k < nz - 1) ? 0.5 * (Km[k] + Km[k + 1]) : 0.0;   // face toward bottom
+                const double st = Ss * thm[k] / (phi * dt) + Cm[k] / dt;
+                lo[k] = -kup / (dz * dz);
+                up[k] = -kdn / (dz * dz);
+                di[k] = st + (kup + kdn) / (dz * dz);
+                rh[k] = Ss * thm[k] / (phi * dt) * pn[k] + Cm[k] / dt * pm[k] - (kdn - kup) / dz +
+                        (thn[k] - thm[k]) / dt;
+                if (k == 0) rh[k] += qtop / dz;
+                if (k == nz - 1) rh[k] -= Km[k] / dz;
+            }
+            dor_thomas(nz, lo.data(), di.data(), up.data(), rh.data(), xn.data());
+            double mx = 0.0;
+            for (int k = 0; k < nz; ++k) mx = std::max(mx, std::fabs(xn[k] - pm[k]));
+            for (int k = 0; k < nz; ++k) pm[k] = xn[k];
+            ++total;
+            if (mx < p->picard_tol) break;
+        }
+        for (int k = 0; k < nz; ++k) {
+            psi[k] = pm[k];
+            double K, C;
+            vg_stable(psi[k], Ks, p->alpha, p->n_vg, p->theta_s, p->theta_r, &thnew[k], &K, &C);
+            thn[k] = thnew[k];
+        }
+    }
+    if (theta_out) for (int k = 0; k < nz; ++k) theta_out[k] = thn[k];
+    if (picard_iters) *picard_iters = total;
+    return DHARA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+"""GPU parity: the HIP library against the CPU oracle on the same seeded inputs.
+
+Tolerances (FP64 path): constitutive and operator values <= 1e-12 relative; pressure head
+<= 1e-8 m after every step; relative mass-balance error < 1e-10 per converged step.  Mixed
+precision: head error <= 1e-6 m against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from dhara_amd import _abi, inputs
+
+pytestmark = pytest.mark.gpu
+
+HEAD_TOL = 1e-8
+
+
+def cp(p):
+    return _abi.Params.from_buffer_copy(p)
+
+
+def pair(hip, orc, w, **overrides):
+    p = cp(w.params)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    g = hip.FlowModel(w.grid, cp(p))
+    o = orc.OracleModel(w.grid, cp(p))
+    w.apply(g)
+    w.apply(o)
+    return g, o
+
+
+def relerr(a, b, floor=1e-300):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))
+
+
+def test_device_present(hip):
+    info = hip.device_info()
+    print(info, flush=True)
+    assert "gfx9" in info
+
+
+def test_constitutive_matches_oracle(hip, orc):
+    g = _abi.Grid(nx=16, ny=8, nz=8, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
+    n = 16 * 8 * 8
+    rng = np.random.default_rng(11)
+    psi = -np.exp(rng.uniform(np.log(1e-6), np.log(100.0), n))
+    psi[:8] = [0.0, 0.5, -0.0, 1e-12, -1e-12, -1e-9, -1e3, 2.0]
+    fields = {"KSAT": np.exp(rng.uniform(np.log(1e-7), np.log(1e-4), n)),
+              "ALPHA": rng.uniform(2.0, 5.0, n), "NVG": rng.uniform(1.3, 1.8, n)}
+    gm = hip.FlowModel(g, _abi.default_params())
+    om = orc.OracleModel(g, _abi.default_params())
+    gm.upload_many(fields)
+    om.upload_many(fields)
+    tg, kg, cg = gm.eval_constitutive(psi)
+    to, ko, co = om.eval_constitutive(psi)
+    assert relerr(tg, to) < 1e-13
+    assert relerr(kg, ko, 1e-300) < 1e-12
+    assert relerr(cg, co, 1e-300) < 1e-12
+    th = rng.uniform(0.1, 0.429, n)
+    assert relerr(gm.eval_inverse(th), om.eval_inverse(th)) < 1e-12
+
+
+def _hetero_c1(nx=12, ny=10, nz=16, seed=3):
+    w = inputs.c1_column(nx=nx, ny=ny, nz=nz)
+    rng = np.random.default_rng(seed)
+    w.fields["KSAT"] = inputs.KS_LOAM * np.exp(rng.normal(0.0, 1.0, nx * ny * nz))
+    return w
+
+
+def test_assembly_and_operator_match_oracle(hip, orc):
+    w = _hetero_c1()
+    gm, om = pair(hip, orc, w)
+    dg, bg = gm.assemble()
+    a3d, do, bo = om.assemble()
+    assert relerr(dg, do) < 1e-13
+    assert np.max(np.abs(bg - bo)) <= 1e-13 * np.max(np.abs(bo))
+    x = np.random.default_rng(5).standard_normal(w.ncells)
+    yg = gm.apply_operator(x)
+    yo = om.apply_operator(x)
+    assert np.max(np.abs(yg - yo)) <= 1e-12 * np.max(np.abs(yo))
+    y32 = gm.apply_operator_f32(x)
+    assert np.max(np.abs(y32 - yo)) <= 1e-5 * np.max(np.abs(yo))
+
+
+@pytest.mark.parametrize("nsteps", [12])
+def test_c1_subsurface_steps(hip, orc, nsteps):
+    w = _hetero_c1()
+    gm, om = pair(hip, orc, w)
+    for s in range(nsteps):
+        sg = gm.step(1)[0]
+        so = om.step(1)[0]
+        dpsi = np.max(np.abs(gm.download("PSI") - om.download("PSI")))
+        print(f"step {s}: picard {sg['picard_iters']}/{so['picard_iters']} "
+              f"lin {sg['lin_iters']}/{so['lin_iters']} dpsi {dpsi:.3e} mb {sg['mb_rel']:.2e}",
+              flush=True)
+        assert dpsi <= HEAD_TOL
+        assert sg["picard_iters"] == so["picard_iters"]
+        assert abs(sg["lin_iters"] - so["lin_iters"]) <= sg["picard_iters"]
+        assert sg["mb_rel"] < 1e-10
+        assert abs(sg["mb_rel"] - so["mb_rel"]) < 1e-10
+    assert np.max(np.abs(gm.download("THETA") - om.download("THETA"))) < 1e-9
+
+
+def test_c2_small_coupled(hip, orc):
+    w = inputs.c2_coupled(nx=24, ny=20, nz=30)
+    gm, om = pair(hip, orc, w)
+    for s in range(8):
+        sg = w.run(gm, 1, start=s)[0]
+        so = w.run(om, 1, start=s)[0]
+        dpsi = np.max(np.abs(gm.download("PSI") - om.download("PSI")))
+        dh = np.max(np.abs(gm.download("PONDING") - om.download("PONDING")))
+        print(f"step {s}: picard {sg['picard_iters']}/{so['picard_iters']} "
+              f"lin {sg['lin_iters']}/{so['lin_iters']} olf {sg['olf_iters']}/{so['olf_iters']} "
+              f"dpsi {dpsi:.3e} dh {dh:.3e} q {sg['outlet_discharge']:.4e}/{so['outlet_discharge']:.4e}",
+              flush=True)
+        assert dpsi <= HEAD_TOL
+        assert dh <= HEAD_TOL
+        assert abs(sg["outlet_discharge"] - so["outlet_discharge"]) <= 1e-8 * max(1e-12, abs(so["outlet_discharge"]))
+
+
+def test_overland_conductance_matches_oracle(hip, orc):
+    w = inputs.c2_coupled(nx=17, ny=13, nz=4)
+    rng = np.random.default_rng(9)
+    h = np.where(rng.uniform(size=17 * 13) < 0.6, rng.uniform(0, 0.05, 17 * 13), 0.0)
+    w.fields["PONDING"] = h
+    gm, om = pair(hip, orc, w)
+    ke, kn = gm.overland_conductance()
+    oke, okn, okw, oks = om.overland_conductance()
+    nx, ny = 17, 13
+    for j in range(1, ny - 1):
+        for i in range(0, nx - 1):
+            c = j * nx + i
+            want = oke[c] if 0 < i < nx - 1 else okw[c + 1]
+            assert abs(ke[c] - want) <= 1e-13 * abs(want), (i, j)
+    for j in range(0, ny - 1):
+        for i in range(1, nx - 1):
+            c = j * nx + i
+            want = okn[c] if 0 < j < ny - 1 else oks[c + nx]
+            assert abs(kn[c] - want) <= 1e-13 * abs(want), (i, j)
+
+
+def test_mixed_precision_head_error(hip, orc):
+    w = inputs.c2_coupled(nx=24, ny=20, nz=30)
+    gm, om = pair(hip, orc, w, precision=_abi.MIXED)
+    sg = w.run(gm, 8)
+    w.run(om, 8)
+    err = np.max(np.abs(gm.download("PSI") - om.download("PSI")))
+    print("mixed max head error", err, "sweeps", [s["refine_sweeps"] for s in sg], flush=True)
+    assert err <= 1e-6
+
+
+def test_dirichlet_faces_and_reference_switches(hip, orc):
+    w = _hetero_c1(nx=10, ny=9, nz=12, seed=7)
+    nx, ny, nz = 10, 9, 12
+    w.fields["BCTYPE_W"] = np.zeros(ny * nz, np.int32)
+    w.fields["BCPSI_W"] = np.full(ny * nz, -0.3)
+    w.fields["BCTYPE_N"] = np.zeros(nx * nz, np.int32)
+    w.fields["BCPSI_N"] = np.full(nx * nz, -0.6)
+    for carry, rule in [(_abi.THETA_CARRY_REFERENCE, _abi.PICARD_REL_PSI),
+                        (_abi.THETA_CARRY_CONSISTENT, _abi.PICARD_REL_THETA)]:
+        gm, om = pair(hip, orc, w, theta_carry=carry, picard_rule=rule, picard_tol=1e-7)
+        for s in range(5):
+            sg = gm.step(1)[0]
+            so = om.step(1)[0]
+            dpsi = np.max(np.abs(gm.download("PSI") - om.download("PSI")))
+            print(f"carry {carry} rule {rule} step {s}: picard {sg['picard_iters']}/{so['picard_iters']}"
+                  f" dpsi {dpsi:.3e}", flush=True)
+            assert dpsi <= HEAD_TOL
+            assert sg["picard_iters"] == so["picard_iters"]
