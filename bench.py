@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled surface/subsurface solver on one MI355X.
+
+Workload (default): the 400x400x30 coupled run -- 1% slope + GRF topography, log-normal Ks,
+20 mm/h rain for 3600 s then 3600 s recession, dt = 30 s, 240 steps -- solved twice from the
+same synthetic inputs:
+  * plain FP64 (FP64 Jacobi-PCG), and
+  * mixed precision (FP32 Jacobi-PCG inner solves + FP64 residual refinement).
+A step is one coupled time step (subsurface Picard solve + overland solve) over the whole
+grid.  `value` is the mixed-precision throughput in 3D cell-steps/s; the FP64 throughput,
+the speed-up, and the head error of the mixed run against the FP64 run are in the same line.
+
+The dominant kernel's roofline comes from HIP events recorded by the library on its own
+stream around every Krylov launch of the timed region.  The CPU baseline is the oracle
+(FP64 C++/OpenMP restatement of the reference algorithm) timed on a bounded window of the
+same workload, started from the GPU FP64 state at that step.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s and %HBM roofline, mixed vs FP64 at ≤1e-6 m head error"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=400)
+    ap.add_argument("--ny", type=int, default=400)
+    ap.add_argument("--nz", type=int, default=30)
+    ap.add_argument("--cpu-start", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
+    return ap.parse_args()
+
+
+def counted_bytes(stats, n3, n2, mixed):
+    """Algorithmic HBM bytes of the steps (per-kernel byte model of the solver)."""
+    w = 4 if mixed else 8
+    total = 0.0
+    for s in stats:
+        P, I, R, Io = s["picard_iters"], s["lin_iters"], s["refine_sweeps"], s["olf_iters"]
+        vg = 56 + (4 if mixed else 0)                  # read x, psi_m, Ks; write theta, K, C, psi_m
+        asm = 80 if not mixed else 84                  # read 7 fields; write D, b, r (or D, b, D32, r32, d32)
+        per_cell = P * (vg + asm) + I * 13 * w + R * 64 + 48
+        per_surf = 13 * 8 * Io + 200                   # overland PCG + surface/overland passes
+        total += n3 * per_cell + n2 * per_surf
+    return total
+
+
+def run_timed(model, w, steps, marks=()):
+    """Run `steps` steps; return (stats, seconds, snapshots at the requested step indices)."""
+    snaps = {}
+    bounds = sorted({0, steps, *[m for m in marks if 0 < m < steps]})
+    stats = []
+    elapsed = 0.0
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        if a in marks:
+            snaps[a] = (model.download("PSI"), model.download("PONDING"))
+        t0 = time.perf_counter()
+        stats += w.run(model, b - a, start=a)       # every library call returns synchronised
+        elapsed += time.perf_counter() - t0
+    if steps in marks:
+        snaps[steps] = (model.download("PSI"), model.download("PONDING"))
+    return stats, elapsed, snaps
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "bench.py measures one GPU; multi-GPU runs are not part of "
+                                   "this build (the domain decomposition is a design note)"}))
+        sys.exit(2)
+    import dhara_amd
+    from dhara_amd import _abi, inputs
+
+    dhara_amd.load_hip_library()            # raises without the HIP build; no fallback
+    dev = dhara_amd.device_info()
+    print(f"# device: {dev}", file=sys.stderr, flush=True)
+    K, W = args.steps, args.warmup
+    nx, ny, nz = args.nx, args.ny, args.nz
+    n3, n2 = nx * ny * nz, nx * ny
+
+    def cp(p):
+        return _abi.Params.from_buffer_copy(p)
+
+    w64 = inputs.c2_coupled(nx=nx, ny=ny, nz=nz)
+    w32 = inputs.c3_coupled_mixed(nx=nx, ny=ny, nz=nz)
+    runs = {}
+    order = [("fp64", w64), ("mixed", w32)]
+    if args.only != "both":
+        order = [o for o in order if o[0] == args.only]
+    cpu_start = min(args.cpu_start, max(K - args.cpu_steps, 0))
+    cpu_marks = (cpu_start, cpu_start + args.cpu_steps)
+    for name, w in order:
+        if W > 0:                                  # warm-up on a throw-away context
+            m = dhara_amd.FlowModel(w.grid, cp(w.params))
+            w.apply(m)
+            w.run(m, W)
+            m.close()
+        m = dhara_amd.FlowModel(w.grid, cp(w.params))
+        w.apply(m)
+        m.profile(True)
+        print(f"# {name}: {K} steps", file=sys.stderr, flush=True)
+        stats, secs, snaps = run_timed(m, w, K, marks=cpu_marks if name == "fp64" else ())
+        prof = m.profile_read()
+        psi = m.download("PSI")
+        h = m.download("PONDING")
+        m.close()
+        runs[name] = dict(stats=stats, secs=secs, snaps=snaps, prof=prof, psi=psi, h=h)
+        print(f"# {name}: {secs:.3f} s, {n3 * K / secs:.4e} cell-steps/s, picard "
+              f"{sum(s['picard_iters'] for s in stats)}, krylov {sum(s['lin_iters'] for s in stats)}",
+              file=sys.stderr, flush=True)
+
+    head = "mixed" if "mixed" in runs else "fp64"
+    r = runs[head]
+    value = n3 * K / r["secs"]
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": 1e3 * r["secs"] / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic (seeded GRF topography and log-normal Ks; no external data)",
+        "config": {"workload": f"coupled {nx}x{ny}x{nz} (surface {nx}x{ny}), 20 mm/h rain 3600 s + "
+                               f"3600 s recession, dt 30 s; {head} run timed over steps 0..{K - 1}",
+                   "precision": head, "device": dev},
+    }
+    for name, rr in runs.items():
+        st = rr["stats"]
+        conv = [s for s in st if s["picard_converged"]]
+        out[name] = {
+            "cell_steps_per_s": n3 * K / rr["secs"], "seconds": rr["secs"],
+            "picard_iters": sum(s["picard_iters"] for s in st),
+            "krylov_iters": sum(s["lin_iters"] for s in st),
+            "refine_sweeps": sum(s["refine_sweeps"] for s in st),
+            "overland_iters": sum(s["olf_iters"] for s in st),
+            "picard_unconverged_steps": sum(1 for s in st if not s["picard_converged"]),
+            "max_mb_rel_converged_steps": max((s["mb_rel"] for s in conv), default=None),
+            "counted_hbm_bytes": counted_bytes(st, n3, n2, name == "mixed"),
+        }
+        out[name]["step_hbm_frac"] = out[name]["counted_hbm_bytes"] / (rr["secs"] * HBM_PEAK)
+    if "fp64" in runs and "mixed" in runs:
+        d = runs["mixed"]["psi"] - runs["fp64"]["psi"]
+        out["mixed_over_fp64"] = runs["fp64"]["secs"] / runs["mixed"]["secs"]
+        out["head_error_vs_fp64"] = {"max_m": float(np.max(np.abs(d))),
+                                     "rms_m": float(np.sqrt(np.mean(d * d))),
+                                     "ponding_max_m": float(np.max(np.abs(runs["mixed"]["h"] - runs["fp64"]["h"])))}
+    # roofline of the dominant kernel, from the library's HIP-event timing
+    pr = r["prof"]
+    kern = []
+    for kname, key in (("pcg3_spmv", "spmv"), ("pcg_update", "update")):
+        nl = pr[f"{key}_launches"]
+        if nl:
+            avg = pr[f"{key}_ms"] / nl * 1e-3
+            kern.append((pr[f"{key}_ms"], kname, avg, pr[f"{key}_bytes"], nl))
+    if kern:
+        kern.sort(reverse=True)
+        tot, kname, avg, byts, nl = kern[0]
+        prec = "f32" if pr["elem_bytes"] == 4 else "f64"
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            try:
+                traffic = json.load(open(tpath)).get(f"k_pcg3_spmv<{prec}>", None) if kname == "pcg3_spmv" else None
+            except Exception:
+                traffic = None
+        out["roofline"] = {"bound": "hbm", "kernel": f"k_{kname}<{prec}>", "achieved": byts / avg / 1e9,
+                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": byts / avg / HBM_PEAK,
+                           "traffic": traffic, "avg_us": avg * 1e6, "launches": nl,
+                           "bytes_per_launch": byts}
+        out["roofline_other_kernels"] = [
+            {"kernel": f"k_{k[1]}<{prec}>", "achieved": k[3] / k[2] / 1e9, "frac": k[3] / k[2] / HBM_PEAK,
+             "avg_us": k[2] * 1e6, "launches": k[4]} for k in kern[1:]]
+    # CPU baseline: oracle on a bounded window, from the GPU FP64 state
+    if not args.no_cpu and "fp64" in runs and cpu_start in runs["fp64"]["snaps"]:
+        from oracle import oracle
+        threads = oracle.set_threads(0)
+        om = oracle.OracleModel(w64.grid, cp(w64.params))
+        om.upload_many(w64.fields)
+        psi0, h0 = runs["fp64"]["snaps"][cpu_start]
+        om.upload("PSI", psi0)
+        om.upload("PONDING", h0)
+        om.init_state(_abi.INIT_FROM_PSI)
+        print(f"# cpu oracle: steps {cpu_start}..{cpu_start + args.cpu_steps - 1} on {threads} threads",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        w64.run(om, args.cpu_steps, start=cpu_start)
+        cs = time.perf_counter() - t0
+        gpu_win = sum(s["wall_ms"] for s in runs["fp64"]["stats"][cpu_start:cpu_start + args.cpu_steps]) * 1e-3
+        psi_c = om.download("PSI")
+        psi_g = runs["fp64"]["snaps"].get(cpu_start + args.cpu_steps, (None,))[0]
+        out["cpu_baseline"] = {
+            "value": n3 * args.cpu_steps / cs, "unit": "cell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FP64 (C++/OpenMP) on steps {cpu_start}..{cpu_start + args.cpu_steps - 1} of the "
+                      f"{nx}x{ny}x{nz} coupled workload, started from the GPU FP64 state",
+            "seconds": cs, "gpu_fp64_same_steps_s": gpu_win,
+            "gpu_fp64_speedup_same_steps": cs / gpu_win if gpu_win > 0 else None,
+            "max_head_diff_vs_gpu_fp64_m": float(np.max(np.abs(psi_c - psi_g))) if psi_g is not None else None,
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -78,6 +78,17 @@ class StepStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class KernelProfile(C.Structure):
+    _fields_ = [
+        ("spmv_launches", C.c_int64), ("spmv_ms", C.c_double), ("spmv_bytes", C.c_double),
+        ("update_launches", C.c_int64), ("update_ms", C.c_double), ("update_bytes", C.c_double),
+        ("elem_bytes", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def default_params(**overrides):
     """Same values as dhara_params_defaults() in include/dhara_hip.h."""
     p = Params(

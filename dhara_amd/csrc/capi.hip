@@ -72,7 +72,15 @@ struct dhara_ctx {
     bool theta_m_current = true;
     int pcg_hint = 8, olf_hint = 8;
     std::vector<void*> allocs;
+    // kernel profiling (dhara_profile_enable)
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    dhara_kernel_profile prof_acc{};
 };
+
+namespace {
+constexpr int kMaxBatch = 64;   // Krylov iterations queued between two host checks
+}
 
 namespace {
 
@@ -284,24 +292,48 @@ dh::OlfArgs olf_args(dhara_ctx* c)
 }
 
 // Queue Krylov iterations in batches; the kernels stop doing work once the device-side
-// state says the solve is done, so over-queued iterations cost one launch each.
-template <class IterFn>
-int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, IterFn iter)
+// state says the solve is done, so over-queued iterations cost one launch each.  With
+// profiling on, events bracket each SpMV and update launch of the batch and only the
+// launches that did work (the first `done` ones of the batch) are accumulated.
+template <class SpmvFn, class UpdFn>
+int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
+               SpmvFn spmv, UpdFn upd)
 {
-    int parity = 0, launched = 0;
-    int batch = std::max(4, hint);
+    int parity = 0, launched = 0, iter_before = 0;
+    int batch = std::max(4, std::min(kMaxBatch, hint));
     for (;;) {
         for (int b = 0; b < batch; ++b) {
-            iter(parity);
+            if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b], c->s));
+            spmv(parity);
+            if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b + 1], c->s));
+            upd(parity);
+            if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b + 2], c->s));
             parity ^= 1;
         }
         LAUNCHCHK(c);
         launched += batch;
         int rc = read_state(c, st);
         if (rc) return rc;
+        if (prof) {
+            const int real = std::max(0, std::min(batch, c->h_st->iter - iter_before));
+            for (int b = 0; b < real; ++b) {
+                float m0 = 0.f, m1 = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&m0, c->ev[3 * b], c->ev[3 * b + 1]));
+                HIPCHK(c, hipEventElapsedTime(&m1, c->ev[3 * b + 1], c->ev[3 * b + 2]));
+                c->prof_acc.spmv_ms += m0;
+                c->prof_acc.update_ms += m1;
+            }
+            c->prof_acc.spmv_launches += real;
+            c->prof_acc.update_launches += real;
+            c->prof_acc.spmv_bytes = 6.0 * elem * c->N;
+            c->prof_acc.update_bytes = 7.0 * elem * c->N;
+            c->prof_acc.elem_bytes = (int)elem;
+        }
+        iter_before = c->h_st->iter;
         if (c->h_st->done) break;
         if (launched > maxit + 4) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
-        batch = std::max(4, std::min(16, (hint - launched) > 0 ? (hint - launched) : 8));
+        const int left = hint - launched;
+        batch = std::max(4, std::min(16, left > 0 ? left : 8));
     }
     hint = std::max(1, c->h_st->iter);
     return DHARA_OK;
@@ -313,10 +345,13 @@ int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
     dh::launch_assemble(c->s, asm_args(c, dir), false);
     LAUNCHCHK(c);
     double* const pp[2] = {c->pv[0], c->pv[1]};
-    int rc = run_krylov(c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, [&](int par) {
-        dh::launch_pcg3_iter_f64(c->s, c->g, dir ? c->flags : nullptr, c->K, c->D, c->x, c->r, pp,
-                                 c->q, par, c->st_sub, red_of(c));
-    });
+    const uint8_t* fl = dir ? c->flags : nullptr;
+    int rc = run_krylov(
+        c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, c->prof, 8.0,
+        [&](int par) { dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, par, c->st_sub, red_of(c)); },
+        [&](int par) {
+            dh::launch_pcg_update_f64(c->s, c->N, c->x, c->r, c->D, pp[(par + 1) & 1], c->q, c->st_sub, red_of(c));
+        });
     if (rc) return rc;
     st->lin_iters += c->h_st->iter;
     st->lin_converged &= c->h_st->conv;
@@ -349,10 +384,17 @@ int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
     if (rc) return rc;
     bool outer_conv = c->h_st->outer_conv != 0;
     while (!outer_conv && sweeps < c->p.refine_maxiter) {
-        rc = run_krylov(c, c->st_sub, c->p.inner_maxiter, c->pcg_hint, [&](int par) {
-            dh::launch_pcg3_iter_f32(c->s, c->g, dir ? c->flags : nullptr, c->K32, c->D32, c->d32,
-                                     c->r32, pp, c->q32, par, c->st_sub, red_of(c));
-        });
+        const uint8_t* fl = dir ? c->flags : nullptr;
+        rc = run_krylov(
+            c, c->st_sub, c->p.inner_maxiter, c->pcg_hint, c->prof, 4.0,
+            [&](int par) {
+                dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, par, c->st_sub,
+                                         red_of(c));
+            },
+            [&](int par) {
+                dh::launch_pcg_update_f32(c->s, c->N, c->d32, c->r32, c->D32, pp[(par + 1) & 1], c->q32,
+                                          c->st_sub, red_of(c));
+            });
         if (rc) return rc;
         st->lin_iters += c->h_st->iter;
         ++sweeps;
@@ -475,10 +517,14 @@ int do_overland_step(dhara_ctx* c, dhara_step_stats* st)
     dh::launch_olf_assemble(c->s, a);
     LAUNCHCHK(c);
     double* const pp[2] = {c->p2[0], c->p2[1]};
-    int rc = run_krylov(c, c->st_olf, c->p.olf_maxiter, c->olf_hint, [&](int par) {
-        dh::launch_pcg2_iter(c->s, c->g2, c->ke, c->kn, c->D2, c->H, c->r2, pp, c->q2, par, c->st_olf,
-                             red_of(c));
-    });
+    // launch_pcg2_iter queues both halves of the iteration; the second callback is empty
+    int rc = run_krylov(
+        c, c->st_olf, c->p.olf_maxiter, c->olf_hint, false, 8.0,
+        [&](int par) {
+            dh::launch_pcg2_iter(c->s, c->g2, c->ke, c->kn, c->D2, c->H, c->r2, pp, c->q2, par, c->st_olf,
+                                 red_of(c));
+        },
+        [&](int) {});
     if (rc) return rc;
     st->olf_iters = c->h_st->iter;
     st->olf_converged = c->h_st->conv;
@@ -651,6 +697,7 @@ int dhara_destroy(dhara_ctx* c)
 {
     if (!c) return DHARA_OK;
     if (c->s) (void)hipStreamSynchronize(c->s);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void* v : c->allocs) (void)hipFree(v);
     if (c->h_st) (void)hipHostFree(c->h_st);
     if (c->h_pst) (void)hipHostFree(c->h_pst);
@@ -855,6 +902,26 @@ int dhara_step(dhara_ctx* c, int32_t nsteps, dhara_step_stats* st)
         if (st) st[s] = tmp;
         if (rc) return rc;
     }
+    return DHARA_OK;
+}
+
+int dhara_profile_enable(dhara_ctx* c, int32_t on)
+{
+    if (!c) return DHARA_E_ARG;
+    if (on && c->ev.empty()) {
+        c->ev.resize(3 * kMaxBatch);
+        for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+    }
+    c->prof = on != 0;
+    std::memset(&c->prof_acc, 0, sizeof(c->prof_acc));
+    return DHARA_OK;
+}
+
+int dhara_profile_read(dhara_ctx* c, dhara_kernel_profile* out)
+{
+    if (!c || !out) return DHARA_E_ARG;
+    *out = c->prof_acc;
+    std::memset(&c->prof_acc, 0, sizeof(c->prof_acc));
     return DHARA_OK;
 }
 

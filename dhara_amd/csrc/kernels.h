@@ -157,6 +157,17 @@ void launch_pcg3_iter_f64(hipStream_t s, const Geo& g, const uint8_t* flags, con
 void launch_pcg3_iter_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
                           const float* D, float* x, float* r, float* const p[2], float* q,
                           int parity, PcgState* st, Reduce red);
+// The two halves of one iteration, for per-kernel timing.
+void launch_pcg3_spmv_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
+                          const double* D, const double* r, double* const p[2], double* q,
+                          int parity, PcgState* st, Reduce red);
+void launch_pcg3_spmv_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
+                          const float* D, const float* r, float* const p[2], float* q,
+                          int parity, PcgState* st, Reduce red);
+void launch_pcg_update_f64(hipStream_t s, int n, double* x, double* r, const double* D,
+                           const double* p, const double* q, PcgState* st, Reduce red);
+void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float* D,
+                           const float* p, const float* q, PcgState* st, Reduce red);
 // y = A x (tests)
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                        const double* D, const double* x, double* y);

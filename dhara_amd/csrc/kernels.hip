@@ -757,15 +757,55 @@ void launch_end_step(hipStream_t s, const EndArgs& a)
 }
 
 template <typename T>
-static void pcg3_iter(hipStream_t s, const Geo& g, const uint8_t* flags, const T* K, const T* D,
-                      T* x, T* r, T* const p[2], T* q, int parity, PcgState* st, Reduce red)
+static void pcg3_spmv(hipStream_t s, const Geo& g, const uint8_t* flags, const T* K, const T* D,
+                      const T* r, T* const p[2], T* q, int parity, PcgState* st, Reduce red)
 {
     const dim3 gr(grid_for(g.n)), bl(kBlock);
     const T* pold = p[parity & 1];
     T* pnew = p[(parity + 1) & 1];
     if (flags) hipLaunchKernelGGL((k_pcg3_spmv<T, true>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, st, red);
     else hipLaunchKernelGGL((k_pcg3_spmv<T, false>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, st, red);
-    hipLaunchKernelGGL((k_pcg_update<T>), gr, bl, 0, s, g.n, x, r, D, (const T*)pnew, (const T*)q, st, red);
+}
+
+template <typename T>
+static void pcg_update(hipStream_t s, int n, T* x, T* r, const T* D, const T* p, const T* q,
+                       PcgState* st, Reduce red)
+{
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, D, p, q, st, red);
+}
+
+template <typename T>
+static void pcg3_iter(hipStream_t s, const Geo& g, const uint8_t* flags, const T* K, const T* D,
+                      T* x, T* r, T* const p[2], T* q, int parity, PcgState* st, Reduce red)
+{
+    pcg3_spmv<T>(s, g, flags, K, D, r, p, q, parity, st, red);
+    pcg_update<T>(s, g.n, x, r, D, p[(parity + 1) & 1], q, st, red);
+}
+
+void launch_pcg3_spmv_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
+                          const double* D, const double* r, double* const p[2], double* q,
+                          int parity, PcgState* st, Reduce red)
+{
+    pcg3_spmv<double>(s, g, flags, K, D, r, p, q, parity, st, red);
+}
+
+void launch_pcg3_spmv_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
+                          const float* D, const float* r, float* const p[2], float* q,
+                          int parity, PcgState* st, Reduce red)
+{
+    pcg3_spmv<float>(s, g, flags, K, D, r, p, q, parity, st, red);
+}
+
+void launch_pcg_update_f64(hipStream_t s, int n, double* x, double* r, const double* D,
+                           const double* p, const double* q, PcgState* st, Reduce red)
+{
+    pcg_update<double>(s, n, x, r, D, p, q, st, red);
+}
+
+void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float* D,
+                           const float* p, const float* q, PcgState* st, Reduce red)
+{
+    pcg_update<float>(s, n, x, r, D, p, q, st, red);
 }
 
 void launch_pcg3_iter_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,

@@ -179,6 +179,10 @@ def load_hip_library(path=HIP_LIB_PATH):
         lib.dhara_apply_subsurface_operator_f32.restype = C.c_int
         lib.dhara_overland_conductance.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         lib.dhara_overland_conductance.restype = C.c_int
+        lib.dhara_profile_enable.argtypes = [C.c_void_p, C.c_int32]
+        lib.dhara_profile_enable.restype = C.c_int
+        lib.dhara_profile_read.argtypes = [C.c_void_p, P(_abi.KernelProfile)]
+        lib.dhara_profile_read.restype = C.c_int
         lib.dhara_set_precision.argtypes = [C.c_void_p, C.c_int32]
         lib.dhara_set_precision.restype = C.c_int
         if lib.dhara_abi_version() != _abi.ABI_VERSION:
@@ -227,6 +231,15 @@ class FlowModel(BaseModel):
         self._check(self._lib.dhara_overland_conductance(self._h, _dptr(ke), _dptr(kn)),
                     "overland_conductance")
         return ke, kn
+
+    def profile(self, on=True):
+        """HIP-event timing of the subsurface Krylov kernels on the context's stream."""
+        self._check(self._lib.dhara_profile_enable(self._h, 1 if on else 0), "profile_enable")
+
+    def profile_read(self):
+        pr = _abi.KernelProfile()
+        self._check(self._lib.dhara_profile_read(self._h, C.byref(pr)), "profile_read")
+        return pr.as_dict()
 
     def set_precision(self, prec):
         self._check(self._lib.dhara_set_precision(self._h, prec), "set_precision")

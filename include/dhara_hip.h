@@ -161,6 +161,19 @@ typedef struct dhara_step_stats {
     double  wall_ms;            /* wall time of the call                                   */
 } dhara_step_stats;
 
+/* Device-time profile of the subsurface Krylov kernels (measurement hook; no reference
+ * counterpart).  Only launches that did work are counted. */
+typedef struct dhara_kernel_profile {
+    int64_t spmv_launches;      /* p-update + stencil SpMV + (p, Ap) kernel                  */
+    double  spmv_ms;            /* summed device time of those launches                      */
+    double  spmv_bytes;         /* algorithmic HBM bytes of one launch                       */
+    int64_t update_launches;    /* x/r update + (r, M^-1 r), (r, r) kernel                   */
+    double  update_ms;
+    double  update_bytes;
+    int32_t elem_bytes;         /* 8 (FP64) or 4 (FP32 inner solves) of the profiled solves  */
+    int32_t reserved;
+} dhara_kernel_profile;
+
 typedef struct dhara_ctx dhara_ctx;
 
 /* ---- defaults ------------------------------------------------------------------------ */
@@ -224,6 +237,13 @@ int dhara_overland_step(dhara_ctx* ctx, dhara_step_stats* stats);
  * RunCoupledFlowModel's time loop (src/device.cu:426-469) without the canopy model.
  * `stats` (may be NULL) receives nsteps records. */
 int dhara_step(dhara_ctx* ctx, int32_t nsteps, dhara_step_stats* stats);
+
+/* ---- measurement ------------------------------------------------------------------------ */
+/* Turn HIP-event timing of the subsurface Krylov kernels on (1) or off (0).  The events are
+ * recorded on the context's stream around every SpMV / update launch. */
+int dhara_profile_enable(dhara_ctx* ctx, int32_t on);
+/* Accumulated profile since the last read; resets the accumulators. */
+int dhara_profile_read(dhara_ctx* ctx, dhara_kernel_profile* out);
 
 /* ---- kernel-level entry points used by the parity tests ------------------------------ */
 /* theta, K, C of the context's soil at the given psi (vanGenuchten, src/global.cu:66-105).

@@ -1,0 +1,104 @@
+"""C-ABI checks that need no GPU: the HIP library loads, exports every entry point the
+header declares, and the ctypes mirror matches the C layout of every structure."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from dhara_amd import _abi
+from dhara_amd.model import HIP_LIB_PATH
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dhara_hip.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dhara_\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(n for n in names if n != "dhara_params_defaults"))
+
+
+def test_library_exports_every_declared_function():
+    assert os.path.exists(HIP_LIB_PATH), "build libdhara_hip.so first (python -m dhara_amd.build)"
+    lib = C.CDLL(HIP_LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.dhara_abi_version() == _abi.ABI_VERSION
+
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "dhara_hip.h"
+#define F(s, m) printf(#s "." #m " %zu\n", offsetof(s, m));
+int main(void) {
+  printf("dhara_grid %zu\n", sizeof(dhara_grid));
+  printf("dhara_params %zu\n", sizeof(dhara_params));
+  printf("dhara_step_stats %zu\n", sizeof(dhara_step_stats));
+  printf("dhara_kernel_profile %zu\n", sizeof(dhara_kernel_profile));
+  F(dhara_params, lin_tol) F(dhara_params, inner_tol) F(dhara_params, olf_tol)
+  F(dhara_params, K0) F(dhara_params, top_bc)
+  F(dhara_step_stats, picard_delta) F(dhara_step_stats, wall_ms)
+  F(dhara_kernel_profile, update_bytes)
+  printf("DHARA_F__COUNT %d\n", (int)DHARA_F__COUNT);
+  printf("DHARA_F_PROCMAP %d\n", (int)DHARA_F_PROCMAP);
+  dhara_params p; dhara_params_defaults(&p);
+  printf("default.lin_tol %.17g\n", p.lin_tol);
+  printf("default.picard_tol %.17g\n", p.picard_tol);
+  printf("default.inner_tol %.17g\n", p.inner_tol);
+  printf("default.theta_s %.17g\n", p.theta_s);
+  return 0;
+}
+"""
+
+
+def test_ctypes_layout_matches_c():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        open(src, "w").write(PROBE)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    vals = dict(line.rsplit(" ", 1) for line in out.strip().splitlines())
+    assert int(vals["dhara_grid"]) == C.sizeof(_abi.Grid)
+    assert int(vals["dhara_params"]) == C.sizeof(_abi.Params)
+    assert int(vals["dhara_step_stats"]) == C.sizeof(_abi.StepStats)
+    assert int(vals["dhara_kernel_profile"]) == C.sizeof(_abi.KernelProfile)
+    for key, cls in (("dhara_params", _abi.Params), ("dhara_step_stats", _abi.StepStats),
+                     ("dhara_kernel_profile", _abi.KernelProfile)):
+        for k, v in vals.items():
+            if k.startswith(key + "."):
+                assert getattr(cls, k.split(".")[1]).offset == int(v), k
+    assert int(vals["DHARA_F__COUNT"]) == len(_abi.F)
+    assert int(vals["DHARA_F_PROCMAP"]) == _abi.F["PROCMAP"]
+    d = _abi.default_params()
+    for k in ("lin_tol", "picard_tol", "inner_tol", "theta_s"):
+        assert float(vals["default." + k]) == getattr(d, k)
+
+
+def test_product_never_loads_the_oracle():
+    pkg = os.path.join(ROOT, "dhara_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in txt and "from oracle" not in txt, f
+                assert "dhara_oracle" not in txt and "libdhara_oracle" not in txt, f
+
+
+def test_oracle_library_exports():
+    from oracle import oracle
+    L = oracle.lib()
+    for n in declared_functions():
+        if n in ("dhara_abi_version", "dhara_device_info", "dhara_get_params", "dhara_set_precision",
+                 "dhara_upload_field_device", "dhara_download_field_device",
+                 "dhara_apply_subsurface_operator_f32", "dhara_profile_enable", "dhara_profile_read",
+                 "dhara_overland_conductance", "dhara_assemble_subsurface"):
+            continue
+        assert hasattr(L, "dor_" + n[len("dhara_"):]), n
