@@ -4,7 +4,9 @@
 // reference (src/device.cu:374-475 -> src/subsurface.cu:686-847 -> src/overland.cu:387-419)
 // is driven from here; all arithmetic runs in kernels.hip.  The host synchronises once per
 // Krylov batch, once per Picard iteration and once at the end of a step (the reference
-// synchronised on every dot product, Picard max and mass-balance reduction).
+// synchronised on every dot product, Picard max and mass-balance reduction).  Reductions the
+// host consumes (Picard increment, mass balance, discharge) are summed on the host from the
+// per-block partials in block order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,17 +56,13 @@ struct dhara_ctx {
     // overland solver
     double *D2 = nullptr, *b2 = nullptr, *r2 = nullptr, *p2[2] = {nullptr, nullptr}, *q2 = nullptr;
     double *ke = nullptr, *kn = nullptr;
-    // reductions
-    double* part = nullptr;
-    unsigned* cnt = nullptr;
+    // reductions: partial buffers A (written by assembly/update kernels) and B (SpMV kernels)
+    double* part[2] = {nullptr, nullptr};
     dh::PcgState* st_sub = nullptr;
     dh::PcgState* st_olf = nullptr;
-    dh::PicardState* pst = nullptr;
-    double* sums = nullptr;
     // pinned host mirrors
     dh::PcgState* h_st = nullptr;
-    dh::PicardState* h_pst = nullptr;
-    double* h_sums = nullptr;
+    double* h_part = nullptr;
 
     bool face_dirichlet[6] = {false, false, false, false, false, false};
     bool has_alpha = false, has_n = false, has_transp = false;
@@ -169,7 +167,20 @@ dh::Soil soil_of(const dhara_ctx* c)
     return s;
 }
 
-dh::Reduce red_of(const dhara_ctx* c) { return dh::Reduce{c->part, c->cnt}; }
+// Sum (or max) the first nv partial rows of the last host-reduced kernel, in block order.
+int host_reduce(dhara_ctx* c, int nparts, int nv, bool take_max, double* out)
+{
+    const size_t span = (size_t)(nv - 1) * dh::kMaxBlocks + nparts;
+    HIPCHK(c, hipMemcpyAsync(c->h_part, c->part[0], span * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    for (int v = 0; v < nv; ++v) {
+        double acc = take_max ? -INFINITY : 0.0;
+        const double* row = c->h_part + (size_t)v * dh::kMaxBlocks;
+        for (int b = 0; b < nparts; ++b) acc = take_max ? std::max(acc, row[b]) : acc + row[b];
+        out[v] = acc;
+    }
+    return DHARA_OK;
+}
 
 bool has_dirichlet(const dhara_ctx* c)
 {
@@ -201,8 +212,7 @@ dh::VgArgs vg_args(dhara_ctx* c, const double* psi, double* theta)
     a.K32 = c->K32;
     a.rule = c->p.picard_rule;
     a.am = c->p.am;
-    a.pst = c->pst;
-    a.red = red_of(c);
+    a.part_out = c->part[0];
     return a;
 }
 
@@ -258,12 +268,8 @@ dh::AsmArgs asm_args(dhara_ctx* c, bool dir)
     a.D32 = c->D32;
     a.r32 = c->r32;
     a.d32 = c->d32;
-    a.lin_tol = c->p.lin_tol;
-    a.inner_tol = c->p.inner_tol;
-    a.maxit = c->p.lin_maxiter;
-    a.inner_maxit = c->p.inner_maxiter;
+    a.part_out = c->part[0];
     a.st = c->st_sub;
-    a.red = red_of(c);
     return a;
 }
 
@@ -283,39 +289,54 @@ dh::OlfArgs olf_args(dhara_ctx* c)
     a.D = c->D2;
     a.b = c->b2;
     a.r = c->r2;
-    a.tol = c->p.olf_tol;
-    a.maxit = c->p.olf_maxiter;
+    a.part_out = c->part[0];
     a.st = c->st_olf;
-    a.red = red_of(c);
-    a.out = c->sums;
     return a;
 }
 
+dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bool mixed_inner,
+                       bool first_sweep)
+{
+    dh::SolveCtl sc{};
+    sc.it = 0;
+    sc.maxit = maxit;
+    sc.mixed_inner = mixed_inner ? 1 : 0;
+    sc.first_sweep = first_sweep ? 1 : 0;
+    sc.lin_tol = mixed_inner ? c->p.lin_tol : c->p.lin_tol;
+    sc.inner_tol = c->p.inner_tol;
+    sc.part_in = c->part[0];
+    sc.nparts = nparts;
+    sc.part_out = c->part[1];
+    sc.st = st;
+    return sc;
+}
+
 // Queue Krylov iterations in batches; the kernels stop doing work once the device-side
-// state says the solve is done, so over-queued iterations cost one launch each.  With
-// profiling on, events bracket each SpMV and update launch of the batch and only the
-// launches that did work (the first `done` ones of the batch) are accumulated.
+// state says the solve is done, so over-queued iterations cost one launch each.  The SpMV
+// kernel of iteration `it` reads partials A and writes B, the update kernel reads B and
+// writes A.  With profiling on, events bracket each SpMV and update launch of the batch and
+// only the launches that did work (the first ones of the batch) are accumulated.
 template <class SpmvFn, class UpdFn>
 int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
                SpmvFn spmv, UpdFn upd)
 {
-    int parity = 0, launched = 0, iter_before = 0;
+    int it = 0;
     int batch = std::max(4, std::min(kMaxBatch, hint));
     for (;;) {
-        for (int b = 0; b < batch; ++b) {
+        const int it0 = it;
+        for (int b = 0; b < batch; ++b, ++it) {
             if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b], c->s));
-            spmv(parity);
+            spmv(it);
             if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b + 1], c->s));
-            upd(parity);
+            upd(it);
             if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b + 2], c->s));
-            parity ^= 1;
         }
         LAUNCHCHK(c);
-        launched += batch;
         int rc = read_state(c, st);
         if (rc) return rc;
+        const int done_iters = c->h_st->done ? c->h_st->iter : it;
         if (prof) {
-            const int real = std::max(0, std::min(batch, c->h_st->iter - iter_before));
+            const int real = std::max(0, std::min(batch, done_iters - it0));
             for (int b = 0; b < real; ++b) {
                 float m0 = 0.f, m1 = 0.f;
                 HIPCHK(c, hipEventElapsedTime(&m0, c->ev[3 * b], c->ev[3 * b + 1]));
@@ -329,10 +350,9 @@ int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, 
             c->prof_acc.update_bytes = 7.0 * elem * c->N;
             c->prof_acc.elem_bytes = (int)elem;
         }
-        iter_before = c->h_st->iter;
         if (c->h_st->done) break;
-        if (launched > maxit + 4) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
-        const int left = hint - launched;
+        if (it > maxit + 1) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
+        const int left = hint - it;
         batch = std::max(4, std::min(16, left > 0 ? left : 8));
     }
     hint = std::max(1, c->h_st->iter);
@@ -346,11 +366,20 @@ int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
     LAUNCHCHK(c);
     double* const pp[2] = {c->pv[0], c->pv[1]};
     const uint8_t* fl = dir ? c->flags : nullptr;
+    dh::SolveCtl sc = solve_ctl(c, c->st_sub, c->p.lin_maxiter, dh::grid_for(c->N), false, true);
     int rc = run_krylov(
         c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, c->prof, 8.0,
-        [&](int par) { dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, par, c->st_sub, red_of(c)); },
-        [&](int par) {
-            dh::launch_pcg_update_f64(c->s, c->N, c->x, c->r, c->D, pp[(par + 1) & 1], c->q, c->st_sub, red_of(c));
+        [&](int it) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, k);
+        },
+        [&](int it) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            k.part_in = c->part[1];
+            k.part_out = c->part[0];
+            dh::launch_pcg_update_f64(c->s, c->N, c->x, c->r, c->D, pp[(it + 1) & 1], c->q, k);
         });
     if (rc) return rc;
     st->lin_iters += c->h_st->iter;
@@ -359,51 +388,57 @@ int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
     return DHARA_OK;
 }
 
+// Mixed precision: the FP64 assembly gives r64 = b - A x; each sweep solves A d = r in FP32
+// (Jacobi-PCG to inner_tol relative), then x += d and r64 = b - A x in FP64.  The first SpMV
+// kernel of every inner solve checks the FP64 residual against lin_tol ||b|| and ends the
+// solve at once when it is met.
 int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
 {
     const bool dir = has_dirichlet(c);
     dh::launch_assemble(c->s, asm_args(c, dir), true);
     LAUNCHCHK(c);
     float* const pp[2] = {c->p32[0], c->p32[1]};
+    const uint8_t* fl = dir ? c->flags : nullptr;
     dh::RefineArgs ra{};
     ra.g = c->g;
     ra.K = c->K;
     ra.D = c->D;
     ra.b = c->b;
     ra.x = c->x;
-    ra.flags = dir ? c->flags : nullptr;
+    ra.flags = fl;
     ra.r32 = c->r32;
     ra.D32 = c->D32;
     ra.d32 = c->d32;
-    ra.lin_tol = c->p.lin_tol;
-    ra.inner_tol = c->p.inner_tol;
+    ra.part_out = c->part[0];
     ra.st = c->st_sub;
-    ra.red = red_of(c);
     int sweeps = 0;
-    int rc = read_state(c, c->st_sub);
-    if (rc) return rc;
-    bool outer_conv = c->h_st->outer_conv != 0;
-    while (!outer_conv && sweeps < c->p.refine_maxiter) {
-        const uint8_t* fl = dir ? c->flags : nullptr;
-        rc = run_krylov(
-            c, c->st_sub, c->p.inner_maxiter, c->pcg_hint, c->prof, 4.0,
-            [&](int par) {
-                dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, par, c->st_sub,
-                                         red_of(c));
+    bool outer_conv = false;
+    for (;;) {
+        // after the last allowed sweep only the convergence check runs (maxit 0)
+        const int maxit = sweeps < c->p.refine_maxiter ? c->p.inner_maxiter : 0;
+        dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, sweeps == 0);
+        int rc = run_krylov(
+            c, c->st_sub, maxit, c->pcg_hint, c->prof && maxit > 0, 4.0,
+            [&](int it) {
+                dh::SolveCtl k = sc;
+                k.it = it;
+                dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
             },
-            [&](int par) {
-                dh::launch_pcg_update_f32(c->s, c->N, c->d32, c->r32, c->D32, pp[(par + 1) & 1], c->q32,
-                                          c->st_sub, red_of(c));
+            [&](int it) {
+                dh::SolveCtl k = sc;
+                k.it = it;
+                k.part_in = c->part[1];
+                k.part_out = c->part[0];
+                dh::launch_pcg_update_f32(c->s, c->N, c->d32, c->r32, c->D32, pp[(it + 1) & 1], c->q32, k);
             });
         if (rc) return rc;
+        if (c->h_st->nan) { st->nan_seen = 1; break; }
+        if (c->h_st->outer_conv) { outer_conv = true; break; }
+        if (maxit == 0) break;
         st->lin_iters += c->h_st->iter;
         ++sweeps;
         dh::launch_refine(c->s, ra);
         LAUNCHCHK(c);
-        rc = read_state(c, c->st_sub);
-        if (rc) return rc;
-        outer_conv = c->h_st->outer_conv != 0;
-        if (c->h_st->nan) { st->nan_seen = 1; break; }
     }
     st->refine_sweeps += sweeps;
     st->lin_converged &= outer_conv ? 1 : 0;
@@ -439,13 +474,14 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
     while (!conv && niter < c->p.picard_maxiter) {
         int rc = mixed ? solve_mixed(c, st) : solve_fp64(c, st);
         if (rc) return rc;
+        if (st->nan_seen) { c->theta_m_current = false; break; }
         dh::launch_vg_picard(c->s, vg_args(c, c->x, c->theta_new));
         LAUNCHCHK(c);
-        HIPCHK(c, hipMemcpyAsync(c->h_pst, c->pst, sizeof(dh::PicardState), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(c, hipStreamSynchronize(c->s));
+        double pic[2];
+        if ((rc = host_reduce(c, dh::grid_for(c->N), 2, true, pic))) return rc;
         ++niter;
-        delta = c->h_pst->delta;
-        if (c->h_pst->nan) { st->nan_seen = 1; c->theta_m_current = false; break; }
+        delta = pic[0];
+        if (pic[1] > 0.0) { st->nan_seen = 1; c->theta_m_current = false; break; }
         if (delta < c->p.picard_tol) {
             conv = 1;
             c->theta_m_current = false;
@@ -484,15 +520,15 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
         ea.bctype[f] = c->bctype[f];
         ea.bcq[f] = c->bcq[f];
     }
-    ea.red = red_of(c);
-    ea.out = c->sums;
+    ea.part_out = c->part[0];
     dh::launch_end_step(c->s, ea);
     LAUNCHCHK(c);
-    HIPCHK(c, hipMemcpyAsync(c->h_sums, c->sums, 9 * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
+    double sums[9];
+    int rce = host_reduce(c, dh::grid_for(c->N), 9, false, sums);
+    if (rce) return rce;
     // theta carry consumed theta_m; the last vG result (theta_new) is theta(psi^n)
     c->theta_m_current = false;
-    const double* s = c->h_sums;
+    const double* s = sums;
     const double dx = c->grid.dx, dy = c->grid.dy, dz = c->grid.dz, dt = c->p.dt;
     const double sdtheta = s[0], sss = s[1], ssink = s[2];
     const double qt = s[3], qb = s[4], qw = s[5], qe = s[6], qs = s[7], qn = s[8];
@@ -517,24 +553,32 @@ int do_overland_step(dhara_ctx* c, dhara_step_stats* st)
     dh::launch_olf_assemble(c->s, a);
     LAUNCHCHK(c);
     double* const pp[2] = {c->p2[0], c->p2[1]};
-    // launch_pcg2_iter queues both halves of the iteration; the second callback is empty
+    dh::SolveCtl sc = solve_ctl(c, c->st_olf, c->p.olf_maxiter, dh::grid_for(c->S), false, true);
+    sc.lin_tol = c->p.olf_tol;
     int rc = run_krylov(
         c, c->st_olf, c->p.olf_maxiter, c->olf_hint, false, 8.0,
-        [&](int par) {
-            dh::launch_pcg2_iter(c->s, c->g2, c->ke, c->kn, c->D2, c->H, c->r2, pp, c->q2, par, c->st_olf,
-                                 red_of(c));
+        [&](int it) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            dh::launch_pcg2_spmv(c->s, c->g2, c->ke, c->kn, c->D2, c->r2, pp, c->q2, k);
         },
-        [&](int) {});
+        [&](int it) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            k.part_in = c->part[1];
+            k.part_out = c->part[0];
+            dh::launch_pcg_update_f64(c->s, c->S, c->H, c->r2, c->D2, pp[(it + 1) & 1], c->q2, k);
+        });
     if (rc) return rc;
     st->olf_iters = c->h_st->iter;
     st->olf_converged = c->h_st->conv;
     if (c->h_st->nan) st->nan_seen = 1;
     dh::launch_olf_post(c->s, a);
     LAUNCHCHK(c);
-    HIPCHK(c, hipMemcpyAsync(c->h_sums, c->sums, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    st->outlet_discharge = c->h_sums[0];
-    st->ring_outflow = c->h_sums[1];
+    double q[2];
+    if ((rc = host_reduce(c, dh::grid_for(c->S), 2, false, q))) return rc;
+    st->outlet_discharge = q[0];
+    st->ring_outflow = q[1];
     return st->nan_seen ? set_err(c, DHARA_E_NUMERIC, "NaN/Inf in the overland state") : DHARA_OK;
 }
 
@@ -668,9 +712,9 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
         }
     }
     if ((rc = dalloc(c, &c->procmap, S)) || (rc = dalloc(c, &c->root, (size_t)grid->nroot_groups * grid->nz)) ||
-        (rc = dalloc(c, &c->flags, N)) || (rc = dalloc(c, &c->part, (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
-        (rc = dalloc(c, &c->cnt, 16)) || (rc = dalloc(c, &c->st_sub, 1)) || (rc = dalloc(c, &c->st_olf, 1)) ||
-        (rc = dalloc(c, &c->pst, 1)) || (rc = dalloc(c, &c->sums, 16))) {
+        (rc = dalloc(c, &c->flags, N)) || (rc = dalloc(c, &c->part[0], (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
+        (rc = dalloc(c, &c->part[1], (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
+        (rc = dalloc(c, &c->st_sub, 1)) || (rc = dalloc(c, &c->st_olf, 1))) {
         dhara_destroy(c);
         return rc;
     }
@@ -682,8 +726,7 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
         }
     }
     if (hipHostMalloc((void**)&c->h_st, sizeof(dh::PcgState)) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_pst, sizeof(dh::PicardState)) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_sums, 16 * sizeof(double)) != hipSuccess) {
+        hipHostMalloc((void**)&c->h_part, (size_t)dh::kMaxSums * dh::kMaxBlocks * sizeof(double)) != hipSuccess) {
         dhara_destroy(c);
         return DHARA_E_ALLOC;
     }
@@ -700,8 +743,7 @@ int dhara_destroy(dhara_ctx* c)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void* v : c->allocs) (void)hipFree(v);
     if (c->h_st) (void)hipHostFree(c->h_st);
-    if (c->h_pst) (void)hipHostFree(c->h_pst);
-    if (c->h_sums) (void)hipHostFree(c->h_sums);
+    if (c->h_part) (void)hipHostFree(c->h_part);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
     return DHARA_OK;

@@ -2,9 +2,9 @@
 //
 //  * Mualem-van Genuchten constitutive law in the stable exp/log form (one log, one log1p,
 //    three exp/expm1 per cell instead of the reference's seven pow, src/global.cu:66-105).
-//  * wave64 shuffle reductions, block reductions through LDS, and a deterministic
-//    last-block-done grid reduction (fixed summation order, so results are identical from
-//    run to run).
+//  * wave64 shuffle reductions, block reductions through LDS, and fence-free grid
+//    reductions: per-block partials summed by the consumer in a fixed order, so results are
+//    identical from run to run.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,30 +17,21 @@ constexpr int kWave = 64;          // CDNA wavefront
 constexpr int kMaxBlocks = 4096;   // capacity of the per-block partial buffers
 constexpr int kMaxSums = 12;       // values reduced together at most
 
-// Solver state of one Krylov solve, resident in device memory.  Written by the last block
-// of each reduction, read by the next kernel (and by the host after a batch).
+// Solver state of one Krylov solve, resident in device memory.  Fields a kernel reads while
+// one of its blocks may write are double-buffered by iteration parity (rz), the rest are
+// written only by the kernel that ends the solve.  The host reads it after a batch.
 struct PcgState {
-    double rz;        // (r, M^-1 r)
-    double pq;        // (p, A p)
-    double rr;        // ||r||^2
+    double rz[2];     // (r_i, M^-1 r_i), slot i & 1, written by the SpMV kernel of iteration i
     double bb;        // ||b||^2 of the outer system
-    double alpha, beta;
-    double tol2;      // stop when rr <= tol2
+    double tol2;      // stop when ||r||^2 <= tol2
+    double rr;        // ||r||^2 when the solve ended
     double rr_outer;  // ||r_64||^2 of the last FP64 residual (mixed precision)
-    int32_t iter;     // iterations of the current solve
-    int32_t maxit;
+    int32_t iter;     // iterations done when the solve ended
     int32_t done;     // 1: solve finished (converged, limit, or breakdown)
     int32_t conv;     // 1: converged
     int32_t nan;
-    int32_t total;    // iterations summed over the refinement sweeps of a Picard iteration
     int32_t outer_conv;
-    int32_t pad;
-};
-
-struct PicardState {
-    double delta;     // max over cells of the Picard increment (rule units)
-    int32_t nan;
-    int32_t pad;
+    int32_t pad[3];
 };
 
 // ---- Mualem-van Genuchten ------------------------------------------------------------
@@ -77,10 +68,15 @@ __device__ __forceinline__ double vg_inverse(double theta, double alpha, double 
 }
 
 // ---- reductions ---------------------------------------------------------------------------
+// No kernel communicates across blocks: a producer kernel writes one partial per block with a
+// plain store, and the consumer -- the next kernel, after the kernel boundary has made the
+// stores visible on every XCD, or the host -- sums the partials in block order.  Every
+// consuming block runs the same fixed-order sum, so all of them see identical totals and the
+// result does not depend on scheduling.  (A last-block-done reduction would need an
+// agent-scope release fence per block, i.e. an L2 writeback per block on a multi-XCD part.)
 __device__ __forceinline__ int lin_tid() { return threadIdx.x + threadIdx.y * blockDim.x; }
 __device__ __forceinline__ int lin_nthreads() { return blockDim.x * blockDim.y; }
 __device__ __forceinline__ int lin_bid() { return blockIdx.x + blockIdx.y * gridDim.x; }
-__device__ __forceinline__ int lin_nblocks() { return gridDim.x * gridDim.y; }
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -121,62 +117,43 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* lds)
     }
 }
 
-__device__ __forceinline__ void store_agent(double* p, double v)
-{
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
-                       (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ double load_agent(const double* p)
-{
-    unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
-}
-
-// Grid-wide reduction of NV values with the last-block-done pattern.  Every block writes
-// its partials, the block that arrives last sums all partials in block order (so the result
-// does not depend on scheduling) and returns true with the totals in `out` (valid in
-// thread 0).  The arrival counter is reset by that block for the next launch.
+// Producer side: reduce NV values over the block and store them at part[i*kMaxBlocks + block].
 template <int NV, bool MAX>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double* part, unsigned* counter,
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
+{
+    __shared__ double lds[NV * 16];
+    block_reduce<NV, MAX>(v, lds);
+    if (lin_tid() == 0) {
+        const int bid = lin_bid();
+#pragma unroll
+        for (int i = 0; i < NV; ++i) part[i * kMaxBlocks + bid] = v[i];
+    }
+}
+
+// Consumer side: the first nv (<= NV) totals of the nparts partials of the previous kernel,
+// returned to every thread of the block.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nparts, int nv,
                                             double (&out)[NV])
 {
     __shared__ double lds[NV * 16];
-    __shared__ int is_last;
-    block_reduce<NV, MAX>(v, lds);
-    const int tid = lin_tid();
-    const int bid = lin_bid(), nb = lin_nblocks();
-    if (tid == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) store_agent(part + i * kMaxBlocks + bid, v[i]);
-        __threadfence();
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+    __shared__ double res[NV];
     double acc[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = MAX ? -INFINITY : 0.0;
-    // fixed order: thread t sums blocks t, t+T, t+2T, ...; then a fixed tree
-    for (int b = tid; b < nb; b += lin_nthreads()) {
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int b = lin_tid(); b < nparts; b += lin_nthreads()) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const double x = load_agent(part + i * kMaxBlocks + b);
-            acc[i] = MAX ? fmax(acc[i], x) : acc[i] + x;
-        }
+        for (int i = 0; i < NV; ++i)
+            if (i < nv) acc[i] += part[i * kMaxBlocks + b];
     }
-    block_reduce<NV, MAX>(acc, lds);
-    if (tid == 0) {
+    block_reduce<NV, false>(acc, lds);
+    if (lin_tid() == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) out[i] = acc[i];
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = 0; i < NV; ++i) res[i] = acc[i];
     }
-    return true;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = res[i];
 }
 
 }  // namespace dh

@@ -30,16 +30,26 @@ struct Geo2 {
     double cx, cy;        // dt/dx^2, dt/dy^2
 };
 
-struct Reduce {
-    double* part;         // kMaxSums * kMaxBlocks partials
-    unsigned* cnt;        // arrival counter (one per concurrently live reduction)
-};
-
 struct Soil {
     const double* Ks;
     const double* alpha_f;   // nullptr: scalar alpha
     const double* n_f;       // nullptr: scalar n
     double alpha, n, ts, tr;
+};
+
+// Control block of one Krylov launch.  `it` is the iteration index within the solve (the
+// host counts launches; kernels after the end of a solve return at once).  Partials of the
+// previous kernel are read from part_in (nparts blocks) and this kernel's go to part_out.
+struct SolveCtl {
+    int it;
+    int maxit;
+    int mixed_inner;      // inner FP32 solve of the mixed path
+    int first_sweep;      // ||b||^2 is taken from part_in (first solve of a Picard iteration)
+    double lin_tol, inner_tol;
+    const double* part_in;
+    int nparts;
+    double* part_out;
+    PcgState* st;
 };
 
 struct VgArgs {
@@ -54,8 +64,7 @@ struct VgArgs {
     float* K32;              // out (mixed precision), may be null
     int rule;
     double am;
-    PicardState* pst;
-    Reduce red;
+    double* part_out;        // Picard pass: [max increment, NaN flag] per block (host-reduced)
 };
 
 struct AsmArgs {
@@ -76,10 +85,8 @@ struct AsmArgs {
     float* D32;              // mixed path
     float* r32;
     float* d32;
-    double lin_tol, inner_tol;
-    int maxit, inner_maxit;
+    double* part_out;        // [rz, rr, bb, rr64] per block
     PcgState* st;
-    Reduce red;
 };
 
 struct RefineArgs {
@@ -92,9 +99,8 @@ struct RefineArgs {
     float* r32;
     const float* D32;
     float* d32;
-    double lin_tol, inner_tol;
+    double* part_out;        // [rz32, rr32, 0, rr64] per block
     PcgState* st;
-    Reduce red;
 };
 
 struct SurfArgs {
@@ -118,8 +124,7 @@ struct EndArgs {
     int carry_reference;
     const int* bctype[6];
     const double* bcq[6];
-    Reduce red;
-    double* out;             // 9 sums
+    double* part_out;        // 9 sums per block (host-reduced)
 };
 
 struct OlfArgs {
@@ -130,12 +135,12 @@ struct OlfArgs {
     double* H;
     double *ke, *kn;
     double *D, *b, *r;
-    double tol;
-    int maxit;
+    double* part_out;
     PcgState* st;
-    Reduce red;
-    double* out;             // outlet sums
 };
+
+// Number of blocks (= partials) of the grid-stride kernels over n cells.
+int grid_for(int n);
 
 // ---- launchers (all asynchronous on `s`) ----------------------------------------------
 void launch_vg(hipStream_t s, const VgArgs& a);                 // theta, K, C of psi
@@ -149,25 +154,19 @@ void launch_assemble(hipStream_t s, const AsmArgs& a, bool mixed);
 void launch_refine(hipStream_t s, const RefineArgs& a);
 void launch_end_step(hipStream_t s, const EndArgs& a);
 
-// One Jacobi-PCG iteration on the 3D operator (spmv+dot, then update+dots).
-// p[0]/p[1] ping-pong; `parity` selects which one is read.
-void launch_pcg3_iter_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
-                          const double* D, double* x, double* r, double* const p[2], double* q,
-                          int parity, PcgState* st, Reduce red);
-void launch_pcg3_iter_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
-                          const float* D, float* x, float* r, float* const p[2], float* q,
-                          int parity, PcgState* st, Reduce red);
-// The two halves of one iteration, for per-kernel timing.
+// Jacobi-PCG iteration `sc.it` on the 3D operator, split in its two kernels:
+//   spmv:   p = M^-1 r + beta p (p[it&1] -> p[(it+1)&1]), q = A p, partial (p, q)
+//   update: x += alpha p, r -= alpha q, partials (r, M^-1 r), (r, r)
 void launch_pcg3_spmv_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                           const double* D, const double* r, double* const p[2], double* q,
-                          int parity, PcgState* st, Reduce red);
+                          const SolveCtl& sc);
 void launch_pcg3_spmv_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
                           const float* D, const float* r, float* const p[2], float* q,
-                          int parity, PcgState* st, Reduce red);
+                          const SolveCtl& sc);
 void launch_pcg_update_f64(hipStream_t s, int n, double* x, double* r, const double* D,
-                           const double* p, const double* q, PcgState* st, Reduce red);
+                           const double* p, const double* q, const SolveCtl& sc);
 void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float* D,
-                           const float* p, const float* q, PcgState* st, Reduce red);
+                           const float* p, const float* q, const SolveCtl& sc);
 // y = A x (tests)
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                        const double* D, const double* x, double* y);
@@ -179,9 +178,9 @@ void launch_cast_f64(hipStream_t s, int n, const float* in, double* out);
 // Overland
 void launch_olf_faces(hipStream_t s, const OlfArgs& a);
 void launch_olf_assemble(hipStream_t s, const OlfArgs& a);
-void launch_pcg2_iter(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
-                      const double* D, double* x, double* r, double* const p[2], double* q,
-                      int parity, PcgState* st, Reduce red);
+void launch_pcg2_spmv(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
+                      const double* D, const double* r, double* const p[2], double* q,
+                      const SolveCtl& sc);
 void launch_olf_post(hipStream_t s, const OlfArgs& a);
 
 }  // namespace dh

@@ -2,23 +2,28 @@
 // wave) solvers.  Every kernel is grid-stride over cells in the reference layout
 // (k*nx*ny + j*nx + i, k = 0 top) and keeps per-cell state in HBM as SoA arrays.
 //
-// Reductions use dh::grid_reduce (last-block-done, fixed order).  The Krylov kernels read
-// PcgState::done at entry and return at once when the solve has finished, so the host can
-// queue iterations in batches without a round trip per iteration.
+// Reductions: every kernel writes one partial per block; the next kernel (or the host) sums
+// them in block order (device_common.h).  The Krylov kernels read PcgState::done at entry and
+// return at once when the solve has finished, so the host can queue iterations in batches
+// without a round trip per iteration.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
 
 namespace dh {
 
-namespace {
-
-inline int grid_for(int n, int per_block = kBlock, int cap = 2048)
+// 1024 blocks of 256 threads = 4 blocks (16 wavefronts) per CU on 256 CUs; larger grids
+// only add partials for the consumer to sum.
+int grid_for(int n)
 {
-    int g = (n + per_block - 1) / per_block;
+    constexpr int cap = 1024;
+    static_assert(cap <= kMaxBlocks, "partial buffers too small");
+    int g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;
     return g < cap ? g : cap;
 }
+
+namespace {
 
 __device__ __forceinline__ void decompose(const Geo& g, int c, int& i, int& j, int& k)
 {
@@ -110,11 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_vg_picard(VgArgs a)
         if (a.K32) a.K32[c] = (float)K;
         a.psi_m[c] = a.am * pm + (1.0 - a.am) * xc;
     }
-    double out[2];
-    if (grid_reduce<2, true>(mx, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
-        a.pst->delta = out[0];
-        a.pst->nan = out[1] > 0.0 ? 1 : 0;
-    }
+    block_partials<2, true>(mx, a.part_out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_vg_inverse(Geo g, Soil s, const double* theta, double* psi)
@@ -189,7 +190,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
     const double dx2inv = -1.0 / (dx * dx), dy2inv = -1.0 / (dy * dy), dz2inv = -1.0 / (dz * dz);
     const double* __restrict__ K = a.K;
     const double* __restrict__ x = a.x;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};   // bb, rr(64), rz(solver precision), rr(solver)
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz, rr (solver precision), bb, rr (FP64)
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;   // a new solve starts
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
         int i, j, k;
         decompose(g, c, i, j, k);
@@ -275,48 +277,23 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
             if (k < g.nz - 1 && !(DIR && (fl & RF_CUT_U))) Ax += dz2inv * ku * x[c + g.nxy];
         }
         const double r = b - Ax;
-        acc[0] += b * b;
-        acc[1] += r * r;
+        acc[2] += b * b;
+        acc[3] += r * r;
         if (MIXED) {
             const float r32 = (float)r, D32 = (float)D;
             a.D32[c] = D32;
             a.r32[c] = r32;
             a.d32[c] = 0.0f;
             const float z32 = r32 / D32;
-            acc[2] += (double)r32 * (double)z32;
-            acc[3] += (double)r32 * (double)r32;
+            acc[0] += (double)r32 * (double)z32;
+            acc[1] += (double)r32 * (double)r32;
         } else {
             a.r[c] = r;
-            acc[2] += r * (r / D);
-            acc[3] += r * r;
+            acc[0] += r * (r / D);
+            acc[1] += r * r;
         }
     }
-    double out[4];
-    if (grid_reduce<4, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
-        PcgState* st = a.st;
-        st->bb = out[0];
-        st->rr_outer = out[1];
-        st->rz = out[2];
-        st->rr = out[3];
-        st->iter = 0;
-        st->total = 0;
-        st->beta = 0.0;
-        st->alpha = 0.0;
-        st->nan = (isfinite(out[0]) && isfinite(out[1])) ? 0 : 1;
-        const double tol2_outer = a.lin_tol * a.lin_tol * out[0];
-        st->outer_conv = (out[1] <= tol2_outer) ? 1 : 0;
-        if (MIXED) {
-            st->tol2 = a.inner_tol * a.inner_tol * out[3];
-            st->maxit = a.inner_maxit;
-            st->conv = (out[3] <= st->tol2) ? 1 : 0;
-            st->done = (st->outer_conv || st->conv || st->nan) ? 1 : 0;
-        } else {
-            st->tol2 = tol2_outer;
-            st->maxit = a.maxit;
-            st->conv = st->outer_conv;
-            st->done = (st->conv || st->nan || a.maxit <= 0) ? 1 : 0;
-        }
-    }
+    block_partials<4, false>(acc, a.part_out);
 }
 
 // FP64 iterative refinement: x <- x + d (pointwise), then r = b - A x in FP64 with the
@@ -332,7 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
 {
     const Geo g = a.g;
     const double* __restrict__ x = a.x;
-    double acc[3] = {0.0, 0.0, 0.0};   // rr64, rz32, rr32
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz32, rr32, (unused), rr64
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;   // next inner solve starts
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
         int i, j, k;
         decompose(g, c, i, j, k);
@@ -345,24 +323,11 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
         a.r32[c] = r32;
         a.d32[c] = 0.0f;
         const float z32 = r32 / a.D32[c];
-        acc[0] += r * r;
-        acc[1] += (double)r32 * (double)z32;
-        acc[2] += (double)r32 * (double)r32;
+        acc[3] += r * r;
+        acc[0] += (double)r32 * (double)z32;
+        acc[1] += (double)r32 * (double)r32;
     }
-    double out[3];
-    if (grid_reduce<3, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
-        PcgState* st = a.st;
-        st->rr_outer = out[0];
-        st->outer_conv = (out[0] <= a.lin_tol * a.lin_tol * st->bb) ? 1 : 0;
-        st->rz = out[1];
-        st->rr = out[2];
-        st->iter = 0;
-        st->beta = 0.0;
-        st->tol2 = a.inner_tol * a.inner_tol * out[2];
-        st->conv = (out[2] <= st->tol2) ? 1 : 0;
-        st->nan = isfinite(out[0]) ? 0 : 1;
-        st->done = (st->outer_conv || st->conv || st->nan) ? 1 : 0;
-    }
+    block_partials<4, false>(acc, a.part_out);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -372,16 +337,88 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
 // Stopping: ||r||_2 <= tol ||b||_2, checked before the first and after every iteration
 // (the order of cusp::monitor, cusp/detail/monitor.inl:178-200).
 // ---------------------------------------------------------------------------------------
+// Start of the SpMV kernel of iteration sc.it: sum the residual partials of the previous
+// kernel, decide whether the solve is over, and return beta.  Every block takes the same
+// decision from the same fixed-order sums; block 0 records it.
+__device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta)
+{
+    PcgState* st = sc.st;
+    if (st->done) return false;
+    double in[4];
+    sum_partials<4>(sc.part_in, sc.nparts, sc.it == 0 ? 4 : 2, in);
+    const double rz = in[0], rr = in[1];
+    const bool lead = lin_bid() == 0 && lin_tid() == 0;
+    double tol2;
+    int outer = 0, nan;
+    if (sc.it == 0) {
+        const double bb = sc.first_sweep ? in[2] : st->bb;
+        if (sc.mixed_inner) {
+            outer = (in[3] <= sc.lin_tol * sc.lin_tol * bb) ? 1 : 0;
+            tol2 = sc.inner_tol * sc.inner_tol * rr;
+        } else {
+            tol2 = sc.lin_tol * sc.lin_tol * bb;
+            outer = (rr <= tol2) ? 1 : 0;
+        }
+        nan = !(isfinite(rr) && isfinite(bb) && isfinite(in[3]));
+        if (lead) {
+            if (sc.first_sweep) st->bb = bb;
+            st->tol2 = tol2;
+            st->rr_outer = in[3];
+            st->outer_conv = outer;
+            st->nan = nan;
+        }
+    } else {
+        tol2 = st->tol2;
+        nan = !isfinite(rr);
+    }
+    const int conv = (rr <= tol2) ? 1 : 0;
+    if (outer || conv || nan || sc.it >= sc.maxit) {
+        if (lead) {
+            st->iter = sc.it;
+            st->conv = conv;
+            st->rr = rr;
+            st->nan = st->nan | nan;
+            if (!sc.mixed_inner) st->outer_conv = conv;
+            st->done = 1;
+        }
+        return false;
+    }
+    if (lead) st->rz[sc.it & 1] = rz;
+    beta = (sc.it == 0) ? 0.0 : rz / st->rz[(sc.it - 1) & 1];
+    return true;
+}
+
+// Start of the update kernel: alpha = (r, z) / (p, A p).
+__device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha)
+{
+    PcgState* st = sc.st;
+    if (st->done) return false;
+    double pq[1];
+    sum_partials<1>(sc.part_in, sc.nparts, 1, pq);
+    const double a = st->rz[sc.it & 1] / pq[0];
+    if (!(pq[0] != 0.0) || !isfinite(a)) {
+        if (lin_bid() == 0 && lin_tid() == 0) {
+            st->iter = sc.it;
+            st->conv = 0;
+            st->nan = 1;
+            st->done = 1;
+        }
+        return false;
+    }
+    alpha = a;
+    return true;
+}
+
 template <typename T, bool DIR>
 __global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __restrict__ flags,
                                                       const T* __restrict__ K, const T* __restrict__ D,
                                                       const T* __restrict__ r, const T* __restrict__ pold,
-                                                      T* __restrict__ pnew, T* __restrict__ q,
-                                                      PcgState* st, Reduce red)
+                                                      T* __restrict__ pnew, T* __restrict__ q, SolveCtl sc)
 {
-    if (st->done) return;
-    const bool first = (st->iter == 0);
-    const T beta = (T)st->beta;
+    double beta_d;
+    if (!pcg_begin(sc, beta_d)) return;
+    const bool first = (sc.it == 0);
+    const T beta = (T)beta_d;
     auto pf = [&](int c) -> T {
         const T z = r[c] / D[c];
         return first ? z : z + beta * pold[c];
@@ -397,27 +434,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __re
         q[c] = qc;
         acc[0] += (double)pc * (double)qc;
     }
-    double out[1];
-    if (grid_reduce<1, false>(acc, red.part, red.cnt, out) && lin_tid() == 0) {
-        st->pq = out[0];
-        const double al = st->rz / out[0];
-        if (!(out[0] != 0.0) || !isfinite(al)) {
-            st->done = 1;
-            st->nan = isfinite(out[0]) ? st->nan : 1;
-            st->alpha = 0.0;
-        } else {
-            st->alpha = al;
-        }
-    }
+    block_partials<1, false>(acc, sc.part_out);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x, T* __restrict__ r,
                                                        const T* __restrict__ D, const T* __restrict__ p,
-                                                       const T* __restrict__ q, PcgState* st, Reduce red)
+                                                       const T* __restrict__ q, SolveCtl sc)
 {
-    if (st->done) return;
-    const T alpha = (T)st->alpha;
+    double alpha_d;
+    if (!update_begin(sc, alpha_d)) return;
+    const T alpha = (T)alpha_d;
     double acc[2] = {0.0, 0.0};
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         const T xc = x[c] + alpha * p[c];
@@ -428,21 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
         acc[0] += (double)rc * (double)zc;
         acc[1] += (double)rc * (double)rc;
     }
-    double out[2];
-    if (grid_reduce<2, false>(acc, red.part, red.cnt, out) && lin_tid() == 0) {
-        const int it = st->iter + 1;
-        const double rz_old = st->rz;
-        st->rz = out[0];
-        st->rr = out[1];
-        st->beta = out[0] / rz_old;
-        st->iter = it;
-        st->total = st->total + 1;
-        const int conv = (out[1] <= st->tol2) ? 1 : 0;
-        const int nan = isfinite(out[1]) ? 0 : 1;
-        st->conv = conv;
-        st->nan = st->nan | nan;
-        st->done = (conv || nan || it >= st->maxit) ? 1 : 0;
-    }
+    block_partials<2, false>(acc, sc.part_out);
 }
 
 template <typename T, bool DIR>
@@ -500,10 +513,7 @@ __global__ __launch_bounds__(kBlock) void k_end_step(EndArgs a)
         a.psi_n[c] = xc;
         a.theta_n[c] = a.carry_reference ? a.theta_m[c] : thn;
     }
-    double out[9];
-    if (grid_reduce<9, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
-        for (int v = 0; v < 9; ++v) a.out[v] = out[v];
-    }
+    block_partials<9, false>(acc, a.part_out);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -582,7 +592,8 @@ __global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
     const double* __restrict__ H = a.H;
     const double* __restrict__ ke = a.ke;
     const double* __restrict__ kn = a.kn;
-    double acc[3] = {0.0, 0.0, 0.0};   // bb, rz, rr
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz, rr, bb, rr
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;   // a new solve starts
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
         const int j = c / nx, i = c - j * nx;
         double D, b, r;
@@ -605,26 +616,12 @@ __global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
         a.D[c] = D;
         a.b[c] = b;
         a.r[c] = r;
-        acc[0] += b * b;
-        acc[1] += r * (r / D);
-        acc[2] += r * r;
+        acc[0] += r * (r / D);
+        acc[1] += r * r;
+        acc[2] += b * b;
+        acc[3] += r * r;
     }
-    double out[3];
-    if (grid_reduce<3, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
-        PcgState* st = a.st;
-        st->bb = out[0];
-        st->rz = out[1];
-        st->rr = out[2];
-        st->iter = 0;
-        st->total = 0;
-        st->beta = 0.0;
-        st->alpha = 0.0;
-        st->tol2 = a.tol * a.tol * out[0];
-        st->maxit = a.maxit;
-        st->conv = (out[2] <= st->tol2) ? 1 : 0;
-        st->nan = isfinite(out[0]) ? 0 : 1;
-        st->done = (st->conv || st->nan || a.maxit <= 0) ? 1 : 0;
-    }
+    block_partials<4, false>(acc, a.part_out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg2_spmv(Geo2 g, const double* __restrict__ ke,
@@ -633,11 +630,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg2_spmv(Geo2 g, const double* __re
                                                       const double* __restrict__ r,
                                                       const double* __restrict__ pold,
                                                       double* __restrict__ pnew, double* __restrict__ q,
-                                                      PcgState* st, Reduce red)
+                                                      SolveCtl sc)
 {
-    if (st->done) return;
-    const bool first = (st->iter == 0);
-    const double beta = st->beta;
+    double beta;
+    if (!pcg_begin(sc, beta)) return;
+    const bool first = (sc.it == 0);
     auto pf = [&](int c) -> double {
         const double z = r[c] / D[c];
         return first ? z : z + beta * pold[c];
@@ -650,18 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg2_spmv(Geo2 g, const double* __re
         q[c] = qc;
         acc[0] += pc * qc;
     }
-    double out[1];
-    if (grid_reduce<1, false>(acc, red.part, red.cnt, out) && lin_tid() == 0) {
-        st->pq = out[0];
-        const double al = st->rz / out[0];
-        if (!(out[0] != 0.0) || !isfinite(al)) {
-            st->done = 1;
-            st->nan = isfinite(out[0]) ? st->nan : 1;
-            st->alpha = 0.0;
-        } else {
-            st->alpha = al;
-        }
-    }
+    block_partials<1, false>(acc, sc.part_out);
 }
 
 // WaterDepthOverland (src/overland.cu:371-383) and the discharge into the Dirichlet ring.
@@ -686,11 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_olf_post(OlfArgs a)
         }
         a.h[c] = fmax(H[c] - a.z[c], 0.0);
     }
-    double out[2];
-    if (grid_reduce<2, false>(acc, a.red.part, a.red.cnt, out) && lin_tid() == 0) {
-        a.out[0] = out[0];
-        a.out[1] = out[1];
-    }
+    block_partials<2, false>(acc, a.part_out);
 }
 
 }  // namespace
@@ -758,68 +740,46 @@ void launch_end_step(hipStream_t s, const EndArgs& a)
 
 template <typename T>
 static void pcg3_spmv(hipStream_t s, const Geo& g, const uint8_t* flags, const T* K, const T* D,
-                      const T* r, T* const p[2], T* q, int parity, PcgState* st, Reduce red)
+                      const T* r, T* const p[2], T* q, const SolveCtl& sc)
 {
     const dim3 gr(grid_for(g.n)), bl(kBlock);
-    const T* pold = p[parity & 1];
-    T* pnew = p[(parity + 1) & 1];
-    if (flags) hipLaunchKernelGGL((k_pcg3_spmv<T, true>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, st, red);
-    else hipLaunchKernelGGL((k_pcg3_spmv<T, false>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, st, red);
+    const T* pold = p[sc.it & 1];
+    T* pnew = p[(sc.it + 1) & 1];
+    if (flags) hipLaunchKernelGGL((k_pcg3_spmv<T, true>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, sc);
+    else hipLaunchKernelGGL((k_pcg3_spmv<T, false>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, sc);
 }
 
 template <typename T>
 static void pcg_update(hipStream_t s, int n, T* x, T* r, const T* D, const T* p, const T* q,
-                       PcgState* st, Reduce red)
+                       const SolveCtl& sc)
 {
-    hipLaunchKernelGGL((k_pcg_update<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, D, p, q, st, red);
-}
-
-template <typename T>
-static void pcg3_iter(hipStream_t s, const Geo& g, const uint8_t* flags, const T* K, const T* D,
-                      T* x, T* r, T* const p[2], T* q, int parity, PcgState* st, Reduce red)
-{
-    pcg3_spmv<T>(s, g, flags, K, D, r, p, q, parity, st, red);
-    pcg_update<T>(s, g.n, x, r, D, p[(parity + 1) & 1], q, st, red);
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, D, p, q, sc);
 }
 
 void launch_pcg3_spmv_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                           const double* D, const double* r, double* const p[2], double* q,
-                          int parity, PcgState* st, Reduce red)
+                          const SolveCtl& sc)
 {
-    pcg3_spmv<double>(s, g, flags, K, D, r, p, q, parity, st, red);
+    pcg3_spmv<double>(s, g, flags, K, D, r, p, q, sc);
 }
 
 void launch_pcg3_spmv_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
                           const float* D, const float* r, float* const p[2], float* q,
-                          int parity, PcgState* st, Reduce red)
+                          const SolveCtl& sc)
 {
-    pcg3_spmv<float>(s, g, flags, K, D, r, p, q, parity, st, red);
+    pcg3_spmv<float>(s, g, flags, K, D, r, p, q, sc);
 }
 
 void launch_pcg_update_f64(hipStream_t s, int n, double* x, double* r, const double* D,
-                           const double* p, const double* q, PcgState* st, Reduce red)
+                           const double* p, const double* q, const SolveCtl& sc)
 {
-    pcg_update<double>(s, n, x, r, D, p, q, st, red);
+    pcg_update<double>(s, n, x, r, D, p, q, sc);
 }
 
 void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float* D,
-                           const float* p, const float* q, PcgState* st, Reduce red)
+                           const float* p, const float* q, const SolveCtl& sc)
 {
-    pcg_update<float>(s, n, x, r, D, p, q, st, red);
-}
-
-void launch_pcg3_iter_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
-                          const double* D, double* x, double* r, double* const p[2], double* q,
-                          int parity, PcgState* st, Reduce red)
-{
-    pcg3_iter<double>(s, g, flags, K, D, x, r, p, q, parity, st, red);
-}
-
-void launch_pcg3_iter_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
-                          const float* D, float* x, float* r, float* const p[2], float* q,
-                          int parity, PcgState* st, Reduce red)
-{
-    pcg3_iter<float>(s, g, flags, K, D, x, r, p, q, parity, st, red);
+    pcg_update<float>(s, n, x, r, D, p, q, sc);
 }
 
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
@@ -859,16 +819,13 @@ void launch_olf_assemble(hipStream_t s, const OlfArgs& a)
     hipLaunchKernelGGL(k_olf_assemble, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
 }
 
-void launch_pcg2_iter(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
-                      const double* D, double* x, double* r, double* const p[2], double* q,
-                      int parity, PcgState* st, Reduce red)
+void launch_pcg2_spmv(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
+                      const double* D, const double* r, double* const p[2], double* q,
+                      const SolveCtl& sc)
 {
-    const dim3 gr(grid_for(g.n)), bl(kBlock);
-    const double* pold = p[parity & 1];
-    double* pnew = p[(parity + 1) & 1];
-    hipLaunchKernelGGL(k_pcg2_spmv, gr, bl, 0, s, g, ke, kn, D, r, pold, pnew, q, st, red);
-    hipLaunchKernelGGL((k_pcg_update<double>), gr, bl, 0, s, g.n, x, r, D, (const double*)pnew,
-                       (const double*)q, st, red);
+    const double* pold = p[sc.it & 1];
+    double* pnew = p[(sc.it + 1) & 1];
+    hipLaunchKernelGGL(k_pcg2_spmv, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, ke, kn, D, r, pold, pnew, q, sc);
 }
 
 void launch_olf_post(hipStream_t s, const OlfArgs& a)
