@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--refine-max", type=int, default=None, help="mixed: FP64 refinement sweeps per Picard iteration")
+    ap.add_argument("--inner-tol", type=float, default=None, help="mixed: FP32 inner relative tolerance")
+    ap.add_argument("--mixed-lin-tol", type=float, default=None, help="mixed: FP64 residual target")
     return ap.parse_args()
 
 
@@ -99,19 +102,29 @@ def main():
 
     w64 = inputs.c2_coupled(nx=nx, ny=ny, nz=nz)
     w32 = inputs.c3_coupled_mixed(nx=nx, ny=ny, nz=nz)
+    if args.refine_max is not None:
+        w32.params.refine_maxiter = args.refine_max
+    if args.inner_tol is not None:
+        w32.params.inner_tol = args.inner_tol
+    if args.mixed_lin_tol is not None:
+        w32.params.lin_tol = args.mixed_lin_tol
     runs = {}
     order = [("fp64", w64), ("mixed", w32)]
     if args.only != "both":
         order = [o for o in order if o[0] == args.only]
     cpu_start = min(args.cpu_start, max(K - args.cpu_steps, 0))
     cpu_marks = (cpu_start, cpu_start + args.cpu_steps)
+    tiles = {}
     for name, w in order:
-        if W > 0:                                  # warm-up on a throw-away context
-            m = dhara_amd.FlowModel(w.grid, cp(w.params))
-            w.apply(m)
-            w.run(m, W)
-            m.close()
+        # warm-up on a throw-away context, then the tile autotuner on its operator; the tuned
+        # shape goes to the library's process-wide table, which the timed context reads
         m = dhara_amd.FlowModel(w.grid, cp(w.params))
+        w.apply(m)
+        w.run(m, max(W, 1))
+        tune_us = m.autotune()
+        m.close()
+        m = dhara_amd.FlowModel(w.grid, cp(w.params))
+        tiles[name] = {"fp64": m.get_tile(8), "fp32": m.get_tile(4), "tune_us": tune_us}
         w.apply(m)
         m.profile(True)
         print(f"# {name}: {K} steps", file=sys.stderr, flush=True)
@@ -136,7 +149,10 @@ def main():
         "data": "synthetic (seeded GRF topography and log-normal Ks; no external data)",
         "config": {"workload": f"coupled {nx}x{ny}x{nz} (surface {nx}x{ny}), 20 mm/h rain 3600 s + "
                                f"3600 s recession, dt 30 s; {head} run timed over steps 0..{K - 1}",
-                   "precision": head, "device": dev},
+                   "precision": head, "device": dev, "spmv_tiles": tiles,
+                   "mixed_solver": {"refine_maxiter": w32.params.refine_maxiter, "inner_tol": w32.params.inner_tol,
+                                    "lin_tol": w32.params.lin_tol},
+                   "fp64_solver": {"lin_tol": w64.params.lin_tol}},
     }
     for name, rr in runs.items():
         st = rr["stats"]

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdhara_hip.so")
-SOURCES = ["kernels.hip", "capi.hip"]
+SOURCES = ["kernels.hip", "capi.hip", "shim.cpp"]
 ARCH = os.environ.get("DHARA_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -31,7 +31,8 @@ def hipcc():
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + FLAGS + ["-o", tmp] + srcs
+    # shim.cpp is host C++ (no kernels); hipcc compiles it as HIP for the vector types
+    cmd = [hipcc()] + FLAGS + ["-o", tmp] + ["-x", "hip"] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -42,5 +43,21 @@ def build(verbose=False):
     return OUT
 
 
+def build_shim_driver(verbose=False):
+    """The reference-shaped host test program (tests/shim/shim_driver), linked against the
+    in-tree libdhara_hip.so through the reference-named shims."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "shim", "shim_driver.cpp")
+    out = os.path.join(root, "tests", "shim", "shim_driver")
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + os.path.join(root, "include"), src, "-o", out, "-L" + HERE, "-ldhara_hip",
+           "-Wl,-rpath,$ORIGIN/../../dhara_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_shim_driver(verbose=True))

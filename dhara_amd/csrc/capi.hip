@@ -70,6 +70,8 @@ struct dhara_ctx {
     bool theta_m_current = true;
     int pcg_hint = 8, olf_hint = 8;
     std::vector<void*> allocs;
+    // SpMV tile shapes: [0] FP64 solves, [1] FP32 inner solves (tx = 0: untiled kernel)
+    dh::TileCfg tile[2] = {{32, 8, 0}, {32, 8, 0}};
     // kernel profiling (dhara_profile_enable)
     bool prof = false;
     std::vector<hipEvent_t> ev;
@@ -78,6 +80,27 @@ struct dhara_ctx {
 
 namespace {
 constexpr int kMaxBatch = 64;   // Krylov iterations queued between two host checks
+
+struct TuneKey {
+    int nx, ny, nz, elem;
+    bool operator==(const TuneKey& o) const { return nx == o.nx && ny == o.ny && nz == o.nz && elem == o.elem; }
+};
+struct TuneEntry {
+    TuneKey key;
+    dh::TileCfg cfg;
+};
+std::vector<TuneEntry> g_tune_table;   // process-wide autotuner results
+
+// Default tile: 32 x 8 columns (128 B rows in FP32), planes per block chosen for ~2048 blocks.
+dh::TileCfg default_tile(const dh::Geo& g)
+{
+    dh::TileCfg t{32, 8, 1};
+    const int ntiles = ((g.nx + 31) / 32) * ((g.ny + 7) / 8);
+    int nkz = std::max(1, std::min(g.nz, (2048 + ntiles - 1) / ntiles));
+    t.kc = (g.nz + nkz - 1) / nkz;
+    while (dh::tile_blocks(g, t) > dh::kMaxBlocks) ++t.kc;
+    return t;
+}
 }
 
 namespace {
@@ -302,6 +325,7 @@ dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bo
     sc.maxit = maxit;
     sc.mixed_inner = mixed_inner ? 1 : 0;
     sc.first_sweep = first_sweep ? 1 : 0;
+    sc.segment_start = 0;
     sc.lin_tol = mixed_inner ? c->p.lin_tol : c->p.lin_tol;
     sc.inner_tol = c->p.inner_tol;
     sc.part_in = c->part[0];
@@ -318,10 +342,13 @@ dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bo
 // only the launches that did work (the first ones of the batch) are accumulated.
 template <class SpmvFn, class UpdFn>
 int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
-               SpmvFn spmv, UpdFn upd)
+               SpmvFn spmv, UpdFn upd, int it_start = 0)
 {
-    int it = 0;
-    int batch = std::max(4, std::min(kMaxBatch, hint));
+    int it = it_start;
+    // first batch: the previous solve's count (the next one is usually within +-2 of it);
+    // then small batches, since an over-queued iteration costs two empty launches and an
+    // under-queued one a host round trip
+    int batch = std::max(2, std::min(kMaxBatch, hint));
     for (;;) {
         const int it0 = it;
         for (int b = 0; b < batch; ++b, ++it) {
@@ -352,10 +379,9 @@ int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, 
         }
         if (c->h_st->done) break;
         if (it > maxit + 1) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
-        const int left = hint - it;
-        batch = std::max(4, std::min(16, left > 0 ? left : 8));
+        batch = 3;
     }
-    hint = std::max(1, c->h_st->iter);
+    hint = std::max(1, c->h_st->iter - it_start);
     return DHARA_OK;
 }
 
@@ -367,17 +393,21 @@ int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
     double* const pp[2] = {c->pv[0], c->pv[1]};
     const uint8_t* fl = dir ? c->flags : nullptr;
     dh::SolveCtl sc = solve_ctl(c, c->st_sub, c->p.lin_maxiter, dh::grid_for(c->N), false, true);
+    const dh::TileCfg tc = c->tile[0];
+    const int spmv_parts = tc.tx ? dh::tile_blocks(c->g, tc) : dh::grid_for(c->N);
     int rc = run_krylov(
         c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, c->prof, 8.0,
         [&](int it) {
             dh::SolveCtl k = sc;
             k.it = it;
-            dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, k);
+            if (tc.tx) dh::launch_pcg3_spmv_tiled_f64(c->s, c->g, tc, fl, c->K, c->D, c->r, pp, c->q, k);
+            else dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, k);
         },
         [&](int it) {
             dh::SolveCtl k = sc;
             k.it = it;
             k.part_in = c->part[1];
+            k.nparts = spmv_parts;
             k.part_out = c->part[0];
             dh::launch_pcg_update_f64(c->s, c->N, c->x, c->r, c->D, pp[(it + 1) & 1], c->q, k);
         });
@@ -388,10 +418,13 @@ int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
     return DHARA_OK;
 }
 
-// Mixed precision: the FP64 assembly gives r64 = b - A x; each sweep solves A d = r in FP32
-// (Jacobi-PCG to inner_tol relative), then x += d and r64 = b - A x in FP64.  The first SpMV
-// kernel of every inner solve checks the FP64 residual against lin_tol ||b|| and ends the
-// solve at once when it is met.
+// Mixed precision: FP32 Jacobi-PCG with FP64 reliable updates.  The FP64 assembly gives
+// r64 = b - A x; one FP32 CG recurrence runs on the correction d (x + d is the iterate).  When
+// the FP32 residual has dropped by inner_tol since the last update, d is folded into the FP64
+// x, the true residual b - A x is recomputed in FP64 and replaces the FP32 one, and the same
+// recurrence continues with its search direction kept (beta from the new residual) -- no
+// restart, so the iteration count stays that of the FP64 solver.  The first SpMV kernel after
+// every update checks the FP64 residual against lin_tol ||b||.
 int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
 {
     const bool dir = has_dirichlet(c);
@@ -411,36 +444,58 @@ int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
     ra.d32 = c->d32;
     ra.part_out = c->part[0];
     ra.st = c->st_sub;
-    int sweeps = 0;
+    const int maxit = c->p.inner_maxiter;
+    const dh::TileCfg tc = c->tile[1];
+    const int spmv_parts = tc.tx ? dh::tile_blocks(c->g, tc) : dh::grid_for(c->N);
+    dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, true);
+    int it = 0, updates = 0;
     bool outer_conv = false;
     for (;;) {
-        // after the last allowed sweep only the convergence check runs (maxit 0)
-        const int maxit = sweeps < c->p.refine_maxiter ? c->p.inner_maxiter : 0;
-        dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, sweeps == 0);
         int rc = run_krylov(
-            c, c->st_sub, maxit, c->pcg_hint, c->prof && maxit > 0, 4.0,
-            [&](int it) {
+            c, c->st_sub, maxit, c->pcg_hint, c->prof, 4.0,
+            [&](int k_it) {
                 dh::SolveCtl k = sc;
-                k.it = it;
-                dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+                k.it = k_it;
+                k.segment_start = (k_it == it) ? 1 : 0;
+                if (tc.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, tc, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+                else dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
             },
-            [&](int it) {
+            [&](int k_it) {
                 dh::SolveCtl k = sc;
-                k.it = it;
+                k.it = k_it;
                 k.part_in = c->part[1];
+                k.nparts = spmv_parts;
                 k.part_out = c->part[0];
-                dh::launch_pcg_update_f32(c->s, c->N, c->d32, c->r32, c->D32, pp[(it + 1) & 1], c->q32, k);
-            });
+                dh::launch_pcg_update_f32(c->s, c->N, c->d32, c->r32, c->D32, pp[(k_it + 1) & 1], c->q32, k);
+            },
+            it);
         if (rc) return rc;
         if (c->h_st->nan) { st->nan_seen = 1; break; }
-        if (c->h_st->outer_conv) { outer_conv = true; break; }
-        if (maxit == 0) break;
-        st->lin_iters += c->h_st->iter;
-        ++sweeps;
+        if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
+        const int it_end = c->h_st->iter;
+        // fold d into x and replace r by the FP64 residual (also after the iteration limit,
+        // so that x carries every correction)
         dh::launch_refine(c->s, ra);
         LAUNCHCHK(c);
+        ++updates;
+        if (it_end >= maxit || updates > c->p.refine_maxiter) {
+            // final convergence check only: a segment start at the limit ends the solve
+            dh::SolveCtl k = sc;
+            k.it = it_end;
+            k.maxit = it_end;
+            k.segment_start = 1;
+            if (tc.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, tc, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+            else dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+            LAUNCHCHK(c);
+            if ((rc = read_state(c, c->st_sub))) return rc;
+            outer_conv = c->h_st->outer_conv != 0;
+            break;
+        }
+        it = it_end;
+        sc.first_sweep = 0;
     }
-    st->refine_sweeps += sweeps;
+    st->lin_iters += c->h_st->iter;
+    st->refine_sweeps += updates;
     st->lin_converged &= outer_conv ? 1 : 0;
     return DHARA_OK;
 }
@@ -688,6 +743,11 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
     g2.nx = grid->nx; g2.ny = grid->ny; g2.n = c->S; g2.dx = grid->dx; g2.dy = grid->dy;
     g2.cx = params->dt / (grid->dx * grid->dx);
     g2.cy = params->dt / (grid->dy * grid->dy);
+    for (int e = 0; e < 2; ++e) {
+        c->tile[e] = default_tile(g);
+        for (const TuneEntry& te : g_tune_table)
+            if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4}) c->tile[e] = te.cfg;
+    }
 
     const size_t N = c->N, S = c->S;
     double** d3[] = {&c->psi_n, &c->psi_m, &c->x, &c->theta_n, &c->theta_m, &c->theta_new, &c->K,
@@ -711,6 +771,7 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
             return DHARA_E_DEVICE;
         }
     }
+    if (grid->nroot_groups > dh::kMaxSums * dh::kMaxBlocks) { dhara_destroy(c); return DHARA_E_ARG; }
     if ((rc = dalloc(c, &c->procmap, S)) || (rc = dalloc(c, &c->root, (size_t)grid->nroot_groups * grid->nz)) ||
         (rc = dalloc(c, &c->flags, N)) || (rc = dalloc(c, &c->part[0], (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
         (rc = dalloc(c, &c->part[1], (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
@@ -875,6 +936,23 @@ int dhara_download_field_device(dhara_ctx* c, int32_t f, void* dev, size_t n)
     return DHARA_OK;
 }
 
+int dhara_scatter_group_field(dhara_ctx* c, int32_t f, const double* values, size_t ngroups)
+{
+    if (!c || !values) return DHARA_E_ARG;
+    if (f != DHARA_F_RAIN && f != DHARA_F_EVAP && f != DHARA_F_TRANSP)
+        return set_err(c, DHARA_E_ARG, "group scatter applies to RAIN, EVAP or TRANSP");
+    if (ngroups != (size_t)c->grid.nroot_groups) return set_err(c, DHARA_E_ARG, "need nroot_groups values");
+    size_t n = 0;
+    double* dst = dfield(c, f, &n);
+    // stage the group values in the partial buffer B (free between kernels of a solve)
+    HIPCHK(c, hipMemcpyAsync(c->part[1], values, ngroups * sizeof(double), hipMemcpyHostToDevice, c->s));
+    dh::launch_group_scatter(c->s, c->S, c->procmap, c->part[1], dst);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (f == DHARA_F_TRANSP) c->has_transp = true;
+    return DHARA_OK;
+}
+
 int dhara_init_state(dhara_ctx* c, int32_t mode)
 {
     if (!c) return DHARA_E_ARG;
@@ -944,6 +1022,134 @@ int dhara_step(dhara_ctx* c, int32_t nsteps, dhara_step_stats* st)
         if (st) st[s] = tmp;
         if (rc) return rc;
     }
+    return DHARA_OK;
+}
+
+int dhara_set_tile(dhara_ctx* c, int32_t elem_bytes, int32_t tx, int32_t ty, int32_t kc)
+{
+    if (!c || (elem_bytes != 8 && elem_bytes != 4)) return DHARA_E_ARG;
+    dh::TileCfg t{tx, ty, kc};
+    if (tx != 0) {
+        if (!dh::tile_supported(t)) return set_err(c, DHARA_E_ARG, "unsupported tile shape");
+        if (dh::tile_blocks(c->g, t) > dh::kMaxBlocks) return set_err(c, DHARA_E_ARG, "tile grid too large; raise kc");
+    }
+    c->tile[elem_bytes == 8 ? 0 : 1] = t;
+    return DHARA_OK;
+}
+
+int dhara_get_tile(const dhara_ctx* c, int32_t elem_bytes, int32_t* tx, int32_t* ty, int32_t* kc)
+{
+    if (!c || (elem_bytes != 8 && elem_bytes != 4) || !tx || !ty || !kc) return DHARA_E_ARG;
+    const dh::TileCfg& t = c->tile[elem_bytes == 8 ? 0 : 1];
+    *tx = t.tx;
+    *ty = t.ty;
+    *kc = t.kc;
+    return DHARA_OK;
+}
+
+}  // extern "C"
+
+// Times one SpMV launch configuration: 3 warm-up + 10 timed launches on a scratch solve
+// state that never converges (tol2 < 0), reading the real K and D and the current residual.
+template <typename T, class Launch>
+static int time_spmv(dhara_ctx* c, Launch launch, float* us)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    for (int rep = 0; rep < 13; ++rep) {
+        if (rep == 3) HIPCHK(c, hipEventRecord(e0, c->s));
+        launch(1 + (rep & 1));
+    }
+    HIPCHK(c, hipEventRecord(e1, c->s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    LAUNCHCHK(c);
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    *us = ms * 1e3f / 10.f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return DHARA_OK;
+}
+
+extern "C" {
+
+int dhara_autotune(dhara_ctx* c, double* best_us)
+{
+    if (!c) return DHARA_E_ARG;
+    const bool mixed = c->p.precision == DHARA_MIXED;
+    if (mixed) {
+        int rc = alloc_mixed(c);
+        if (rc) return rc;
+        dh::launch_cast_f32(c->s, c->N, c->D, c->D32);
+        dh::launch_cast_f32(c->s, c->N, c->r, c->r32);
+    }
+    // scratch solve state: it >= 1 reads (rz, rr) partials = 1, rz[0] = rz[1] = 1, tol2 = -1
+    dh::PcgState tune{};
+    tune.rz[0] = tune.rz[1] = 1.0;
+    tune.tol2 = -1.0;
+    dh::PcgState* st = nullptr;
+    int rc = dalloc(c, &st, 1);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(st, &tune, sizeof(tune), hipMemcpyHostToDevice, c->s));
+    std::vector<double> ones(2 * dh::kMaxBlocks, 1.0);
+    HIPCHK(c, hipMemcpyAsync(c->part[0], ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    const int shapes[5][2] = {{32, 8}, {64, 4}, {16, 16}, {32, 4}, {64, 8}};
+    std::vector<int> kcs = {c->g.nz, (c->g.nz + 1) / 2, (c->g.nz + 2) / 3, (c->g.nz + 3) / 4, 8, 4};
+    const uint8_t* fl = has_dirichlet(c) ? c->flags : nullptr;
+    double best[2] = {0.0, 0.0};
+    for (int e = 0; e < 2; ++e) {
+        if (e == 1 && !mixed) continue;
+        float best_t = 1e30f;
+        dh::TileCfg best_cfg{0, 0, 0};
+        auto consider = [&](const dh::TileCfg& t) -> int {
+            dh::SolveCtl sc{};
+            sc.maxit = 1 << 30;
+            sc.part_in = c->part[0];
+            sc.nparts = dh::grid_for(c->N);
+            sc.part_out = c->part[1];
+            sc.st = st;
+            float us = 0.f;
+            int r2;
+            if (e == 0) {
+                double* const pp[2] = {c->pv[0], c->pv[1]};
+                r2 = time_spmv<double>(c, [&](int it) {
+                    dh::SolveCtl k = sc;
+                    k.it = it;
+                    if (t.tx) dh::launch_pcg3_spmv_tiled_f64(c->s, c->g, t, fl, c->K, c->D, c->r, pp, c->q, k);
+                    else dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, k);
+                }, &us);
+            } else {
+                float* const pp[2] = {c->p32[0], c->p32[1]};
+                r2 = time_spmv<float>(c, [&](int it) {
+                    dh::SolveCtl k = sc;
+                    k.it = it;
+                    if (t.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, t, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+                    else dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+                }, &us);
+            }
+            if (r2) return r2;
+            if (us < best_t) { best_t = us; best_cfg = t; }
+            return DHARA_OK;
+        };
+        if ((rc = consider(dh::TileCfg{0, 0, 0}))) return rc;
+        for (const auto& sh : shapes)
+            for (int kc : kcs) {
+                dh::TileCfg t{sh[0], sh[1], std::max(1, std::min(kc, c->g.nz))};
+                if (dh::tile_blocks(c->g, t) > dh::kMaxBlocks) continue;
+                if ((rc = consider(t))) return rc;
+            }
+        c->tile[e] = best_cfg;
+        best[e] = best_t;
+        const TuneKey key{c->g.nx, c->g.ny, c->g.nz, e == 0 ? 8 : 4};
+        bool found = false;
+        for (TuneEntry& te : g_tune_table)
+            if (te.key == key) { te.cfg = best_cfg; found = true; }
+        if (!found) g_tune_table.push_back(TuneEntry{key, best_cfg});
+    }
+    // the scratch state stays in c->allocs until dhara_destroy
+    if (best_us) { best_us[0] = best[0]; best_us[1] = best[1]; }
     return DHARA_OK;
 }
 

@@ -45,6 +45,9 @@ struct SolveCtl {
     int maxit;
     int mixed_inner;      // inner FP32 solve of the mixed path
     int first_sweep;      // ||b||^2 is taken from part_in (first solve of a Picard iteration)
+    int segment_start;    // part_in holds a fresh residual (assembly or FP64 reliable update):
+                          // read all four partials, re-check the outer residual, restart the
+                          // inner target, keep the search direction
     double lin_tol, inner_tol;
     const double* part_in;
     int nparts;
@@ -148,6 +151,7 @@ void launch_vg_picard(hipStream_t s, const VgArgs& a);          // + increment +
 void launch_vg_inverse(hipStream_t s, const Geo& g, const Soil& soil, const double* theta,
                        double* psi);
 void launch_row_flags(hipStream_t s, const Geo& g, const int* const bctype[6], uint8_t* flags);
+void launch_group_scatter(hipStream_t s, int nxy, const int* procmap, const double* groups, double* out);
 void launch_surface_pre(hipStream_t s, const SurfArgs& a);
 void launch_surface_post(hipStream_t s, const SurfArgs& a);
 void launch_assemble(hipStream_t s, const AsmArgs& a, bool mixed);
@@ -167,6 +171,20 @@ void launch_pcg_update_f64(hipStream_t s, int n, double* x, double* r, const dou
                            const double* p, const double* q, const SolveCtl& sc);
 void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float* D,
                            const float* p, const float* q, const SolveCtl& sc);
+// LDS-tiled 2.5D variant of the SpMV kernel: a TX x TY tile of columns marches through a
+// chunk of kc planes; the p/K planes are staged in LDS with a one-cell halo, the vertical
+// neighbours stay in registers.  Returns the number of blocks (= partials for the update).
+struct TileCfg {
+    int tx, ty, kc;
+};
+bool tile_supported(const TileCfg& t);
+int tile_blocks(const Geo& g, const TileCfg& t);
+void launch_pcg3_spmv_tiled_f64(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
+                                const double* K, const double* D, const double* r, double* const p[2],
+                                double* q, const SolveCtl& sc);
+void launch_pcg3_spmv_tiled_f32(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
+                                const float* K, const float* D, const float* r, float* const p[2],
+                                float* q, const SolveCtl& sc);
 // y = A x (tests)
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                        const double* D, const double* x, double* y);

@@ -146,6 +146,14 @@ __global__ __launch_bounds__(kBlock) void k_row_flags(Geo g, BcTypes bt, uint8_t
     }
 }
 
+// SendFluxDataToGrids (src/subsurface.cu:364-378): group value -> surface cells.
+__global__ __launch_bounds__(kBlock) void k_group_scatter(int nxy, const int* procmap, const double* groups,
+                                                          double* out)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxy; t += gridDim.x * blockDim.x)
+        out[t] = groups[procmap[t]];
+}
+
 // EstimateFluxes (src/global.cu:172-191) + IdentifyTopBoundary (206-257).  Both only use
 // quantities fixed within a step, so the top flux is computed once per step.
 __global__ __launch_bounds__(kBlock) void k_surface_pre(SurfArgs a)
@@ -344,13 +352,14 @@ __device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta)
 {
     PcgState* st = sc.st;
     if (st->done) return false;
+    const bool fresh = sc.it == 0 || sc.segment_start;
     double in[4];
-    sum_partials<4>(sc.part_in, sc.nparts, sc.it == 0 ? 4 : 2, in);
+    sum_partials<4>(sc.part_in, sc.nparts, fresh ? 4 : 2, in);
     const double rz = in[0], rr = in[1];
     const bool lead = lin_bid() == 0 && lin_tid() == 0;
     double tol2;
     int outer = 0, nan;
-    if (sc.it == 0) {
+    if (fresh) {
         const double bb = sc.first_sweep ? in[2] : st->bb;
         if (sc.mixed_inner) {
             outer = (in[3] <= sc.lin_tol * sc.lin_tol * bb) ? 1 : 0;
@@ -456,6 +465,135 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
         acc[1] += (double)rc * (double)rc;
     }
     block_partials<2, false>(acc, sc.part_out);
+}
+
+// LDS-tiled 2.5D SpMV.  Block = TX x TY threads owning TX x TY columns; the block marches
+// through planes k0..k0+kc-1.  For each plane the new search direction p = M^-1 r + beta p
+// and the conductivity K are written into a (TY+2) x (TX+2) LDS tile (the one-cell halo is
+// computed by the first 2(TX+TY) threads); the k-1 / k+1 values of the thread's own column
+// come from registers (one plane of look-ahead).  Two LDS buffers alternate by plane parity,
+// so one barrier per plane suffices.  Blocks are numbered so that each XCD (workgroups are
+// dealt to the 8 XCDs round-robin) gets a contiguous band of tiles: neighbouring tiles, whose
+// halos overlap, then share that XCD's L2.
+constexpr int kXcd = 8;
+
+__device__ __forceinline__ int xcd_tile(int w, int nb)
+{
+    // workgroup w runs on XCD w % 8; give XCD x the tiles [x*q + min(x, rem), ...)
+    const int x = w % kXcd, slot = w / kXcd;
+    const int q = nb / kXcd, rem = nb % kXcd;
+    return x * q + (x < rem ? x : rem) + slot;
+}
+
+template <typename T, bool DIR, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_t* __restrict__ flags,
+                                                             const T* __restrict__ K, const T* __restrict__ D,
+                                                             const T* __restrict__ r, const T* __restrict__ pold,
+                                                             T* __restrict__ pnew, T* __restrict__ q,
+                                                             SolveCtl sc, int kc, int ntx, int nty, int nkz)
+{
+    double beta_d;
+    if (!pcg_begin(sc, beta_d)) return;
+    const bool first = (sc.it == 0);
+    const T beta = (T)beta_d;
+    __shared__ T sp[2][TY + 2][TX + 2];
+    __shared__ T sk[2][TY + 2][TX + 2];
+
+    const int nb = ntx * nty * nkz;
+    const int t = xcd_tile(blockIdx.x, nb);
+    const int bz = t % nkz;
+    const int bxy = t / nkz;
+    const int bx = bxy % ntx, by = bxy / ntx;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * TX + tx, j = by * TY + ty;
+    const int k0 = bz * kc;
+    const int k1 = min(k0 + kc, g.nz);
+    const bool in = (i < g.nx) && (j < g.ny);
+    const int col = j * g.nx + i;
+
+    // halo cell of this thread (if any): top row, bottom row, left column, right column
+    const int lid = ty * TX + tx;
+    int hx = -2, hy = -2;
+    if (lid < TX) { hx = lid; hy = -1; }
+    else if (lid < 2 * TX) { hx = lid - TX; hy = TY; }
+    else if (lid < 2 * TX + TY) { hx = -1; hy = lid - 2 * TX; }
+    else if (lid < 2 * TX + 2 * TY) { hx = TX; hy = lid - 2 * TX - TY; }
+    const int hi = bx * TX + hx, hj = by * TY + hy;
+    const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
+    const int hcol = hj * g.nx + hi;
+
+    auto pval = [&](int c) -> T {
+        const T z = r[c] / D[c];
+        return first ? z : z + beta * pold[c];
+    };
+    const T wx = (T)g.wx, wy = (T)g.wy, wz = (T)g.wz, half = T(0.5);
+
+    T pm = T(0), km = T(0), pc = T(0), kcur = T(0), dc = T(1);
+    if (in) {
+        if (k0 > 0) {
+            const int c = (k0 - 1) * g.nxy + col;
+            pm = pval(c);
+            km = K[c];
+        }
+        if (k0 < k1) {
+            const int c = k0 * g.nxy + col;
+            dc = D[c];
+            const T z = r[c] / dc;
+            pc = first ? z : z + beta * pold[c];
+            kcur = K[c];
+        }
+    }
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        // look-ahead: plane k+1 of the own column
+        T pn = T(0), kn = T(0), dn = T(1);
+        if (in && k + 1 < g.nz) {
+            const int c = (k + 1) * g.nxy + col;
+            dn = D[c];
+            const T z = r[c] / dn;
+            pn = first ? z : z + beta * pold[c];
+            kn = K[c];
+        }
+        const int buf = k & 1;
+        sp[buf][ty + 1][tx + 1] = pc;
+        sk[buf][ty + 1][tx + 1] = kcur;
+        if (has_halo) {
+            const int c = k * g.nxy + hcol;
+            sp[buf][hy + 1][hx + 1] = pval(c);
+            sk[buf][hy + 1][hx + 1] = K[c];
+        }
+        __syncthreads();
+        if (in) {
+            const int c = k * g.nxy + col;
+            T qv;
+            const unsigned fl = DIR ? flags[c] : 0u;
+            if (DIR && (fl & RF_IDENTITY)) {
+                qv = pc;
+            } else {
+                qv = dc * pc;
+                if (k > 0 && !(DIR && (fl & RF_CUT_D))) qv -= wz * (half * (kcur + km)) * pm;
+                if (j > 0 && !(DIR && (fl & RF_CUT_S)))
+                    qv -= wy * (half * (kcur + sk[buf][ty][tx + 1])) * sp[buf][ty][tx + 1];
+                if (i > 0 && !(DIR && (fl & RF_CUT_W)))
+                    qv -= wx * (half * (kcur + sk[buf][ty + 1][tx])) * sp[buf][ty + 1][tx];
+                if (i < g.nx - 1 && !(DIR && (fl & RF_CUT_E)))
+                    qv -= wx * (half * (kcur + sk[buf][ty + 1][tx + 2])) * sp[buf][ty + 1][tx + 2];
+                if (j < g.ny - 1 && !(DIR && (fl & RF_CUT_N)))
+                    qv -= wy * (half * (kcur + sk[buf][ty + 2][tx + 1])) * sp[buf][ty + 2][tx + 1];
+                if (k < g.nz - 1 && !(DIR && (fl & RF_CUT_U))) qv -= wz * (half * (kcur + kn)) * pn;
+            }
+            pnew[c] = pc;
+            q[c] = qv;
+            acc += (double)pc * (double)qv;
+        }
+        pm = pc;
+        km = kcur;
+        pc = pn;
+        kcur = kn;
+        dc = dn;
+    }
+    double v[1] = {acc};
+    block_partials<1, false>(v, sc.part_out);
 }
 
 template <typename T, bool DIR>
@@ -702,6 +840,11 @@ void launch_row_flags(hipStream_t s, const Geo& g, const int* const bctype[6], u
     hipLaunchKernelGGL(k_row_flags, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, bt, flags);
 }
 
+void launch_group_scatter(hipStream_t s, int nxy, const int* procmap, const double* groups, double* out)
+{
+    hipLaunchKernelGGL(k_group_scatter, dim3(grid_for(nxy)), dim3(kBlock), 0, s, nxy, procmap, groups, out);
+}
+
 void launch_surface_pre(hipStream_t s, const SurfArgs& a)
 {
     hipLaunchKernelGGL(k_surface_pre, dim3(grid_for(a.nxy)), dim3(kBlock), 0, s, a);
@@ -780,6 +923,60 @@ void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float
                            const float* p, const float* q, const SolveCtl& sc)
 {
     pcg_update<float>(s, n, x, r, D, p, q, sc);
+}
+
+bool tile_supported(const TileCfg& t)
+{
+    return t.kc >= 1 && ((t.tx == 32 && t.ty == 8) || (t.tx == 64 && t.ty == 4) || (t.tx == 16 && t.ty == 16) ||
+                         (t.tx == 32 && t.ty == 4) || (t.tx == 64 && t.ty == 8));
+}
+
+int tile_blocks(const Geo& g, const TileCfg& t)
+{
+    const int ntx = (g.nx + t.tx - 1) / t.tx, nty = (g.ny + t.ty - 1) / t.ty;
+    const int nkz = (g.nz + t.kc - 1) / t.kc;
+    return ntx * nty * nkz;
+}
+
+template <typename T, int TX, int TY>
+static void tiled_launch(hipStream_t s, const Geo& g, int kc, const uint8_t* flags, const T* K, const T* D,
+                         const T* r, const T* pold, T* pnew, T* q, const SolveCtl& sc)
+{
+    const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, nkz = (g.nz + kc - 1) / kc;
+    const dim3 gr(ntx * nty * nkz), bl(TX, TY);
+    if (flags)
+        hipLaunchKernelGGL((k_pcg3_spmv_tiled<T, true, TX, TY>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q,
+                           sc, kc, ntx, nty, nkz);
+    else
+        hipLaunchKernelGGL((k_pcg3_spmv_tiled<T, false, TX, TY>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q,
+                           sc, kc, ntx, nty, nkz);
+}
+
+template <typename T>
+static void tiled_dispatch(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags, const T* K,
+                           const T* D, const T* r, T* const p[2], T* q, const SolveCtl& sc)
+{
+    const T* pold = p[sc.it & 1];
+    T* pnew = p[(sc.it + 1) & 1];
+    if (t.tx == 32 && t.ty == 8) tiled_launch<T, 32, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
+    else if (t.tx == 64 && t.ty == 4) tiled_launch<T, 64, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
+    else if (t.tx == 16 && t.ty == 16) tiled_launch<T, 16, 16>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
+    else if (t.tx == 32 && t.ty == 4) tiled_launch<T, 32, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
+    else tiled_launch<T, 64, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
+}
+
+void launch_pcg3_spmv_tiled_f64(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
+                                const double* K, const double* D, const double* r, double* const p[2],
+                                double* q, const SolveCtl& sc)
+{
+    tiled_dispatch<double>(s, g, t, flags, K, D, r, p, q, sc);
+}
+
+void launch_pcg3_spmv_tiled_f32(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
+                                const float* K, const float* D, const float* r, float* const p[2],
+                                float* q, const SolveCtl& sc)
+{
+    tiled_dispatch<float>(s, g, t, flags, K, D, r, p, q, sc);
 }
 
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,

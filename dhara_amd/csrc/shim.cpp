@@ -1,0 +1,280 @@
+// shim.cpp -- reference-named C++ entry points (include/dhara_shim.h) over the C-ABI.
+//
+// Holds one process-wide solver context, as the reference drove one device from rank 0.
+// All arithmetic happens in the C-ABI library; this file only converts units and moves the
+// host arrays of the reference's flow-model classes in and out.
+#define DHARA_SHIM_NO_MPI
+#include "../../include/dhara_shim.h"
+
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+dhara_ctx* g_ctx = nullptr;
+dhara_grid g_grid{};
+dhara_params g_params{};
+dhara_step_stats g_last{};
+double g_dt_s = 1800.0;
+double g_dt_h = 0.5;
+int32_t g_precision = DHARA_FP64;
+double g_lin_tol = 1e-12;
+int32_t g_carry_reference = 0;
+
+void check(int rc, const char* what)
+{
+    if (rc != DHARA_OK) {
+        std::string msg = std::string(what) + ": " + (g_ctx ? dhara_last_error(g_ctx) : "no context");
+        throw dhara_shim_error(rc, msg);
+    }
+}
+
+void need_ctx()
+{
+    if (!g_ctx) throw dhara_shim_error(DHARA_E_STATE, "CopyDataToDevice has not been called");
+}
+
+void up(int field, const double* host, size_t n)
+{
+    if (host) check(dhara_upload_field(g_ctx, field, host, n), "upload");
+}
+
+void up_i(int field, const int* host, size_t n)
+{
+    if (host) check(dhara_upload_field_i32(g_ctx, field, reinterpret_cast<const int32_t*>(host), n), "upload");
+}
+
+void up_scaled(int field, const double* host, size_t n, double scale)
+{
+    if (!host) return;
+    std::vector<double> v(host, host + n);
+    for (double& x : v) x *= scale;
+    check(dhara_upload_field(g_ctx, field, v.data(), n), "upload");
+}
+
+void down(int field, double* host, size_t n)
+{
+    if (host) check(dhara_download_field(g_ctx, field, host, n), "download");
+}
+
+}  // namespace
+
+dhara_ctx* dhara_shim_context() { return g_ctx; }
+const dhara_step_stats* dhara_shim_last_stats() { return &g_last; }
+
+void dhara_shim_set_options(int32_t precision, double lin_tol, int32_t theta_carry_reference)
+{
+    g_precision = precision;
+    g_lin_tol = lin_tol;
+    g_carry_reference = theta_carry_reference;
+}
+
+void CopyDataToDevice(ProjectClass* project, TimeForcingClass* /*timeforcings*/,
+                      OverlandFlowClass* overland_host, OverlandFlowClass* /*overland_dev*/,
+                      SubsurfaceFlowClass* ssf, SubsurfaceFlowClass* /*subsurface_dev*/, int3 globsize)
+{
+    if (g_ctx) {
+        dhara_destroy(g_ctx);
+        g_ctx = nullptr;
+    }
+    g_dt_h = project->dtimehr;
+    g_dt_s = project->dtimehr * 3600.0;
+    const size_t nx = globsize.x, ny = globsize.y, nz = globsize.z;
+    const size_t N = nx * ny * nz, S = nx * ny, SYZ = ny * nz, SXZ = nx * nz;
+    int procs = 1;
+    if (ssf->procmap)
+        for (size_t i = 0; i < S; ++i) procs = std::max(procs, ssf->procmap[i] + 1);
+    g_grid.nx = globsize.x;
+    g_grid.ny = globsize.y;
+    g_grid.nz = globsize.z;
+    g_grid.nroot_groups = procs;
+    g_grid.dx = project->dx_meter;
+    g_grid.dy = project->dy_meter;
+    g_grid.dz = project->dz_meter;
+    dhara_params_defaults(&g_params);
+    dhara_params& p = g_params;
+    p.dt = g_dt_s;
+    p.theta_s = ssf->theta_s;
+    p.theta_r = ssf->theta_r;
+    p.alpha = ssf->alpha * 100.0;          // 1/cm -> 1/m (the kernels used psi*100)
+    p.n_vg = ssf->poresize;
+    p.Ss = ssf->Ss;
+    p.porosity = ssf->porosity;
+    p.picard_maxiter = ssf->maxiter;
+    p.picard_rule = ssf->picardmethod == 0 ? DHARA_PICARD_REL_PSI : DHARA_PICARD_REL_THETA;
+    p.picard_tol = ssf->picardmethod == 0 ? ssf->tolerance_psi : ssf->tolerance_theta;
+    p.am = 0.0;                            // the reference kernels used am = 0 (include/devconst.h:21)
+    p.precision = g_precision;
+    p.lin_tol = g_lin_tol;
+    p.hmin = overland_host->hmin;
+    p.delta = overland_host->delta;
+    p.K0 = overland_host->K0;
+    p.theta_carry = g_carry_reference ? DHARA_THETA_CARRY_REFERENCE : DHARA_THETA_CARRY_CONSISTENT;
+    p.dirichlet_face_quirk = 1;
+    p.top_bc = DHARA_TOP_SURFACE;
+    check(dhara_create(&g_grid, &g_params, &g_ctx), "dhara_create");
+    const double per_step = 1.0 / g_dt_s;  // m per step -> m/s
+    up(DHARA_F_MANNING, overland_host->mann, S);
+    up(DHARA_F_ZTOPO, overland_host->ztopo, S);
+    up(DHARA_F_PONDING, overland_host->ph ? overland_host->ph : overland_host->waterdepth, S);
+    up_i(DHARA_F_PROCMAP, ssf->procmap, S);
+    up_i(DHARA_F_BCTYPE_B, ssf->bcb, S);
+    up_i(DHARA_F_BCTYPE_T, ssf->bct, S);
+    up_i(DHARA_F_BCTYPE_E, ssf->bce, SYZ);
+    up_i(DHARA_F_BCTYPE_W, ssf->bcw, SYZ);
+    up_i(DHARA_F_BCTYPE_N, ssf->bcn, SXZ);
+    up_i(DHARA_F_BCTYPE_S, ssf->bcs, SXZ);
+    up(DHARA_F_BCPSI_B, ssf->bcpsib, S);
+    up(DHARA_F_BCPSI_T, ssf->bcpsit, S);
+    up(DHARA_F_BCPSI_E, ssf->bcpsie, SYZ);
+    up(DHARA_F_BCPSI_W, ssf->bcpsiw, SYZ);
+    up(DHARA_F_BCPSI_N, ssf->bcpsin, SXZ);
+    up(DHARA_F_BCPSI_S, ssf->bcpsis, SXZ);
+    up_scaled(DHARA_F_BCQ_B, ssf->bcqb, S, per_step);
+    up_scaled(DHARA_F_BCQ_T, ssf->bcqt, S, per_step);
+    up_scaled(DHARA_F_BCQ_E, ssf->bcqe, SYZ, per_step);
+    up_scaled(DHARA_F_BCQ_W, ssf->bcqw, SYZ, per_step);
+    up_scaled(DHARA_F_BCQ_N, ssf->bcqn, SXZ, per_step);
+    up_scaled(DHARA_F_BCQ_S, ssf->bcqs, SXZ, per_step);
+    up_scaled(DHARA_F_KSAT, ssf->ksat, N, per_step);
+    up(DHARA_F_THETA, ssf->thetan, N);
+    check(dhara_init_state(g_ctx, DHARA_INIT_FROM_THETA), "init_state");
+}
+
+void PreRunningFlowModel(ProjectClass* project, SubsurfaceFlowClass*& /*subsurface_dev*/, int rank,
+                         int /*procsize*/, int3 /*globsize*/, MPI_Comm* /*cartComm*/)
+{
+    need_ctx();
+    if (rank == 0 && project->folderoutput) {
+        struct stat st;
+        if (stat(project->folderoutput, &st) == -1) mkdir(project->folderoutput, 0700);
+    }
+}
+
+void GatherRootDensityToDevice(SubsurfaceFlowClass* ssf, int procsize, int3 globsize)
+{
+    need_ctx();
+    const size_t n = (size_t)g_grid.nroot_groups * globsize.z;
+    if ((int)procsize < g_grid.nroot_groups) throw dhara_shim_error(DHARA_E_ARG, "procsize below the rank map");
+    check(dhara_upload_field(g_ctx, DHARA_F_ROOT, ssf->rda, n), "upload rda");
+}
+
+void SendFluxDataToGridsFromHost(SubsurfaceFlowClass* ssf, ProjectClass* /*project*/, int /*procsize*/)
+{
+    need_ctx();
+    const int G = g_grid.nroot_groups;
+    std::vector<double> v(G);
+    // ppt_ground [mm per step] -> rain [m/s]; hpoten = ph + ppt/1000 (src/global.cu:181)
+    if (ssf->ppt_root) {
+        for (int g = 0; g < G; ++g) v[g] = ssf->ppt_root[g] / 1000.0 / g_dt_s;
+        check(dhara_scatter_group_field(g_ctx, DHARA_F_RAIN, v.data(), G), "scatter ppt");
+    }
+    // E_soil: hpoten -= et * sec_p_mm2dt_p_m * dt with dt in hours (src/global.cu:181)
+    if (ssf->E_soil_root) {
+        for (int g = 0; g < G; ++g) v[g] = ssf->E_soil_root[g] * (g_dt_s / 1000.0) * g_dt_h / g_dt_s;
+        check(dhara_scatter_group_field(g_ctx, DHARA_F_EVAP, v.data(), G), "scatter E_soil");
+    }
+    // TR [mm/s]: sink per step TR*sec_p_mm2dt_p_m*root/dz (src/subsurface.cu:115) -> m/s
+    if (ssf->TR_root) {
+        for (int g = 0; g < G; ++g) v[g] = ssf->TR_root[g] / 1000.0;
+        check(dhara_scatter_group_field(g_ctx, DHARA_F_TRANSP, v.data(), G), "scatter TR");
+    }
+}
+
+void SubsurfaceFlowModel(TimeForcingClass*& /*timeforcings*/, OverlandFlowClass*& overland_host,
+                         OverlandFlowClass*& /*overland_dev*/, SubsurfaceFlowClass*& ssf,
+                         SubsurfaceFlowClass*& /*subsurface_dev*/, int /*rank*/, int /*procsize*/,
+                         int3 globsize, int t, int /*num_steps*/)
+{
+    need_ctx();
+    const size_t N = (size_t)globsize.x * globsize.y * globsize.z, S = (size_t)globsize.x * globsize.y;
+    dhara_step_stats st;
+    check(dhara_subsurface_step(g_ctx, &st), "subsurface step");
+    g_last = st;
+    if (ssf->mb_subsurfaceW) ssf->mb_subsurfaceW[t] = st.mb_ref;
+    down(DHARA_F_PSI, ssf->psiout, N);      // src/subsurface.cu:842-846
+    down(DHARA_F_THETA, ssf->thetaout, N);
+    down(DHARA_F_QTOP, ssf->qss, S);
+    if (overland_host) {
+        down(DHARA_F_HPOTEN, overland_host->hpoten, S);
+        down(DHARA_F_QCAPA, overland_host->qcapa, S);
+    }
+}
+
+void OverlandFlowModel(TimeForcingClass*& /*timeforcings*/, OverlandFlowClass*& /*overland_dev*/,
+                       SubsurfaceFlowClass*& /*subsurface_dev*/, int /*rank*/, int /*procsize*/,
+                       int3 /*globsize*/, int /*t*/, int /*num_steps*/)
+{
+    need_ctx();
+    dhara_step_stats st;
+    std::memset(&st, 0, sizeof(st));
+    check(dhara_overland_step(g_ctx, &st), "overland step");
+    g_last.olf_iters = st.olf_iters;
+    g_last.olf_converged = st.olf_converged;
+    g_last.outlet_discharge = st.outlet_discharge;
+    g_last.ring_outflow = st.ring_outflow;
+}
+
+void CopyLayers(SubsurfaceFlowClass*& ssf, int t, int sizexy, int sizez)
+{
+    need_ctx();
+    const size_t N = (size_t)sizexy * sizez;
+    std::vector<double> psi(N), th(N);
+    down(DHARA_F_PSI, psi.data(), N);
+    down(DHARA_F_THETA, th.data(), N);
+    for (int k = 0; k < sizez; ++k) {
+        double sp = 0.0, stt = 0.0;
+        for (int c = k * sizexy; c < (k + 1) * sizexy - 1; ++c) {   // reference range, last cell excluded
+            sp += psi[c];
+            stt += th[c];
+        }
+        if (ssf->psi_col) ssf->psi_col[t * sizez + k] = sp;
+        if (ssf->theta_col) ssf->theta_col[t * sizez + k] = stt;
+    }
+}
+
+void SaveModelResults(ProjectClass* /*project*/, OverlandFlowClass* overland_host,
+                      OverlandFlowClass* /*overland_dev*/, SubsurfaceFlowClass* ssf,
+                      SubsurfaceFlowClass* /*subsurface_dev*/, int3 globsize, int /*t*/)
+{
+    need_ctx();
+    const size_t N = (size_t)globsize.x * globsize.y * globsize.z, S = (size_t)globsize.x * globsize.y;
+    if (overland_host) {
+        down(DHARA_F_PONDING, overland_host->waterdepth, S);
+        down(DHARA_F_WATERELEV, overland_host->waterelev, S);
+    }
+    down(DHARA_F_PSI, ssf->psiout, N);
+    down(DHARA_F_THETA, ssf->thetaout, N);
+}
+
+void RunCoupledFlowModelHIP(ProjectClass* project, TimeForcingClass* timeforcings,
+                            OverlandFlowClass* overland_host, OverlandFlowClass* overland_dev,
+                            SubsurfaceFlowClass* subsurface_host, SubsurfaceFlowClass* subsurface_dev,
+                            int rank, int procsize, int3 globsize,
+                            void (*canopy_step)(int t, void* user), void* user)
+{
+    need_ctx();
+    const int num_steps = project->num_steps;
+    const int sizexy = globsize.x * globsize.y;
+    for (int t = 0; t < num_steps; ++t) {
+        if (canopy_step) canopy_step(t, user);
+        SendFluxDataToGridsFromHost(subsurface_host, project, procsize);
+        SubsurfaceFlowModel(timeforcings, overland_host, overland_dev, subsurface_host, subsurface_dev, rank,
+                            procsize, globsize, t, num_steps);
+        OverlandFlowModel(timeforcings, overland_dev, subsurface_dev, rank, procsize, globsize, t, num_steps);
+        if (project->savestat) CopyLayers(subsurface_host, t, sizexy, globsize.z);
+    }
+}
+
+void FreeDeviceMemory(OverlandFlowClass* /*overland_dev*/, SubsurfaceFlowClass* /*subsurface_dev*/,
+                      int /*rank*/, int /*procsize*/)
+{
+    if (g_ctx) {
+        dhara_destroy(g_ctx);
+        g_ctx = nullptr;
+    }
+}

@@ -183,6 +183,12 @@ def load_hip_library(path=HIP_LIB_PATH):
         lib.dhara_profile_enable.restype = C.c_int
         lib.dhara_profile_read.argtypes = [C.c_void_p, P(_abi.KernelProfile)]
         lib.dhara_profile_read.restype = C.c_int
+        lib.dhara_set_tile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        lib.dhara_set_tile.restype = C.c_int
+        lib.dhara_get_tile.argtypes = [C.c_void_p, C.c_int32] + [P(C.c_int32)] * 3
+        lib.dhara_get_tile.restype = C.c_int
+        lib.dhara_autotune.argtypes = [C.c_void_p, P(C.c_double)]
+        lib.dhara_autotune.restype = C.c_int
         lib.dhara_set_precision.argtypes = [C.c_void_p, C.c_int32]
         lib.dhara_set_precision.restype = C.c_int
         if lib.dhara_abi_version() != _abi.ABI_VERSION:
@@ -231,6 +237,21 @@ class FlowModel(BaseModel):
         self._check(self._lib.dhara_overland_conductance(self._h, _dptr(ke), _dptr(kn)),
                     "overland_conductance")
         return ke, kn
+
+    def set_tile(self, elem_bytes, tx, ty=0, kc=0):
+        """Tile shape of the subsurface SpMV for FP64 (8) or FP32 (4) solves; tx = 0: untiled."""
+        self._check(self._lib.dhara_set_tile(self._h, elem_bytes, tx, ty, kc), "set_tile")
+
+    def get_tile(self, elem_bytes):
+        v = [C.c_int32() for _ in range(3)]
+        self._check(self._lib.dhara_get_tile(self._h, elem_bytes, *[C.byref(x) for x in v]), "get_tile")
+        return tuple(x.value for x in v)
+
+    def autotune(self):
+        """Pick the fastest SpMV tile shape on the current operator; returns us/launch [fp64, fp32]."""
+        out = (C.c_double * 2)()
+        self._check(self._lib.dhara_autotune(self._h, out), "autotune")
+        return list(out)
 
     def profile(self, on=True):
         """HIP-event timing of the subsurface Krylov kernels on the context's stream."""

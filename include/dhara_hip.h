@@ -129,8 +129,10 @@ typedef struct dhara_params {
     /* subsurface linear solve                                                          */
     int32_t lin_maxiter, precision;
     double  lin_tol;                              /* ||r||_2 <= lin_tol * ||b||_2         */
-    int32_t refine_maxiter, inner_maxiter;        /* DHARA_MIXED only                     */
-    double  inner_tol;                            /* FP32 inner relative tolerance        */
+    int32_t refine_maxiter, inner_maxiter;        /* DHARA_MIXED: max FP64 reliable updates
+                                                     per solve, max FP32 iterations        */
+    double  inner_tol;                            /* DHARA_MIXED: FP32 residual reduction
+                                                     between two FP64 reliable updates     */
     /* overland flow (src/overland.cu:387-419, include/devconst.h:25-28)                 */
     int32_t overland, olf_maxiter;
     double  olf_tol;
@@ -187,7 +189,7 @@ static inline void dhara_params_defaults(dhara_params* p)
     p->picard_maxiter = 20; p->picard_rule = DHARA_PICARD_ABS_PSI;
     p->picard_tol = 1e-6; p->am = 0.0;
     p->lin_maxiter = 2000; p->precision = DHARA_FP64; p->lin_tol = 1e-12;
-    p->refine_maxiter = 8; p->inner_maxiter = 2000; p->inner_tol = 1e-5;
+    p->refine_maxiter = 16; p->inner_maxiter = 2000; p->inner_tol = 1e-3;
     p->overland = 1; p->olf_maxiter = 2000; p->olf_tol = 1e-12;
     p->hmin = 1e-5; p->delta = 1e-7; p->K0 = 1e-5;
     p->theta_carry = DHARA_THETA_CARRY_CONSISTENT;
@@ -224,6 +226,12 @@ int dhara_download_field_i32(dhara_ctx* ctx, int32_t field, int32_t* host, size_
 int dhara_upload_field_device(dhara_ctx* ctx, int32_t field, const void* dev, size_t n);
 int dhara_download_field_device(dhara_ctx* ctx, int32_t field, void* dev, size_t n);
 
+/* Per-group forcing (reference: one value per MPI rank gathered to rank 0): field[cell] =
+ * values[procmap[cell]] for a surface field (DHARA_F_RAIN, _EVAP, _TRANSP).  Replaces the
+ * H2D copy + SendFluxDataToGrids of GatherFluxesDomain (src/subsurface.cu:643-659,
+ * 364-378); `values` has nroot_groups entries (host memory). */
+int dhara_scatter_group_field(dhara_ctx* ctx, int32_t field, const double* values, size_t ngroups);
+
 /* psi <-> theta initialisation: CopyDataToDevice's vanGenuchtenInverse
  * (src/device.cu:323-330) followed by PreRunningFlowModel (src/global.cu:270-296). */
 int dhara_init_state(dhara_ctx* ctx, int32_t mode);
@@ -244,6 +252,19 @@ int dhara_step(dhara_ctx* ctx, int32_t nsteps, dhara_step_stats* stats);
 int dhara_profile_enable(dhara_ctx* ctx, int32_t on);
 /* Accumulated profile since the last read; resets the accumulators. */
 int dhara_profile_read(dhara_ctx* ctx, dhara_kernel_profile* out);
+
+/* ---- tuning ------------------------------------------------------------------------------ */
+/* Tile shape of the LDS-tiled subsurface SpMV kernel for one precision (elem_bytes 8: FP64
+ * solves, 4: FP32 inner solves).  tx = 0 selects the untiled one-cell-per-thread kernel.
+ * Supported (tx, ty): (32,8) (64,4) (16,16) (32,4) (64,8); kc >= 1 planes per block. */
+int dhara_set_tile(dhara_ctx* ctx, int32_t elem_bytes, int32_t tx, int32_t ty, int32_t kc);
+int dhara_get_tile(const dhara_ctx* ctx, int32_t elem_bytes, int32_t* tx, int32_t* ty, int32_t* kc);
+/* Times the candidate tile shapes on the context's current operator (HIP events, a few
+ * launches each) for the precisions the context uses, keeps the fastest, and records it in a
+ * process-wide table keyed by (nx, ny, nz, elem_bytes) that later contexts of the same shape
+ * reuse without timing.  best_us (may be NULL) receives the winner's time per launch for
+ * [FP64, FP32]. */
+int dhara_autotune(dhara_ctx* ctx, double* best_us);
 
 /* ---- kernel-level entry points used by the parity tests ------------------------------ */
 /* theta, K, C of the context's soil at the given psi (vanGenuchten, src/global.cu:66-105).

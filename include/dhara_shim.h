@@ -1,0 +1,133 @@
+/*
+ * dhara_shim.h -- reference-named C++ entry points over the dhara C-ABI.
+ *
+ * A host program shaped like the reference's src/numerical.cc + src/host.cc (which fill the
+ * flow-model classes below and call these functions) links against libdhara_hip.so instead of
+ * the CUDA/CUSP objects.  The classes repeat the member lists of the reference's
+ * include/class.h (ProjectClass 3-34, TimeForcingClass 74-94, OverlandFlowClass 97-130,
+ * SubsurfaceFlowClass 133-218) member for member, so existing host code that fills them
+ * compiles unchanged; only the host arrays are used -- the `*_dev` objects the reference
+ * filled with cudaMalloc'ed pointers are accepted and ignored (the library owns device memory).
+ *
+ * Units follow the reference (include/devconst.h:6-21, src/host.cc:53, src/global.cu:79-96,
+ * 181): alpha in 1/cm, K and fluxes in m per time step, dt in hours (ProjectClass::dtimehr),
+ * ppt_ground in mm per step, E_soil and TR in mm/s scaled by sec_p_mm2dt_p_m = 1.8.  The shim
+ * converts to SI at the boundary.  Grid spacing and soil constants are taken from the
+ * ProjectClass / SubsurfaceFlowClass values (the reference kernels used compile-time
+ * constants instead, src/device.cu:174-206).
+ *
+ * The CUSP/Thrust container arguments of SubsurfaceFlowModel / OverlandFlowModel
+ * (include/subsurface.h:30-38, include/overland.h:8-11) are dropped; MPI_Comm* parameters stay
+ * (define DHARA_SHIM_NO_MPI to get a placeholder MPI_Comm type when building without MPI).
+ * Errors throw dhara_shim_error (the reference called exit()).
+ */
+#ifndef DHARA_SHIM_H
+#define DHARA_SHIM_H
+
+#include <hip/hip_vector_types.h>   /* int2 / int3, as the reference's CUDA vector types */
+
+#include <stdexcept>
+#include <string>
+
+#include "dhara_hip.h"
+
+#ifdef DHARA_SHIM_NO_MPI
+typedef int MPI_Comm;
+#else
+#include <mpi.h>
+#endif
+
+class ProjectClass {
+public:
+    const char *name, *folderoutput, *mlcanoutput, *olfoutput, *ssfoutput, *ssf1doutput, *statoutput;
+    int co2elevation, numberofplants, planttype, savemlcan, saveolf, savessf, savestat;
+    int saveinterval, printiterval, num_steps, verbose, version;
+    double co2concentration, dtimehr, dx_meter, dy_meter, dz_meter;
+};
+
+class TimeForcingClass {
+public:
+    int *doy, *years;
+    double *rg, *pa, *lwdn, *zen, *u, *ppt, *ta, *ea, *lai, *vpd, *decdoy, *hour;
+};
+
+class OverlandFlowClass {
+public:
+    double *a2d, *hpoten, *ke, *kn, *ks, *kw, *mann, *ph, *rhs2d, *qcapa, *u, *v;
+    double *waterdepth, *waterelev, *ztopo;
+    double delta, dx, dy, dt, hmin, hcri, K0, hn, hs, hw, he;
+};
+
+class SubsurfaceFlowClass {
+public:
+    int *bcb, *bce, *bcn, *bcs, *bct, *bcw, *type, *procmap;
+    double *bcpsib, *bcpsie, *bcpsin, *bcpsis, *bcpsit, *bcpsiw;
+    double *bcqb, *bcqe, *bcqn, *bcqs, *bcqt, *bcqw;
+    double *a3d, *deltam, *cnp1m, *knp1m, *ksat, *psin, *psinp1m, *psinp1mp1, *psiout, *qss;
+    double *rhs3d, *thetan, *thetanp1m, *thetanp1mp1, *thetaout;
+    double *quflux, *qdflux, *qwflux, *qeflux, *qsflux, *qnflux, *dtheta, *transp, *evapo, *ssflux;
+    double *mb_subsurfaceW;
+    double *TR, *TR_root, *ppt_ground, *ppt_root, *E_soil, *E_soil_root, *rda, *psi_col, *theta_col;
+    int maxiter, picardmethod;
+    double alpha, am, airdry, dx, dy, dz, dt, poresize, porosity, psimin, Ss, theta_s, theta_r;
+    double tolerance_psi, tolerance_theta;
+};
+
+class dhara_shim_error : public std::runtime_error {
+public:
+    dhara_shim_error(int code, const std::string& what) : std::runtime_error(what), code(code) {}
+    int code;
+};
+
+/* The context the shims drive (created by CopyDataToDevice, destroyed by FreeDeviceMemory). */
+dhara_ctx* dhara_shim_context();
+/* Last step's statistics (Picard/Krylov counts, mass balance, discharge). */
+const dhara_step_stats* dhara_shim_last_stats();
+/* Solver options the reference fixed at compile time; call before CopyDataToDevice. */
+void dhara_shim_set_options(int32_t precision, double lin_tol, int32_t theta_carry_reference);
+
+/* src/device.cu:223-337 (+ the device half of src/numerical.cc:127-188): uploads the host
+ * classes, theta -> psi (vanGenuchtenInverse), psi -> theta/K/C. */
+void CopyDataToDevice(ProjectClass* project, TimeForcingClass* timeforcings,
+                      OverlandFlowClass* overland_host, OverlandFlowClass* overland_dev,
+                      SubsurfaceFlowClass* subsurface_host, SubsurfaceFlowClass* subsurface_dev,
+                      int3 globsize);
+/* src/global.cu:270-296: output folder + initial vG (already applied by CopyDataToDevice). */
+void PreRunningFlowModel(ProjectClass* project, SubsurfaceFlowClass*& subsurface_dev, int rank,
+                         int procsize, int3 globsize, MPI_Comm* cartComm);
+/* H2D half of src/host.cc:241-259: the gathered root fractions rda[procsize*nz]. */
+void GatherRootDensityToDevice(SubsurfaceFlowClass* subsurface_host, int procsize, int3 globsize);
+/* H2D + scatter half of GatherFluxesDomain (src/subsurface.cu:632-662): TR_root, ppt_root and
+ * E_soil_root (one value per rank, already gathered) onto the surface cells via procmap. */
+void SendFluxDataToGridsFromHost(SubsurfaceFlowClass* subsurface_host, ProjectClass* project,
+                                 int procsize);
+/* src/subsurface.cu:686-847: one subsurface step; fills mb_subsurfaceW[t], psiout, thetaout. */
+void SubsurfaceFlowModel(TimeForcingClass*& timeforcings, OverlandFlowClass*& overland_host,
+                         OverlandFlowClass*& overland_dev, SubsurfaceFlowClass*& subsurface_host,
+                         SubsurfaceFlowClass*& subsurface_dev, int rank, int procsize, int3 globsize,
+                         int t, int num_steps);
+/* src/overland.cu:387-419: one overland step. */
+void OverlandFlowModel(TimeForcingClass*& timeforcings, OverlandFlowClass*& overland_dev,
+                       SubsurfaceFlowClass*& subsurface_dev, int rank, int procsize, int3 globsize,
+                       int t, int num_steps);
+/* D2H half of src/device.cu:135-147 (CopyLayers): per-layer sums of psi and theta into
+ * psi_col / theta_col[t*nz + k], over cells k*nx*ny .. (k+1)*nx*ny - 2 like the reference. */
+void CopyLayers(SubsurfaceFlowClass*& subsurface_host, int t, int sizexy, int sizez);
+/* D2H half of SaveModelResults (src/io.cc:866-901): waterdepth, waterelev, psiout, thetaout
+ * into the host classes; writing NetCDF stays with the caller. */
+void SaveModelResults(ProjectClass* project, OverlandFlowClass* overland_host,
+                      OverlandFlowClass* overland_dev, SubsurfaceFlowClass* subsurface_host,
+                      SubsurfaceFlowClass* subsurface_dev, int3 globsize, int t);
+/* Time loop of RunCoupledFlowModel (src/device.cu:426-469) without the canopy model: each
+ * step calls canopy_step(t, user) (which fills TR_root / ppt_root / E_soil_root on the host),
+ * scatters them, runs the subsurface and overland steps, CopyLayers if project->savestat. */
+void RunCoupledFlowModelHIP(ProjectClass* project, TimeForcingClass* timeforcings,
+                            OverlandFlowClass* overland_host, OverlandFlowClass* overland_dev,
+                            SubsurfaceFlowClass* subsurface_host, SubsurfaceFlowClass* subsurface_dev,
+                            int rank, int procsize, int3 globsize,
+                            void (*canopy_step)(int t, void* user), void* user);
+/* src/numerical.cc:279-336 (device half). */
+void FreeDeviceMemory(OverlandFlowClass* overland_dev, SubsurfaceFlowClass* subsurface_dev, int rank,
+                      int procsize);
+
+#endif

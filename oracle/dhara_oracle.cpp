@@ -905,6 +905,18 @@ int dor_download_field_i32(dor_ctx* c, int32_t f, int32_t* host, size_t n)
     return DHARA_OK;
 }
 
+// SendFluxDataToGrids, src/subsurface.cu:364-378: data[cell] = dataroot[procmap[cell]].
+int dor_scatter_group_field(dor_ctx* c, int32_t f, const double* values, size_t ngroups)
+{
+    if (!c || !values) return DHARA_E_ARG;
+    if (f != DHARA_F_RAIN && f != DHARA_F_EVAP && f != DHARA_F_TRANSP) return fail(c, DHARA_E_ARG, "bad field");
+    if (ngroups != (size_t)c->g.nroot_groups) return fail(c, DHARA_E_ARG, "need nroot_groups values");
+    size_t n = 0;
+    std::vector<double>* v = dfield(*c, f, &n);
+    for (size_t t = 0; t < n; ++t) (*v)[t] = values[c->procmap[t]];
+    return DHARA_OK;
+}
+
 int dor_init_state(dor_ctx* c, int32_t mode)
 {
     if (!c) return DHARA_E_ARG;

@@ -43,6 +43,7 @@ int dor_upload_field(dor_ctx* ctx, int32_t field, const double* host, size_t n);
 int dor_upload_field_i32(dor_ctx* ctx, int32_t field, const int32_t* host, size_t n);
 int dor_download_field(dor_ctx* ctx, int32_t field, double* host, size_t n);
 int dor_download_field_i32(dor_ctx* ctx, int32_t field, int32_t* host, size_t n);
+int dor_scatter_group_field(dor_ctx* ctx, int32_t field, const double* values, size_t ngroups);
 int dor_init_state(dor_ctx* ctx, int32_t mode);
 int dor_subsurface_step(dor_ctx* ctx, dhara_step_stats* stats);
 int dor_overland_step(dor_ctx* ctx, dhara_step_stats* stats);

@@ -1,0 +1,160 @@
+// A host program shaped like the reference's src/numerical.cc + src/host.cc, linked against
+// libdhara_hip.so through the reference-named shims (include/dhara_shim.h).  It fills the
+// flow-model classes the way SetInitialConditionsHost / SetBoundaryConditionsHost /
+// SetMPIGPUMapping do (src/host.cc:32-226), runs the coupled time loop with a constant
+// per-rank canopy forcing, and writes the final state as raw doubles:
+//   psi[N] theta[N] waterdepth[S] mb[num_steps] psi_col[num_steps*nz]
+// usage: shim_driver nx ny nz num_steps out.bin
+#define DHARA_SHIM_NO_MPI
+#include "../../include/dhara_shim.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+struct Forcing {
+    SubsurfaceFlowClass* ssf;
+    int procs;
+    int num_steps;
+};
+
+static void canopy(int t, void* user)
+{
+    Forcing* f = static_cast<Forcing*>(user);
+    for (int r = 0; r < f->procs; ++r) {
+        f->ssf->TR_root[r] = 2e-5 * (1 + r);                      // mm/s
+        f->ssf->ppt_root[r] = (t < f->num_steps / 2) ? 4.0 + r : 0.0;   // mm per step
+        f->ssf->E_soil_root[r] = 1e-5;                             // mm/s
+    }
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 6) {
+        std::fprintf(stderr, "usage: %s nx ny nz num_steps out.bin\n", argv[0]);
+        return 2;
+    }
+    const int nx = std::atoi(argv[1]), ny = std::atoi(argv[2]), nz = std::atoi(argv[3]);
+    const int num_steps = std::atoi(argv[4]);
+    const int3 globsize = make_int3(nx, ny, nz);
+    const int S = nx * ny, N = S * nz;
+    const int tx = 2, ty = 2, procs = tx * ty;
+
+    ProjectClass project{};
+    project.dtimehr = 0.5;
+    project.dx_meter = 1.2;
+    project.dy_meter = 1.2;
+    project.dz_meter = 0.2;
+    project.num_steps = num_steps;
+    project.savestat = 1;
+    project.folderoutput = nullptr;
+    TimeForcingClass tf{};
+    OverlandFlowClass olf_h{}, olf_d{};
+    SubsurfaceFlowClass ssf_h{}, ssf_d{};
+    std::vector<double> mann(S), wd(S), we(S), ph(S), z(S), hp(S), qc(S);
+    olf_h.mann = mann.data();
+    olf_h.waterdepth = wd.data();
+    olf_h.waterelev = we.data();
+    olf_h.ph = ph.data();
+    olf_h.ztopo = z.data();
+    olf_h.hpoten = hp.data();
+    olf_h.qcapa = qc.data();
+    olf_h.delta = 1e-7;
+    olf_h.hmin = 1e-5;
+    olf_h.K0 = 1e-5;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) z[j * nx + i] = 0.02 * i + 0.01 * std::sin(0.7 * j);
+    // SetInitialConditionsHost (src/host.cc:44-62)
+    for (int i = 0; i < S; ++i) {
+        mann[i] = 0.025;
+        wd[i] = 0.0;
+        we[i] = z[i] + wd[i];
+        ph[i] = 0.0;
+    }
+    std::vector<double> ksat(N), thetan(N), psiout(N), thetaout(N), qss(S);
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < S; ++i) {
+            ksat[k * S + i] = 0.0025;
+            thetan[k * S + i] = 0.3 + 0.01 * k;
+        }
+    ssf_h.ksat = ksat.data();
+    ssf_h.thetan = thetan.data();
+    ssf_h.psiout = psiout.data();
+    ssf_h.thetaout = thetaout.data();
+    ssf_h.qss = qss.data();
+    // SetBoundaryConditionsHost (src/host.cc:81-164): every face flux type, zero flux
+    std::vector<int> bxy(S, 1), byz(ny * nz, 1), bxz(nx * nz, 1);
+    std::vector<double> zxy(S, 0.0), zyz(ny * nz, 0.0), zxz(nx * nz, 0.0), pxy(S, -1.0);
+    ssf_h.bct = ssf_h.bcb = bxy.data();
+    ssf_h.bce = ssf_h.bcw = byz.data();
+    ssf_h.bcn = ssf_h.bcs = bxz.data();
+    ssf_h.bcpsit = ssf_h.bcpsib = pxy.data();
+    ssf_h.bcpsie = ssf_h.bcpsiw = zyz.data();
+    ssf_h.bcpsin = ssf_h.bcpsis = zxz.data();
+    ssf_h.bcqt = ssf_h.bcqb = zxy.data();
+    ssf_h.bcqe = ssf_h.bcqw = zyz.data();
+    ssf_h.bcqn = ssf_h.bcqs = zxz.data();
+    // SetMPIGPUMapping (src/host.cc:201-226) for a 2 x 2 rank topology
+    std::vector<int> procmap(S);
+    const int xd[2] = {0, nx / 2}, yd[2] = {0, ny / 2};
+    int proc = 0;
+    for (int jo = 0; jo < ty; ++jo) {
+        const int jmax = jo < ty - 1 ? yd[jo + 1] : ny;
+        for (int io = 0; io < tx; ++io) {
+            const int imax = io < tx - 1 ? xd[io + 1] : nx;
+            for (int j = yd[jo]; j < jmax; ++j)
+                for (int i = xd[io]; i < imax; ++i) procmap[j * nx + i] = proc;
+            ++proc;
+        }
+    }
+    ssf_h.procmap = procmap.data();
+    // flow-model keys of examples/1-EBI/ebi.cfg with a looser Picard tolerance
+    ssf_h.maxiter = 10;
+    ssf_h.picardmethod = 0;
+    ssf_h.tolerance_psi = 1e-6;
+    ssf_h.tolerance_theta = 1e-4;
+    ssf_h.alpha = 0.02;
+    ssf_h.poresize = 1.8;
+    ssf_h.porosity = 0.45;
+    ssf_h.Ss = 5e-4;
+    ssf_h.theta_r = 0.1;
+    ssf_h.theta_s = 0.45;
+    std::vector<double> TR(procs), ppt(procs), Es(procs), rda(procs * nz), mb(num_steps),
+        psicol(num_steps * nz), thetacol(num_steps * nz);
+    ssf_h.TR_root = TR.data();
+    ssf_h.ppt_root = ppt.data();
+    ssf_h.E_soil_root = Es.data();
+    ssf_h.rda = rda.data();
+    ssf_h.mb_subsurfaceW = mb.data();
+    ssf_h.psi_col = psicol.data();
+    ssf_h.theta_col = thetacol.data();
+    for (int r = 0; r < procs; ++r) {      // root fractions decaying with depth, sum 1
+        double s = 0.0;
+        for (int k = 0; k < nz; ++k) s += std::exp(-0.4 * k);
+        for (int k = 0; k < nz; ++k) rda[r * nz + k] = std::exp(-0.4 * k) / s;
+    }
+    try {
+        CopyDataToDevice(&project, &tf, &olf_h, &olf_d, &ssf_h, &ssf_d, globsize);
+        SubsurfaceFlowClass* sd = &ssf_d;
+        PreRunningFlowModel(&project, sd, 0, procs, globsize, nullptr);
+        GatherRootDensityToDevice(&ssf_h, procs, globsize);
+        Forcing f{&ssf_h, procs, num_steps};
+        RunCoupledFlowModelHIP(&project, &tf, &olf_h, &olf_d, &ssf_h, &ssf_d, 0, procs, globsize, canopy, &f);
+        SaveModelResults(&project, &olf_h, &olf_d, &ssf_h, &ssf_d, globsize, num_steps - 1);
+        FreeDeviceMemory(&olf_d, &ssf_d, 0, procs);
+    } catch (const dhara_shim_error& e) {
+        std::fprintf(stderr, "shim error %d: %s\n", e.code, e.what());
+        return 1;
+    }
+    FILE* fo = std::fopen(argv[5], "wb");
+    if (!fo) return 1;
+    std::fwrite(psiout.data(), sizeof(double), N, fo);
+    std::fwrite(thetaout.data(), sizeof(double), N, fo);
+    std::fwrite(wd.data(), sizeof(double), S, fo);
+    std::fwrite(mb.data(), sizeof(double), num_steps, fo);
+    std::fwrite(psicol.data(), sizeof(double), (size_t)num_steps * nz, fo);
+    std::fclose(fo);
+    std::printf("shim_driver: %d steps on %dx%dx%d done\n", num_steps, nx, ny, nz);
+    return 0;
+}

@@ -99,6 +99,7 @@ def test_oracle_library_exports():
         if n in ("dhara_abi_version", "dhara_device_info", "dhara_get_params", "dhara_set_precision",
                  "dhara_upload_field_device", "dhara_download_field_device",
                  "dhara_apply_subsurface_operator_f32", "dhara_profile_enable", "dhara_profile_read",
-                 "dhara_overland_conductance", "dhara_assemble_subsurface"):
+                 "dhara_overland_conductance", "dhara_assemble_subsurface", "dhara_set_tile",
+                 "dhara_get_tile", "dhara_autotune"):
             continue
         assert hasattr(L, "dor_" + n[len("dhara_"):]), n

@@ -167,3 +167,55 @@ def test_dirichlet_faces_and_reference_switches(hip, orc):
                   f" dpsi {dpsi:.3e}", flush=True)
             assert dpsi <= HEAD_TOL
             assert sg["picard_iters"] == so["picard_iters"]
+
+
+@pytest.mark.parametrize("tile", [(32, 8, 7), (64, 4, 30), (16, 16, 4), (32, 4, 11), (64, 8, 5)])
+def test_tiled_spmv_matches_untiled(hip, orc, tile):
+    # odd sizes: nx not a multiple of any tile width, ny not of any height, kc not dividing nz
+    w = inputs.c2_coupled(nx=37, ny=21, nz=30)
+    ref = hip.FlowModel(w.grid, cp(w.params))
+    til = hip.FlowModel(w.grid, cp(w.params))
+    w.apply(ref)
+    w.apply(til)
+    ref.set_tile(8, 0)
+    til.set_tile(8, *tile)
+    for s in range(3):
+        a = w.run(ref, 1, start=s)[0]
+        b = w.run(til, 1, start=s)[0]
+        d = np.max(np.abs(ref.download("PSI") - til.download("PSI")))
+        print(f"tile {tile} step {s}: lin {a['lin_iters']}/{b['lin_iters']} dpsi {d:.2e}", flush=True)
+        assert a["lin_iters"] == b["lin_iters"] and a["picard_iters"] == b["picard_iters"]
+        assert d < 1e-11
+
+
+def test_tiled_mixed_and_dirichlet_against_oracle(hip, orc):
+    w = _hetero_c1(nx=35, ny=19, nz=13, seed=17)
+    nx, ny, nz = 35, 19, 13
+    w.fields["BCTYPE_E"] = np.zeros(ny * nz, np.int32)
+    w.fields["BCPSI_E"] = np.full(ny * nz, -0.4)
+    w.fields["BCTYPE_S"] = np.zeros(nx * nz, np.int32)
+    w.fields["BCPSI_S"] = np.full(nx * nz, -0.7)
+    gm, om = pair(hip, orc, w, precision=_abi.MIXED)
+    gm.set_tile(8, 32, 8, 5)
+    gm.set_tile(4, 16, 16, 6)
+    for s in range(4):
+        sg = gm.step(1)[0]
+        om.step(1)
+        d = np.max(np.abs(gm.download("PSI") - om.download("PSI")))
+        print(f"mixed+dirichlet step {s}: picard {sg['picard_iters']} sweeps {sg['refine_sweeps']} dpsi {d:.2e}",
+              flush=True)
+        assert d < 1e-8
+
+
+def test_autotune_then_parity(hip, orc):
+    w = inputs.c2_coupled(nx=48, ny=40, nz=20, precision=_abi.MIXED)
+    gm, om = pair(hip, orc, w, precision=_abi.MIXED)
+    best = gm.autotune()
+    print("autotune:", best, gm.get_tile(8), gm.get_tile(4), flush=True)
+    assert best[0] > 0 and best[1] > 0
+    w.run(gm, 3)
+    w.run(om, 3)
+    assert np.max(np.abs(gm.download("PSI") - om.download("PSI"))) <= 1e-6
+    # a second context of the same shape picks the tuned tile from the table
+    g2 = hip.FlowModel(w.grid, cp(w.params))
+    assert g2.get_tile(4) == gm.get_tile(4)

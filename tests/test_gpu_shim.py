@@ -1,0 +1,94 @@
+"""The reference-named C++ shims (include/dhara_shim.h): a host program shaped like the
+reference's numerical.cc/host.cc (tests/shim/shim_driver.cpp), in the reference's units, must
+give the same state as the C-ABI driven with the SI inputs converted here independently, and
+as the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dhara_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "shim", "shim_driver")
+
+
+def si_inputs(nx, ny, nz):
+    S, N = nx * ny, nx * ny * nz
+    dt = 0.5 * 3600.0
+    g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=4, dx=1.2, dy=1.2, dz=0.2)
+    p = _abi.default_params(dt=dt, theta_s=0.45, theta_r=0.1, alpha=2.0, n_vg=1.8, Ss=5e-4,
+                            porosity=0.45, picard_maxiter=10, picard_rule=_abi.PICARD_REL_PSI,
+                            picard_tol=1e-6, am=0.0, hmin=1e-5, delta=1e-7, K0=1e-5,
+                            top_bc=_abi.TOP_SURFACE, overland=1)
+    i = np.arange(nx)
+    j = np.arange(ny)
+    z = (0.02 * i[None, :] + 0.01 * np.sin(0.7 * j)[:, None]).ravel()
+    procmap = np.zeros((ny, nx), np.int32)
+    procmap[: ny // 2, nx // 2:] = 1
+    procmap[ny // 2:, : nx // 2] = 2
+    procmap[ny // 2:, nx // 2:] = 3
+    k = np.arange(nz)
+    prof = np.exp(-0.4 * k) / np.exp(-0.4 * k).sum()
+    fields = {
+        "KSAT": np.full(N, 0.0025 / dt),                        # m per step -> m/s
+        "THETA": np.repeat(0.3 + 0.01 * k, S),
+        "MANNING": np.full(S, 0.025), "ZTOPO": z, "PONDING": np.zeros(S),
+        "PROCMAP": procmap.ravel(), "ROOT": np.tile(prof, 4),
+    }
+    return g, p, fields
+
+
+def forcing(t, nsteps):
+    r = np.arange(4)
+    tr = 2e-5 * (1 + r) / 1000.0                                   # mm/s -> m/s
+    ppt = (4.0 + r if t < nsteps // 2 else 0.0 * r) / 1000.0 / 1800.0  # mm/step -> m/s
+    ev = 1e-5 * np.ones(4) * 1.8 * 0.5 / 1800.0                     # et*1.8*dt_h per step
+    return tr, ppt, ev
+
+
+def run_model(m, nsteps, procmap):
+    mbs = []
+    for t in range(nsteps):
+        tr, ppt, ev = forcing(t, nsteps)
+        m.upload("TRANSP", tr[procmap])
+        m.upload("RAIN", ppt[procmap])
+        m.upload("EVAP", ev[procmap])
+        mbs.append(m.step(1)[0]["mb_ref"])
+    return np.array(mbs)
+
+
+def test_reference_shaped_host_program_matches_c_abi(hip, orc, tmp_path):
+    nx, ny, nz, nsteps = 20, 16, 15, 6
+    out = tmp_path / "shim.bin"
+    print("running", DRIVER, flush=True)
+    r = subprocess.run([DRIVER, str(nx), str(ny), str(nz), str(nsteps), str(out)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    S, N = nx * ny, nx * ny * nz
+    raw = np.fromfile(out, dtype=np.float64)
+    psi_s, th_s = raw[:N], raw[N:2 * N]
+    h_s = raw[2 * N:2 * N + S]
+    mb_s = raw[2 * N + S:2 * N + S + nsteps]
+    col_s = raw[2 * N + S + nsteps:].reshape(nsteps, nz)
+    g, p, fields = si_inputs(nx, ny, nz)
+    pm = fields["PROCMAP"]
+    gm = hip.FlowModel(g, _abi.Params.from_buffer_copy(p))
+    om = orc.OracleModel(g, _abi.Params.from_buffer_copy(p))
+    for m in (gm, om):
+        m.upload_many(fields)
+        m.init_state(_abi.INIT_FROM_THETA)
+    mb_g = run_model(gm, nsteps, pm)
+    run_model(om, nsteps, pm)
+    psi_g = gm.download("PSI")
+    assert np.max(np.abs(psi_s - psi_g)) < 1e-12
+    assert np.max(np.abs(th_s - gm.download("THETA"))) < 1e-12
+    assert np.max(np.abs(h_s - gm.download("PONDING"))) < 1e-12
+    assert np.allclose(mb_s, mb_g, rtol=1e-10, atol=1e-16)
+    assert np.max(np.abs(psi_s - om.download("PSI"))) < 1e-8
+    layers = psi_g.reshape(nz, S)
+    assert np.allclose(col_s[-1], layers[:, :-1].sum(axis=1), rtol=1e-12)
