@@ -89,6 +89,16 @@ class KernelProfile(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class LossSpec(C.Structure):
+    """dhara_loss_spec: misfit of outlet discharge and column storage over a forward run."""
+    _fields_ = [
+        ("nsteps", C.c_int32), ("checkpoint_every", C.c_int32),
+        ("w_q", C.c_double), ("w_w", C.c_double),
+        ("q_target", C.POINTER(C.c_double)), ("w_target", C.POINTER(C.c_double)),
+        ("adj_tol", C.c_double), ("adj_maxiter", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 def default_params(**overrides):
     """Same values as dhara_params_defaults() in include/dhara_hip.h."""
     p = Params(
@@ -145,5 +155,16 @@ def declare(lib, prefix):
         fn = getattr(lib, prefix + name)
         fn.argtypes = args
         fn.restype = C.c_int
+    adjoint = {
+        "observe": [vp, C.c_int32, dp, dp],
+        "adjoint_run": [vp, P(LossSpec), dp, dp, dp, dp, P(StepStats)],
+        "adjoint_residual": [vp, dp, dp],
+        "adjoint_jvp": [vp, dp, C.c_int32, dp, dp],
+    }
+    for name, args in adjoint.items():
+        fn = getattr(lib, prefix + name, None)
+        if fn is not None:
+            fn.argtypes = args
+            fn.restype = C.c_int
     getattr(lib, prefix + "last_error").argtypes = [vp]
     getattr(lib, prefix + "last_error").restype = C.c_char_p

@@ -1,0 +1,1203 @@
+// adjoint.hip -- discrete adjoint of the coupled step: gfx950 kernels and the reverse sweep.
+//
+// No reference counterpart (the reference computes no sensitivities).  What is
+// differentiated is this library's own step (kernels.hip / capi.hip), which restates
+// src/subsurface.cu:686-847, src/global.cu:172-257 and src/overland.cu:387-419:
+//   surface   hp = h^n + (rain - evap) dt, q = the branch of EstimateFluxes/IdentifyTopBoundary
+//             in force (formula, Ks cap, hp/dt or storage deficit), from psi^n, theta^n, K(psi^n)
+//   Richards  F(psi^{n+1}) = 0, the fixed point of the modified Picard iteration:
+//             F_c = (theta_c - theta^n_c)/dt + Ss theta_c (psi_c - psi^n_c)/(phi dt)
+//                   + sum_{faces f of c} (+-) G_f - [top] q/dz + [bottom] K_c/dz - sources,
+//             G_f = w_f K_f (psi_a - psi_b) + [vertical] K_f/dz, K_f = (K_a + K_b)/2,
+//             a = lower-index cell of f (F_a += G_f, F_b -= G_f)
+//   ponding   h' = max(hp - q dt, 0)
+//   overland  H = h' + z, face conductances k_f(H, h', n_manning), one solve A2 H^{n+1} = H
+//             with the ring fixed, h^{n+1} = max(H^{n+1} - z, 0), outlet discharge Q.
+// Reverse step: J^T lambda = dL/dpsi^{n+1} (J = dF/dpsi^{n+1}: BiCGStab with Jacobi
+// preconditioning on the transposed 7-point Jacobian stencil), A2 mu = dL/dH^{n+1} (A2 is
+// symmetric after ring elimination, so the forward's own CG kernels solve it), then
+// pointwise chain-rule kernels.  Forward states of a checkpoint interval are recomputed from
+// its checkpoint; the forward is deterministic, so they are the forward's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace dhc;
+
+struct dhara_adjoint {
+    int cap_T = 0, cap_k = 0, cap_seg = 0;
+    std::vector<void*> allocs;
+    // per 3D cell
+    double *th1 = nullptr, *K1 = nullptr, *C1 = nullptr, *dK1 = nullptr, *dtha1 = nullptr, *dthn1 = nullptr,
+           *dKa1 = nullptr, *dKn1 = nullptr, *Ld = nullptr, *Jd = nullptr;
+    double *lam = nullptr, *psib = nullptr, *psinb = nullptr, *thnb = nullptr, *gK = nullptr, *ga = nullptr,
+           *gn = nullptr;
+    double *br = nullptr, *brh = nullptr, *bp = nullptr, *bv = nullptr, *bs = nullptr, *bt = nullptr,
+           *by = nullptr, *bz = nullptr;
+    double *fin_psi = nullptr, *tmp3 = nullptr;
+    // per surface cell
+    double *hp = nullptr, *qq = nullptr, *hpost = nullptr, *hb = nullptr, *hbp = nullptr, *hpb = nullptr,
+           *qb = nullptr, *hnb = nullptr, *Hpre = nullptr, *mu = nullptr, *Hb1 = nullptr, *W = nullptr,
+           *fin_h = nullptr;
+    int* qbr = nullptr;
+    double *fa_e = nullptr, *fb_e = nullptr, *fc_e = nullptr, *fa_n = nullptr, *fb_n = nullptr, *fc_n = nullptr;
+    // checkpoints, interval tape, misfits
+    double *ck_psi = nullptr, *ck_h = nullptr;
+    double *tp_psi = nullptr, *tp_h = nullptr, *tp_H = nullptr;
+    double *rw = nullptr, *wtarget = nullptr;
+};
+
+namespace dh {
+namespace {
+
+enum QBranch { Q_CAPA = 0, Q_KS = 1, Q_HP = 2, Q_DEFICIT = 3 };
+
+struct VgD {
+    double th, K, C, dK, dth_a, dth_n, dK_a, dK_n;
+};
+
+__device__ __forceinline__ double alpha_of(const Soil& s, int c) { return s.alpha_f ? s.alpha_f[c] : s.alpha; }
+__device__ __forceinline__ double n_of(const Soil& s, int c) { return s.n_f ? s.n_f[c] : s.n; }
+
+// theta, K, C of the forward (vg_eval) and the derivatives the adjoint needs.  With
+// x = alpha|psi|, t = x^n, m = 1 - 1/n, Se = (1+t)^-m, u = t/(1+t):
+//   d ln Se = -dm ln(1+t) - m t/(1+t) d ln t,  d(u^m) = u^m (dm ln u + m d ln t/(1+t)),
+//   dtheta = (ts-tr) Se d ln Se,  dK = K/2 d ln Se - 2 Ks sqrt(Se) (1-u^m) d(u^m),
+// with d ln t = n/psi (psi), n/alpha (alpha), ln x and dm = 1/n^2 (n).
+__device__ __forceinline__ VgD vg_deriv(double psi, double Ks, double a, double n, double ts, double tr)
+{
+    VgD o;
+    vg_eval(psi, Ks, a, n, ts, tr, o.th, o.K, o.C);
+    o.dK = o.dth_a = o.dth_n = o.dK_a = o.dK_n = 0.0;
+    if (!(psi < 0.0)) return o;
+    const double m = 1.0 - 1.0 / n;
+    const double lx = log(-a * psi);
+    const double t = exp(n * lx);
+    const double l1t = log1p(t);
+    const double Se = exp(-m * l1t);
+    const double lu = n * lx - l1t;
+    const double um = exp(m * lu);
+    const double w = -expm1(m * lu);
+    const double sqSe = sqrt(Se);
+    const double tt = t / (1.0 + t);
+    auto var = [&](double dlt, double dm, double& dth, double& dK) {
+        const double dlnSe = -dm * l1t - m * tt * dlt;
+        dth = (ts - tr) * Se * dlnSe;
+        const double dum = um * (dm * lu + m * dlt / (1.0 + t));
+        dK = o.K * 0.5 * dlnSe - 2.0 * Ks * sqSe * w * dum;
+    };
+    double dth_psi;
+    var(n / psi, 0.0, dth_psi, o.dK);
+    var(n / a, 0.0, o.dth_a, o.dK_a);
+    var(lx, 1.0 / (n * n), o.dth_n, o.dK_n);
+    return o;
+}
+
+__device__ __forceinline__ void cell_ijk(const Geo& g, int c, int& i, int& j, int& k)
+{
+    k = c / g.nxy;
+    const int rem = c - k * g.nxy;
+    j = rem / g.nx;
+    i = rem - j * g.nx;
+}
+
+// Visit the faces of cell c: f(nb, w, vertical, c_is_a) with c_is_a when nb has the higher index.
+template <class Fn>
+__device__ __forceinline__ void for_faces(const Geo& g, int c, Fn f)
+{
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    if (i > 0) f(c - 1, g.wx, false, false);
+    if (i < g.nx - 1) f(c + 1, g.wx, false, true);
+    if (j > 0) f(c - g.nx, g.wy, false, false);
+    if (j < g.ny - 1) f(c + g.nx, g.wy, false, true);
+    if (k > 0) f(c - g.nxy, g.wz, true, false);
+    if (k < g.nz - 1) f(c + g.nxy, g.wz, true, true);
+}
+
+struct LinArgs {
+    Geo g;
+    Soil soil;
+    double dt, Ss, phi;
+    const double* psi1;
+    const double* psin;
+    double *th1, *K1, *C1, *dK1, *dtha1, *dthn1, *dKa1, *dKn1, *Ld;
+};
+
+// Constitutive state and derivatives at psi^{n+1}; Ld = d(local part of F_c)/dpsi_c.
+__global__ __launch_bounds__(kBlock) void k_adj_vg(LinArgs a)
+{
+    const Geo g = a.g;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const VgD d = vg_deriv(a.psi1[c], a.soil.Ks[c], alpha_of(a.soil, c), n_of(a.soil, c), a.soil.ts, a.soil.tr);
+        a.th1[c] = d.th;
+        a.K1[c] = d.K;
+        a.C1[c] = d.C;
+        a.dK1[c] = d.dK;
+        a.dtha1[c] = d.dth_a;
+        a.dthn1[c] = d.dth_n;
+        a.dKa1[c] = d.dK_a;
+        a.dKn1[c] = d.dK_n;
+        double ld = d.C / a.dt + a.Ss * (d.C * (a.psi1[c] - a.psin[c]) + d.th) / (a.phi * a.dt);
+        if (c / g.nxy == g.nz - 1) ld += d.dK / g.dz;
+        a.Ld[c] = ld;
+    }
+}
+
+// gamma_f = w (psi_a - psi_b) + [vertical] 1/dz for the face of c towards nb
+__device__ __forceinline__ double face_gamma(const Geo& g, const double* psi, int c, int nb, double w, bool vert,
+                                             bool c_is_a)
+{
+    const double d = c_is_a ? psi[c] - psi[nb] : psi[nb] - psi[c];
+    return w * d + (vert ? 1.0 / g.dz : 0.0);
+}
+
+// out = J^T v (TRANS) or J v, with partial sums (dotw, out) and (out, out).
+template <bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_adj_jac(Geo g, const double* __restrict__ psi1,
+                                                    const double* __restrict__ K1, const double* __restrict__ dK1,
+                                                    const double* __restrict__ Ld, const double* __restrict__ v,
+                                                    double* __restrict__ out, const double* __restrict__ dotw,
+                                                    double* part)
+{
+    double acc2[2] = {0.0, 0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const double vc = v[c], Kc = K1[c], dKc = dK1[c];
+        double acc = Ld[c] * vc;
+        for_faces(g, c, [&](int nb, double w, bool vert, bool c_is_a) {
+            const double Kf = 0.5 * (Kc + K1[nb]);
+            const double gm = face_gamma(g, psi1, c, nb, w, vert, c_is_a);
+            if (TRANS) {
+                // (v_a - v_b) dG_f/dpsi_c
+                if (c_is_a) acc += (vc - v[nb]) * (w * Kf + 0.5 * dKc * gm);
+                else acc += (v[nb] - vc) * (-w * Kf + 0.5 * dKc * gm);
+            } else {
+                const double kk = 0.5 * gm * (dKc * vc + dK1[nb] * v[nb]);
+                acc += w * Kf * (vc - v[nb]) + (c_is_a ? kk : -kk);
+            }
+        });
+        out[c] = acc;
+        if (dotw) acc2[0] += dotw[c] * acc;
+        acc2[1] += acc * acc;
+    }
+    block_partials<2, false>(acc2, part);
+}
+
+// Diagonal of J (= diagonal of J^T), the Jacobi preconditioner of the reverse solve.
+__global__ __launch_bounds__(kBlock) void k_adj_jdiag(Geo g, const double* __restrict__ psi1,
+                                                      const double* __restrict__ K1, const double* __restrict__ dK1,
+                                                      const double* __restrict__ Ld, double* __restrict__ Jd)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        double d = Ld[c];
+        for_faces(g, c, [&](int nb, double w, bool vert, bool c_is_a) {
+            const double Kf = 0.5 * (K1[c] + K1[nb]);
+            const double gm = face_gamma(g, psi1, c, nb, w, vert, c_is_a);
+            d += w * Kf + (c_is_a ? 0.5 : -0.5) * dK1[c] * gm;
+        });
+        Jd[c] = d;
+    }
+}
+
+// ---- BiCGStab vector kernels (van der Vorst 1992, Jacobi right preconditioning) ---------
+__global__ __launch_bounds__(kBlock) void k_bi_init(int n, const double* b, double* r, double* rh, double* p,
+                                                    double* v, double* x, double* part)
+{
+    double acc[1] = {0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const double bc = b[c];
+        r[c] = bc;
+        rh[c] = bc;
+        p[c] = 0.0;
+        v[c] = 0.0;
+        x[c] = 0.0;
+        acc[0] += bc * bc;
+    }
+    block_partials<1, false>(acc, part);
+}
+
+// p = r + beta (p - omega v), y = p / Jd
+__global__ __launch_bounds__(kBlock) void k_bi_p(int n, const double* r, const double* v, const double* Jd,
+                                                 double beta, double omega, double* p, double* y)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const double pc = r[c] + beta * (p[c] - omega * v[c]);
+        p[c] = pc;
+        y[c] = pc / Jd[c];
+    }
+}
+
+// s = r - alpha v, z = s / Jd, partial (s, s)
+__global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* r, const double* v, const double* Jd,
+                                                 double alpha, double* s, double* z, double* part)
+{
+    double acc[1] = {0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const double sc = r[c] - alpha * v[c];
+        s[c] = sc;
+        z[c] = sc / Jd[c];
+        acc[0] += sc * sc;
+    }
+    block_partials<1, false>(acc, part);
+}
+
+// x += alpha y + omega z, r = s - omega t, partials (rh, r), (r, r); omega = 0: x += alpha y
+__global__ __launch_bounds__(kBlock) void k_bi_x(int n, double* x, const double* y, const double* z,
+                                                 const double* s, const double* t, const double* rh, double alpha,
+                                                 double omega, double* r, double* part)
+{
+    double acc[2] = {0.0, 0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        x[c] += alpha * y[c] + omega * z[c];
+        const double rc = s[c] - omega * t[c];
+        r[c] = rc;
+        acc[0] += rh[c] * rc;
+        acc[1] += rc * rc;
+    }
+    block_partials<2, false>(acc, part);
+}
+
+// ---- surface pass of the forward, recomputed with its branch ----------------------------
+struct SurfAdj {
+    int nxy;
+    Soil soil;
+    double dt, dz;
+    int surface_mode;   // 1: top flux from EstimateFluxes/IdentifyTopBoundary
+    const double* psin;
+    const double* hn;
+    const double* rain;
+    const double* evap;
+    double* hp;
+    double* q;
+    int* br;
+    double* hpost;
+};
+
+__global__ __launch_bounds__(kBlock) void k_adj_surface_fwd(SurfAdj a)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nxy; t += gridDim.x * blockDim.x) {
+        const double Ks = a.soil.Ks[t];
+        double th, K, Cc;
+        vg_eval(a.psin[t], Ks, alpha_of(a.soil, t), n_of(a.soil, t), a.soil.ts, a.soil.tr, th, K, Cc);
+        const double h = a.hn[t];
+        const double hp = h + a.rain[t] * a.dt - a.evap[t] * a.dt;
+        double qc = -K * ((a.psin[t] - hp - 0.5 * a.dz) / (0.5 * a.dz));
+        int qcb = Q_CAPA;
+        if (h / a.dt > Ks && qc < Ks) { qc = Ks; qcb = Q_KS; }
+        double q;
+        int b;
+        if (hp > 0.0) {
+            if (hp / a.dt > qc) { q = qc; b = qcb; }
+            else { q = hp / a.dt; b = Q_HP; }
+            const double def = (a.soil.ts - th) * a.dz / a.dt;
+            if (def < q) { q = def; b = Q_DEFICIT; }
+            if (Ks < q) { q = Ks; b = Q_KS; }
+        } else {
+            q = hp / a.dt;
+            b = Q_HP;
+        }
+        a.hp[t] = hp;
+        a.q[t] = q;
+        a.br[t] = b;
+        a.hpost[t] = a.surface_mode ? fmax(hp - q * a.dt, 0.0) : h;
+    }
+}
+
+// ---- subsurface residual (for the parity tests) -----------------------------------------
+struct ResArgs {
+    Geo g;
+    double dt, Ss, phi;
+    const double *psi1, *psin, *thn, *th1, *K1, *qtop;
+    const double* bcq[6];
+    const double *transp, *root;
+    const int* procmap;
+    int has_transp;
+    double* F;
+};
+
+__global__ __launch_bounds__(kBlock) void k_adj_residual(ResArgs a)
+{
+    const Geo g = a.g;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        cell_ijk(g, c, i, j, k);
+        const int t = j * g.nx + i;
+        double f = (a.th1[c] - a.thn[c]) / a.dt + a.Ss * a.th1[c] * (a.psi1[c] - a.psin[c]) / (a.phi * a.dt);
+        for_faces(g, c, [&](int nb, double w, bool vert, bool c_is_a) {
+            const double Kf = 0.5 * (a.K1[c] + a.K1[nb]);
+            const double G = w * Kf * (c_is_a ? a.psi1[c] - a.psi1[nb] : a.psi1[nb] - a.psi1[c]) +
+                             (vert ? Kf / g.dz : 0.0);
+            f += c_is_a ? G : -G;
+        });
+        if (k == 0) f -= a.qtop[t] / g.dz;
+        if (k == g.nz - 1) f += a.K1[c] / g.dz;
+        if (i == 0) f -= a.bcq[FW][k * g.ny + j] / g.dx;
+        if (i == g.nx - 1) f += a.bcq[FE][k * g.ny + j] / g.dx;
+        if (j == 0) f -= a.bcq[FS][k * g.nx + i] / g.dy;
+        if (j == g.ny - 1) f += a.bcq[FN][k * g.nx + i] / g.dy;
+        if (a.has_transp) {
+            const double tr = a.transp[t];
+            if (tr > 0.0) f += tr * a.root[a.procmap[t] * g.nz + k] / g.dz;
+        }
+        a.F[c] = f;
+    }
+}
+
+// ---- reverse chain-rule kernels -----------------------------------------------------------
+// Column-storage misfit after a step: psib += r C, galpha += r dtheta/dalpha, gn += r dtheta/dn.
+__global__ __launch_bounds__(kBlock) void k_adj_wmisfit(Geo g, const double* rw, const double* C1,
+                                                        const double* dtha1, const double* dthn1, double* psib,
+                                                        double* ga, double* gn)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const double r = rw[c % g.nxy] * g.dz;
+        psib[c] += r * C1[c];
+        ga[c] += r * dtha1[c];
+        gn[c] += r * dthn1[c];
+    }
+}
+
+struct ParamArgs {
+    Geo g;
+    double dt, Ss, phi;
+    const double *lam, *psi1, *psin, *th1, *K1, *dtha1, *dthn1, *dKa1, *dKn1;
+    double *gK, *ga, *gn, *psinb, *thnb;
+};
+
+// -lambda^T dF/dp through theta(psi^{n+1}) and K(psi^{n+1}); explicit psi^n and theta^n terms.
+__global__ __launch_bounds__(kBlock) void k_adj_param(ParamArgs a)
+{
+    const Geo g = a.g;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const double lc = a.lam[c];
+        double Kb = (c / g.nxy == g.nz - 1) ? lc / g.dz : 0.0;
+        for_faces(g, c, [&](int nb, double w, bool vert, bool c_is_a) {
+            const double gm = face_gamma(g, a.psi1, c, nb, w, vert, c_is_a);
+            Kb += (c_is_a ? lc - a.lam[nb] : a.lam[nb] - lc) * 0.5 * gm;
+        });
+        const double cth = 1.0 / a.dt + a.Ss * (a.psi1[c] - a.psin[c]) / (a.phi * a.dt);
+        a.gK[c] -= Kb * a.K1[c];
+        a.ga[c] -= lc * cth * a.dtha1[c] + Kb * a.dKa1[c];
+        a.gn[c] -= lc * cth * a.dthn1[c] + Kb * a.dKn1[c];
+        a.psinb[c] = lc * a.Ss * a.th1[c] / (a.phi * a.dt);
+        a.thnb[c] = lc / a.dt;
+    }
+}
+
+struct SurfRevArgs {
+    int nxy;
+    Soil soil;
+    double dt, dz;
+    int surface_mode;
+    const double *psin, *hp, *lam, *qb, *hbp;
+    const int* br;
+    double *hpb, *psinb, *thnb, *gK, *ga, *gn, *hnb;
+};
+
+// Reverse of the ponding update and of EstimateFluxes/IdentifyTopBoundary for the branch the
+// forward took; hnb = dL/dh^n.
+__global__ __launch_bounds__(kBlock) void k_adj_surface_rev(SurfRevArgs a)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nxy; t += gridDim.x * blockDim.x) {
+        if (!a.surface_mode) { a.hnb[t] = a.hbp[t]; continue; }
+        const double Ks = a.soil.Ks[t];
+        double hpb = a.hpb[t];
+        const double q = a.lam[t] / a.dz + a.qb[t];
+        const VgD d = vg_deriv(a.psin[t], Ks, alpha_of(a.soil, t), n_of(a.soil, t), a.soil.ts, a.soil.tr);
+        double Knb = 0.0;
+        switch (a.br[t]) {
+        case Q_CAPA:
+            Knb = -q * (a.psin[t] - a.hp[t] - 0.5 * a.dz) / (0.5 * a.dz);
+            a.psinb[t] += -q * d.K / (0.5 * a.dz);
+            hpb += q * d.K / (0.5 * a.dz);
+            break;
+        case Q_KS: a.gK[t] += q * Ks; break;
+        case Q_HP: hpb += q / a.dt; break;
+        default: a.thnb[t] += -q * a.dz / a.dt; break;
+        }
+        a.psinb[t] += Knb * d.dK;
+        a.gK[t] += Knb * d.K;
+        a.ga[t] += Knb * d.dK_a;
+        a.gn[t] += Knb * d.dK_n;
+        a.hnb[t] = hpb;
+    }
+}
+
+// Ponding update reverse: h' = max(hp - q dt, 0) -> hpb, qb.
+__global__ __launch_bounds__(kBlock) void k_adj_ponding(int nxy, double dt, const double* hp, const double* q,
+                                                        const double* hbp, double* hpb, double* qb)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxy; t += gridDim.x * blockDim.x) {
+        const bool wet = hp[t] - q[t] * dt > 0.0;
+        hpb[t] = wet ? hbp[t] : 0.0;
+        qb[t] = wet ? -dt * hbp[t] : 0.0;
+    }
+}
+
+// theta^n = theta(psi^n): psinb += thnb C(psi^n), parameters through theta.
+__global__ __launch_bounds__(kBlock) void k_adj_theta_n(Geo g, Soil s, const double* psin, const double* thnb,
+                                                        double* psinb, double* ga, double* gn)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const double tb = thnb[c];
+        if (tb == 0.0) continue;
+        const VgD d = vg_deriv(psin[c], s.Ks[c], alpha_of(s, c), n_of(s, c), s.ts, s.tr);
+        psinb[c] += tb * d.C;
+        ga[c] += tb * d.dth_a;
+        gn[c] += tb * d.dth_n;
+    }
+}
+
+// Column storage W = sum_k theta dz, its misfit r = 2 ww (W - W*) and partial sums of ww (W - W*)^2.
+__global__ __launch_bounds__(kBlock) void k_colstore(Geo g, const double* theta, const double* wt, double ww,
+                                                     double* W, double* rw, double* part)
+{
+    double acc[1] = {0.0};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.nxy; t += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < g.nz; ++k) s += theta[k * g.nxy + t] * g.dz;
+        if (W) W[t] = s;
+        const double d = s - (wt ? wt[t] : 0.0);
+        if (rw) rw[t] = 2.0 * ww * d;
+        acc[0] += ww * d * d;
+    }
+    block_partials<1, false>(acc, part);
+}
+
+// ---- overland reverse ---------------------------------------------------------------------
+__device__ __forceinline__ bool ring2(const Geo2& g, int i, int j)
+{
+    return i == 0 || j == 0 || i == g.nx - 1 || j == g.ny - 1;
+}
+
+struct OlfAdj {
+    Geo2 g;
+    double hmin, delta, K0, rq;
+    const double *H1, *Hpre, *hpost, *z, *mann, *ke, *kn, *hb, *mu;
+    double *Hb1, *D, *r, *x, *part;
+    PcgState* st;
+    double *fa_e, *fb_e, *fc_e, *fa_n, *fb_n, *fc_n, *hbp;
+};
+
+// dL/dH^{n+1}: depth clip and outlet discharge Q = sum ke (H_(1,j) - H_(0,j)) dy/dx.
+__global__ __launch_bounds__(kBlock) void k_adj_olf_rhs(OlfAdj a)
+{
+    const Geo2 g = a.g;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.n; t += gridDim.x * blockDim.x) {
+        const int j = t / g.nx, i = t - j * g.nx;
+        double v = (a.H1[t] - a.z[t] > 0.0) ? a.hb[t] : 0.0;
+        if (j > 0 && j < g.ny - 1 && g.nx > 2) {
+            if (i == 1) v += a.rq * a.ke[t - 1] * g.dy / g.dx;
+            if (i == 0) v -= a.rq * a.ke[t] * g.dy / g.dx;
+        }
+        a.Hb1[t] = v;
+    }
+}
+
+// Start of the interior solve A2 mu = Hb1 (ring rows identity with zero right-hand side):
+// D as the forward assembly, r = b, x = 0, partials [rz, rr, bb, rr] for the CG kernels.
+__global__ __launch_bounds__(kBlock) void k_adj_olf_init(OlfAdj a)
+{
+    const Geo2 g = a.g;
+    const double dtodx2 = -g.cx, dtody2 = -g.cy;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.n; t += gridDim.x * blockDim.x) {
+        const int j = t / g.nx, i = t - j * g.nx;
+        double D = 1.0, b = 0.0;
+        if (!ring2(g, i, j)) {
+            D = 1.0 - dtodx2 * (a.ke[t] + a.ke[t - 1]) - dtody2 * (a.kn[t] + a.kn[t - g.nx]);
+            b = a.Hb1[t];
+        }
+        a.D[t] = D;
+        a.r[t] = b;
+        a.x[t] = 0.0;
+        acc[0] += b * (b / D);
+        acc[1] += b * b;
+        acc[2] += b * b;
+        acc[3] += b * b;
+    }
+    block_partials<4, false>(acc, a.part);
+}
+
+// Ring cells: mu_r = Hb1_r + sum over interior neighbours of c_f k_f mu_nb.
+__global__ __launch_bounds__(kBlock) void k_adj_olf_ring(OlfAdj a, double* mu)
+{
+    const Geo2 g = a.g;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.n; t += gridDim.x * blockDim.x) {
+        const int j = t / g.nx, i = t - j * g.nx;
+        if (!ring2(g, i, j)) continue;
+        double v = a.Hb1[t];
+        if (i + 1 < g.nx && !ring2(g, i + 1, j)) v += g.cx * a.ke[t] * mu[t + 1];
+        if (i > 0 && !ring2(g, i - 1, j)) v += g.cx * a.ke[t - 1] * mu[t - 1];
+        if (j + 1 < g.ny && !ring2(g, i, j + 1)) v += g.cy * a.kn[t] * mu[t + g.nx];
+        if (j > 0 && !ring2(g, i, j - 1)) v += g.cy * a.kn[t - g.nx] * mu[t - g.nx];
+        mu[t] = v;
+    }
+}
+
+// Derivative data of the forward's face conductance (olf_face_k, same branch logic).
+__device__ __forceinline__ bool face_active(double gdir, double gcross, double h_nb, double h_c, bool up_nb,
+                                            double hmin, double delta, double& S, double& hav)
+{
+    S = sqrt(gdir * gdir + gcross * gcross);
+    const double hup = up_nb ? h_nb : h_c;
+    hav = (h_nb + h_c) / 2.;
+    return hup > hmin && hav > hmin && fabs(S) > delta;
+}
+
+// Per face: kbar = dL/dk_f from the solve (-c_f (H_a - H_b)(mu_a - mu_b), interior rows only)
+// and from Q, times dk/d(direct gradient), dk/d(cross gradient), dk/dh.
+__global__ __launch_bounds__(kBlock) void k_adj_olf_faces(OlfAdj a)
+{
+    const Geo2 g = a.g;
+    const int nx = g.nx;
+    const double* H = a.Hpre;
+    const double* h = a.hpost;
+    const double* mu = a.mu;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const int j = c / nx, i = c - j * nx;
+        double fa = 0.0, fb = 0.0, fc = 0.0;
+        if (i < nx - 1 && j > 0 && j < g.ny - 1) {
+            const bool ai = !ring2(g, i, j), bi = !ring2(g, i + 1, j);
+            double kb = -g.cx * (a.H1[c] - a.H1[c + 1]) * ((ai ? mu[c] : 0.0) - (bi ? mu[c + 1] : 0.0));
+            if (i == 0 && nx > 2) kb += a.rq * (a.H1[c + 1] - a.H1[c]) * g.dy / g.dx;
+            const double gx = (H[c + 1] - H[c]) / g.dx;
+            const double gy = (H[c + nx + 1] + H[c + nx] - H[c - nx + 1] - H[c - nx]) / (4 * g.dy);
+            const bool up_east = ai ? (H[c + 1] > H[c]) : !(H[c] > H[c + 1]);
+            double S, hav;
+            if (face_active(gx, gy, h[c + 1], h[c], up_east, a.hmin, a.delta, S, hav)) {
+                const double k = a.ke[c];
+                fa = kb * (-0.5 * k * gx / (S * S));
+                fb = kb * (-0.5 * k * gy / (S * S));
+                fc = kb * (5. / 6.) * k / hav;
+            }
+        }
+        a.fa_e[c] = fa;
+        a.fb_e[c] = fb;
+        a.fc_e[c] = fc;
+        fa = fb = fc = 0.0;
+        if (j < g.ny - 1 && i > 0 && i < nx - 1) {
+            const bool ai = !ring2(g, i, j), bi = !ring2(g, i, j + 1);
+            const double kb = -g.cy * (a.H1[c] - a.H1[c + nx]) * ((ai ? mu[c] : 0.0) - (bi ? mu[c + nx] : 0.0));
+            const double gyy = (H[c + nx] - H[c]) / g.dy;
+            const double gxx = (H[c + nx + 1] + H[c + 1] - H[c + nx - 1] - H[c - 1]) / (4 * g.dx);
+            const bool up_north = ai ? (H[c + nx] > H[c]) : !(H[c] > H[c + nx]);
+            double S, hav;
+            if (face_active(gyy, gxx, h[c + nx], h[c], up_north, a.hmin, a.delta, S, hav)) {
+                const double k = a.kn[c];
+                fa = kb * (-0.5 * k * gyy / (S * S));
+                fb = kb * (-0.5 * k * gxx / (S * S));
+                fc = kb * (5. / 6.) * k / hav;
+            }
+        }
+        a.fa_n[c] = fa;
+        a.fb_n[c] = fb;
+        a.fc_n[c] = fc;
+    }
+}
+
+// dL/dh' = mu (H = h' + z) + the face terms that reference each cell (gathered; faces that
+// do not exist hold zeros, so index wrap-around at row ends adds nothing).
+__global__ __launch_bounds__(kBlock) void k_adj_olf_gather(OlfAdj a)
+{
+    const Geo2 g = a.g;
+    const int nx = g.nx, S = g.n;
+    auto at = [&](const double* f, int e) -> double { return (e >= 0 && e < S) ? f[e] : 0.0; };
+    const double ex = 1.0 / g.dx, ey = 1.0 / g.dy, cx4 = 1.0 / (4 * g.dx), cy4 = 1.0 / (4 * g.dy);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < S; t += gridDim.x * blockDim.x) {
+        double Hb = a.mu[t];
+        Hb += (at(a.fa_e, t - 1) - a.fa_e[t]) * ex;
+        Hb += (at(a.fb_e, t - nx - 1) + at(a.fb_e, t - nx) - at(a.fb_e, t + nx - 1) - at(a.fb_e, t + nx)) * cy4;
+        Hb += (at(a.fa_n, t - nx) - a.fa_n[t]) * ey;
+        Hb += (at(a.fb_n, t - nx - 1) + at(a.fb_n, t - 1) - at(a.fb_n, t - nx + 1) - at(a.fb_n, t + 1)) * cx4;
+        const double hb = a.fc_e[t] + at(a.fc_e, t - 1) + a.fc_n[t] + at(a.fc_n, t - nx);
+        a.hbp[t] = Hb + hb;
+    }
+}
+
+}  // namespace
+
+}  // namespace dh
+
+// ---- launchers (this file only: the argument structs are local to it) --------------------------------------------------
+namespace dh {
+
+void launch_adj_colstore(hipStream_t s, const Geo& g, const double* theta, const double* wt, double ww, double* W,
+                         double* rw, double* part)
+{
+    hipLaunchKernelGGL(k_colstore, dim3(grid_for(g.nxy)), dim3(kBlock), 0, s, g, theta, wt, ww, W, rw, part);
+}
+
+void launch_adj_vg(hipStream_t s, const LinArgs& a)
+{
+    hipLaunchKernelGGL(k_adj_vg, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_jac(hipStream_t s, const Geo& g, bool trans, const double* psi1, const double* K1,
+                    const double* dK1, const double* Ld, const double* v, double* out, const double* dotw,
+                    double* part)
+{
+    const dim3 gr(grid_for(g.n)), bl(kBlock);
+    if (trans) hipLaunchKernelGGL(k_adj_jac<true>, gr, bl, 0, s, g, psi1, K1, dK1, Ld, v, out, dotw, part);
+    else hipLaunchKernelGGL(k_adj_jac<false>, gr, bl, 0, s, g, psi1, K1, dK1, Ld, v, out, dotw, part);
+}
+
+void launch_adj_jdiag(hipStream_t s, const Geo& g, const double* psi1, const double* K1, const double* dK1,
+                      const double* Ld, double* Jd)
+{
+    hipLaunchKernelGGL(k_adj_jdiag, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, psi1, K1, dK1, Ld, Jd);
+}
+
+void launch_bi_init(hipStream_t s, int n, const double* b, double* r, double* rh, double* p, double* v, double* x,
+                    double* part)
+{
+    hipLaunchKernelGGL(k_bi_init, dim3(grid_for(n)), dim3(kBlock), 0, s, n, b, r, rh, p, v, x, part);
+}
+
+void launch_bi_p(hipStream_t s, int n, const double* r, const double* v, const double* Jd, double beta,
+                 double omega, double* p, double* y)
+{
+    hipLaunchKernelGGL(k_bi_p, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, v, Jd, beta, omega, p, y);
+}
+
+void launch_bi_s(hipStream_t s, int n, const double* r, const double* v, const double* Jd, double alpha,
+                 double* sv, double* z, double* part)
+{
+    hipLaunchKernelGGL(k_bi_s, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, v, Jd, alpha, sv, z, part);
+}
+
+void launch_bi_x(hipStream_t s, int n, double* x, const double* y, const double* z, const double* sv,
+                 const double* t, const double* rh, double alpha, double omega, double* r, double* part)
+{
+    hipLaunchKernelGGL(k_bi_x, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, y, z, sv, t, rh, alpha, omega, r, part);
+}
+
+void launch_adj_surface_fwd(hipStream_t s, const SurfAdj& a)
+{
+    hipLaunchKernelGGL(k_adj_surface_fwd, dim3(grid_for(a.nxy)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_residual(hipStream_t s, const ResArgs& a)
+{
+    hipLaunchKernelGGL(k_adj_residual, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_wmisfit(hipStream_t s, const Geo& g, const double* rw, const double* C1, const double* dtha1,
+                        const double* dthn1, double* psib, double* ga, double* gn)
+{
+    hipLaunchKernelGGL(k_adj_wmisfit, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, rw, C1, dtha1, dthn1, psib, ga, gn);
+}
+
+void launch_adj_param(hipStream_t s, const ParamArgs& a)
+{
+    hipLaunchKernelGGL(k_adj_param, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_surface_rev(hipStream_t s, const SurfRevArgs& a)
+{
+    hipLaunchKernelGGL(k_adj_surface_rev, dim3(grid_for(a.nxy)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_ponding(hipStream_t s, int nxy, double dt, const double* hp, const double* q, const double* hbp,
+                        double* hpb, double* qb)
+{
+    hipLaunchKernelGGL(k_adj_ponding, dim3(grid_for(nxy)), dim3(kBlock), 0, s, nxy, dt, hp, q, hbp, hpb, qb);
+}
+
+void launch_adj_theta_n(hipStream_t s, const Geo& g, const Soil& soil, const double* psin, const double* thnb,
+                        double* psinb, double* ga, double* gn)
+{
+    hipLaunchKernelGGL(k_adj_theta_n, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, soil, psin, thnb, psinb, ga, gn);
+}
+
+void launch_adj_olf_rhs(hipStream_t s, const OlfAdj& a)
+{
+    hipLaunchKernelGGL(k_adj_olf_rhs, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_olf_init(hipStream_t s, const OlfAdj& a)
+{
+    hipLaunchKernelGGL(k_adj_olf_init, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_olf_ring(hipStream_t s, const OlfAdj& a, double* mu)
+{
+    hipLaunchKernelGGL(k_adj_olf_ring, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a, mu);
+}
+
+void launch_adj_olf_faces(hipStream_t s, const OlfAdj& a)
+{
+    hipLaunchKernelGGL(k_adj_olf_faces, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_adj_olf_gather(hipStream_t s, const OlfAdj& a)
+{
+    hipLaunchKernelGGL(k_adj_olf_gather, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+}  // namespace dh
+
+// =========================================================================================
+// Host side
+// =========================================================================================
+namespace dhc {
+
+void free_adjoint(dhara_ctx* c)
+{
+    if (!c->adj) return;
+    for (void* v : c->adj->allocs) (void)hipFree(v);
+    delete c->adj;
+    c->adj = nullptr;
+}
+
+}  // namespace dhc
+
+namespace {
+
+template <typename T>
+int aalloc(dhara_ctx* c, T** p, size_t n)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return set_err(c, DHARA_E_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->adj->allocs.push_back(v);
+    e = hipMemsetAsync(v, 0, std::max<size_t>(n, 1) * sizeof(T), c->s);
+    if (e != hipSuccess) return set_err(c, DHARA_E_DEVICE, "hipMemset");
+    *p = static_cast<T*>(v);
+    return DHARA_OK;
+}
+
+// Buffers for T steps with checkpoint interval k (grown on demand).
+int ensure_adjoint(dhara_ctx* c, int T, int k)
+{
+    const int nseg = (T + k - 1) / k;
+    if (c->adj && T <= c->adj->cap_T && k <= c->adj->cap_k && nseg <= c->adj->cap_seg) return DHARA_OK;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    free_adjoint(c);
+    c->adj = new dhara_adjoint;
+    dhara_adjoint& a = *c->adj;
+    a.cap_T = T;
+    a.cap_k = k;
+    a.cap_seg = nseg;
+    const size_t N = c->N, S = c->S;
+    int rc;
+    double** n3[] = {&a.th1, &a.K1, &a.C1, &a.dK1, &a.dtha1, &a.dthn1, &a.dKa1, &a.dKn1, &a.Ld, &a.Jd,
+                     &a.lam, &a.psib, &a.psinb, &a.thnb, &a.gK, &a.ga, &a.gn, &a.br, &a.brh, &a.bp,
+                     &a.bv, &a.bs, &a.bt, &a.by, &a.bz, &a.fin_psi, &a.tmp3};
+    for (double** p : n3)
+        if ((rc = aalloc(c, p, N))) return rc;
+    double** n2[] = {&a.hp, &a.qq, &a.hpost, &a.hb, &a.hbp, &a.hpb, &a.qb, &a.hnb, &a.Hpre, &a.mu, &a.Hb1,
+                     &a.W, &a.fin_h, &a.fa_e, &a.fb_e, &a.fc_e, &a.fa_n, &a.fb_n, &a.fc_n};
+    for (double** p : n2)
+        if ((rc = aalloc(c, p, S))) return rc;
+    if ((rc = aalloc(c, &a.qbr, S)) || (rc = aalloc(c, &a.ck_psi, N * nseg)) || (rc = aalloc(c, &a.ck_h, S * nseg)) ||
+        (rc = aalloc(c, &a.tp_psi, N * (k + 1))) || (rc = aalloc(c, &a.tp_h, S * (k + 1))) ||
+        (rc = aalloc(c, &a.tp_H, S * k)) || (rc = aalloc(c, &a.rw, S * T)) || (rc = aalloc(c, &a.wtarget, S * T)))
+        return rc;
+    return DHARA_OK;
+}
+
+int check_adjoint_config(dhara_ctx* c)
+{
+    if (!c->initialized) return set_err(c, DHARA_E_STATE, "dhara_init_state not called after a state upload");
+    if (c->p.precision != DHARA_FP64) return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs FP64 precision");
+    if (c->p.am != 0.0) return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs am = 0");
+    if (c->p.theta_carry != DHARA_THETA_CARRY_CONSISTENT)
+        return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs the consistent theta carry");
+    if (has_dirichlet(c)) return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs flux faces");
+    return DHARA_OK;
+}
+
+int one_step(dhara_ctx* c, dhara_step_stats* st)
+{
+    std::memset(st, 0, sizeof(*st));
+    int rc = do_subsurface_step(c, st);
+    if (rc == DHARA_OK && c->p.overland) rc = do_overland_step(c, st);
+    return rc;
+}
+
+// Set the model state to (psi, h) the way the end of a step leaves it.
+int restore_state(dhara_ctx* c, const double* psi, const double* h)
+{
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->psi_n, psi, nb, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->x, psi, nb, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->psi_m, psi, nb, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->h, h, (size_t)c->S * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    dh::launch_vg(c->s, vg_args(c, c->psi_n, c->theta_n));
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(c->theta_new, c->theta_n, nb, hipMemcpyDeviceToDevice, c->s));
+    c->theta_m_current = false;
+    return DHARA_OK;
+}
+
+int colstore(dhara_ctx* c, const double* wt, double ww, double* W, double* rw, double* sum)
+{
+    dh::launch_adj_colstore(c->s, c->g, c->theta_n, wt, ww, W, rw, c->part[0]);
+    LAUNCHCHK(c);
+    return host_reduce(c, dh::grid_for(c->S), 1, false, sum);
+}
+
+dh::LinArgs lin_args(dhara_ctx* c, const double* psi1, const double* psin)
+{
+    dhara_adjoint& a = *c->adj;
+    dh::LinArgs l{};
+    l.g = c->g;
+    l.soil = soil_of(c);
+    l.dt = c->p.dt;
+    l.Ss = c->p.Ss;
+    l.phi = c->p.porosity;
+    l.psi1 = psi1;
+    l.psin = psin;
+    l.th1 = a.th1; l.K1 = a.K1; l.C1 = a.C1; l.dK1 = a.dK1; l.dtha1 = a.dtha1; l.dthn1 = a.dthn1;
+    l.dKa1 = a.dKa1; l.dKn1 = a.dKn1; l.Ld = a.Ld;
+    return l;
+}
+
+dh::SurfAdj surf_adj(dhara_ctx* c, const double* psin, const double* hn)
+{
+    dhara_adjoint& a = *c->adj;
+    dh::SurfAdj s{};
+    s.nxy = c->S;
+    s.soil = soil_of(c);
+    s.dt = c->p.dt;
+    s.dz = c->grid.dz;
+    s.surface_mode = c->p.top_bc == DHARA_TOP_SURFACE ? 1 : 0;
+    s.psin = psin;
+    s.hn = hn;
+    s.rain = c->rain;
+    s.evap = c->evap;
+    s.hp = a.hp;
+    s.q = a.qq;
+    s.br = a.qbr;
+    s.hpost = a.hpost;
+    return s;
+}
+
+// BiCGStab on J^T lambda = b (b, lambda in the adjoint's buffers).
+int solve_jt(dhara_ctx* c, const double* psi1, const double* b, double* x, double tol, int maxit, long* iters,
+             bool* conv)
+{
+    dhara_adjoint& a = *c->adj;
+    const int n = c->N, np = dh::grid_for(c->N);
+    dh::launch_bi_init(c->s, n, b, a.br, a.brh, a.bp, a.bv, x, c->part[0]);
+    LAUNCHCHK(c);
+    double bb;
+    int rc = host_reduce(c, np, 1, false, &bb);
+    if (rc) return rc;
+    *conv = true;
+    if (bb == 0.0) return DHARA_OK;
+    const double bn = std::sqrt(bb);
+    double rho = 1.0, alpha = 1.0, omega = 1.0, rho1 = bb;
+    int it = 0;
+    *conv = false;
+    while (it < maxit) {
+        if (rho1 == 0.0 || !std::isfinite(rho1)) break;
+        const double beta = (rho1 / rho) * (alpha / omega);
+        rho = rho1;
+        dh::launch_bi_p(c->s, n, a.br, a.bv, a.Jd, beta, omega, a.bp, a.by);
+        dh::launch_adj_jac(c->s, c->g, true, psi1, a.K1, a.dK1, a.Ld, a.by, a.bv, a.brh, c->part[0]);
+        LAUNCHCHK(c);
+        double d2[2];
+        if ((rc = host_reduce(c, np, 2, false, d2))) return rc;
+        alpha = rho / d2[0];
+        if (!std::isfinite(alpha)) break;
+        dh::launch_bi_s(c->s, n, a.br, a.bv, a.Jd, alpha, a.bs, a.bz, c->part[0]);
+        LAUNCHCHK(c);
+        double ss;
+        if ((rc = host_reduce(c, np, 1, false, &ss))) return rc;
+        ++it;
+        if (std::sqrt(ss) <= tol * bn) {
+            dh::launch_bi_x(c->s, n, x, a.by, a.bz, a.bs, a.bt, a.brh, alpha, 0.0, a.br, c->part[0]);
+            LAUNCHCHK(c);
+            *conv = true;
+            break;
+        }
+        dh::launch_adj_jac(c->s, c->g, true, psi1, a.K1, a.dK1, a.Ld, a.bz, a.bt, a.bs, c->part[0]);
+        LAUNCHCHK(c);
+        if ((rc = host_reduce(c, np, 2, false, d2))) return rc;
+        omega = d2[1] > 0.0 ? d2[0] / d2[1] : 0.0;
+        dh::launch_bi_x(c->s, n, x, a.by, a.bz, a.bs, a.bt, a.brh, alpha, omega, a.br, c->part[0]);
+        LAUNCHCHK(c);
+        if ((rc = host_reduce(c, np, 2, false, d2))) return rc;
+        rho1 = d2[0];
+        if (std::sqrt(d2[1]) <= tol * bn) { *conv = true; break; }
+        if (omega == 0.0) break;
+    }
+    *iters += it;
+    return DHARA_OK;
+}
+
+// Reverse of the overland step of one forward step: hb (dL/dh^{n+1}) -> hbp (dL/dh').
+int overland_reverse(dhara_ctx* c, const double* H1, double rq, double tol, int maxit, long* iters, bool* conv)
+{
+    dhara_adjoint& ad = *c->adj;
+    // face conductances of the forward at H = h' + z
+    dh::OlfArgs oa = olf_args(c);
+    oa.h = ad.hpost;
+    oa.H = ad.Hpre;
+    dh::launch_olf_faces(c->s, oa);
+    dh::OlfAdj a{};
+    a.g = c->g2;
+    a.hmin = c->p.hmin;
+    a.delta = c->p.delta;
+    a.K0 = c->p.K0;
+    a.rq = rq;
+    a.H1 = H1;
+    a.Hpre = ad.Hpre;
+    a.hpost = ad.hpost;
+    a.z = c->z;
+    a.mann = c->mann;
+    a.ke = c->ke;
+    a.kn = c->kn;
+    a.hb = ad.hb;
+    a.mu = ad.mu;
+    a.Hb1 = ad.Hb1;
+    a.D = c->D2;
+    a.r = c->r2;
+    a.x = ad.mu;
+    a.part = c->part[0];
+    a.st = c->st_olf;
+    a.fa_e = ad.fa_e; a.fb_e = ad.fb_e; a.fc_e = ad.fc_e;
+    a.fa_n = ad.fa_n; a.fb_n = ad.fb_n; a.fc_n = ad.fc_n;
+    a.hbp = ad.hbp;
+    dh::launch_adj_olf_rhs(c->s, a);
+    dh::launch_adj_olf_init(c->s, a);
+    LAUNCHCHK(c);
+    double* const pp[2] = {c->p2[0], c->p2[1]};
+    dh::SolveCtl sc = solve_ctl(c, c->st_olf, maxit, dh::grid_for(c->S), false, true);
+    sc.lin_tol = tol;
+    int hint = c->olf_hint;
+    int rc = run_krylov(
+        c, c->st_olf, maxit, hint, false, 8.0,
+        [&](int it, const dh::LaunchEv& ev) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            dh::launch_pcg2_spmv(c->s, c->g2, c->ke, c->kn, c->D2, c->r2, pp, c->q2, k);
+        },
+        [&](int it, const dh::LaunchEv& ev) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            k.part_in = c->part[1];
+            k.part_out = c->part[0];
+            dh::launch_pcg_update_f64(c->s, c->S, ad.mu, c->r2, c->D2, pp[(it + 1) & 1], c->q2, k);
+        });
+    if (rc) return rc;
+    *iters += c->h_st->iter;
+    *conv = *conv && c->h_st->conv;
+    dh::launch_adj_olf_ring(c->s, a, ad.mu);
+    dh::launch_adj_olf_faces(c->s, a);
+    dh::launch_adj_olf_gather(c->s, a);
+    LAUNCHCHK(c);
+    return DHARA_OK;
+}
+
+}  // namespace
+
+// =========================================================================================
+// C-ABI
+// =========================================================================================
+extern "C" {
+
+int dhara_observe(dhara_ctx* c, int32_t nsteps, double* q, double* w)
+{
+    if (!c || nsteps < 0) return DHARA_E_ARG;
+    if (!c->initialized) return set_err(c, DHARA_E_STATE, "dhara_init_state not called after a state upload");
+    int rc = ensure_adjoint(c, 1, 1);
+    if (rc) return rc;
+    for (int s = 0; s < nsteps; ++s) {
+        dhara_step_stats st;
+        if ((rc = one_step(c, &st))) return rc;
+        if (q) q[s] = c->p.overland ? st.outlet_discharge : 0.0;
+        if (w) {
+            double sum;
+            if ((rc = colstore(c, nullptr, 0.0, c->adj->W, nullptr, &sum))) return rc;
+            HIPCHK(c, hipMemcpyAsync(w + (size_t)s * c->S, c->adj->W, (size_t)c->S * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->s));
+            HIPCHK(c, hipStreamSynchronize(c->s));
+        }
+    }
+    return DHARA_OK;
+}
+
+int dhara_adjoint_residual(dhara_ctx* c, const double* psi_new, double* F)
+{
+    if (!c || !psi_new || !F) return DHARA_E_ARG;
+    int rc = check_adjoint_config(c);
+    if (rc || (rc = ensure_adjoint(c, 1, 1))) return rc;
+    dhara_adjoint& a = *c->adj;
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(a.tmp3, psi_new, nb, hipMemcpyHostToDevice, c->s));
+    dh::launch_adj_vg(c->s, lin_args(c, a.tmp3, c->psi_n));
+    dh::launch_adj_surface_fwd(c->s, surf_adj(c, c->psi_n, c->h));
+    dh::ResArgs r{};
+    r.g = c->g;
+    r.dt = c->p.dt;
+    r.Ss = c->p.Ss;
+    r.phi = c->p.porosity;
+    r.psi1 = a.tmp3;
+    r.psin = c->psi_n;
+    r.thn = c->theta_n;
+    r.th1 = a.th1;
+    r.K1 = a.K1;
+    r.qtop = c->p.top_bc == DHARA_TOP_SURFACE ? a.qq : c->bcq[dh::FT];
+    for (int f = 0; f < 6; ++f) r.bcq[f] = c->bcq[f];
+    r.transp = c->transp;
+    r.root = c->root;
+    r.procmap = c->procmap;
+    r.has_transp = c->has_transp ? 1 : 0;
+    r.F = a.lam;
+    dh::launch_adj_residual(c->s, r);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(F, a.lam, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_adjoint_jvp(dhara_ctx* c, const double* psi_new, int32_t transpose, const double* v, double* out)
+{
+    if (!c || !psi_new || !v || !out) return DHARA_E_ARG;
+    int rc = check_adjoint_config(c);
+    if (rc || (rc = ensure_adjoint(c, 1, 1))) return rc;
+    dhara_adjoint& a = *c->adj;
+    const size_t nb = (size_t)c->N * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(a.tmp3, psi_new, nb, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(a.br, v, nb, hipMemcpyHostToDevice, c->s));
+    dh::launch_adj_vg(c->s, lin_args(c, a.tmp3, c->psi_n));
+    dh::launch_adj_jac(c->s, c->g, transpose != 0, a.tmp3, a.K1, a.dK1, a.Ld, a.br, a.bv, nullptr, c->part[0]);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(out, a.bv, nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_adjoint_run(dhara_ctx* c, const dhara_loss_spec* spec, double* loss, double* g_lnks, double* g_alpha,
+                      double* g_n, dhara_step_stats* stats)
+{
+    if (!c || !spec || spec->nsteps < 1 || spec->checkpoint_every < 1) return DHARA_E_ARG;
+    int rc = check_adjoint_config(c);
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int T = spec->nsteps, kc = std::min(spec->checkpoint_every, spec->nsteps);
+    if ((rc = ensure_adjoint(c, T, kc))) return rc;
+    dhara_adjoint& a = *c->adj;
+    const size_t N = c->N, S = c->S, nb3 = N * sizeof(double), nb2 = S * sizeof(double);
+    const double tol = spec->adj_tol > 0 ? spec->adj_tol : 1e-12;
+    const int maxit = spec->adj_maxiter > 0 ? spec->adj_maxiter : 5000;
+    const bool olf = c->p.overland != 0;
+    if (spec->w_target)
+        HIPCHK(c, hipMemcpyAsync(a.wtarget, spec->w_target, (size_t)T * nb2, hipMemcpyHostToDevice, c->s));
+    // forward with checkpoints
+    std::vector<double> rq(T, 0.0);
+    double L = 0.0;
+    for (int s = 0; s < T; ++s) {
+        if (s % kc == 0) {
+            const int seg = s / kc;
+            HIPCHK(c, hipMemcpyAsync(a.ck_psi + seg * N, c->psi_n, nb3, hipMemcpyDeviceToDevice, c->s));
+            HIPCHK(c, hipMemcpyAsync(a.ck_h + seg * S, c->h, nb2, hipMemcpyDeviceToDevice, c->s));
+        }
+        dhara_step_stats st;
+        if ((rc = one_step(c, &st))) return rc;
+        const double Q = olf ? st.outlet_discharge : 0.0;
+        const double dq = Q - (spec->q_target ? spec->q_target[s] : 0.0);
+        L += spec->w_q * dq * dq;
+        rq[s] = 2.0 * spec->w_q * dq;
+        double lw;
+        if ((rc = colstore(c, spec->w_target ? a.wtarget + s * S : nullptr, spec->w_w, nullptr, a.rw + s * S, &lw)))
+            return rc;
+        L += lw;
+    }
+    if (loss) *loss = L;
+    long bicg_iters = 0, olf_iters = 0;
+    bool bicg_conv = true, olf_conv = true;
+    if (g_lnks || g_alpha || g_n) {
+        HIPCHK(c, hipMemcpyAsync(a.fin_psi, c->psi_n, nb3, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(c, hipMemcpyAsync(a.fin_h, c->h, nb2, hipMemcpyDeviceToDevice, c->s));
+        for (double* p : {a.psib, a.gK, a.ga, a.gn}) HIPCHK(c, hipMemsetAsync(p, 0, nb3, c->s));
+        HIPCHK(c, hipMemsetAsync(a.hb, 0, nb2, c->s));
+        const dh::Soil soil = soil_of(c);
+        const bool surf = c->p.top_bc == DHARA_TOP_SURFACE;
+        const int nseg = (T + kc - 1) / kc;
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            const int n0 = seg * kc, n1 = std::min(T, n0 + kc);
+            if ((rc = restore_state(c, a.ck_psi + seg * N, a.ck_h + seg * S))) return rc;
+            HIPCHK(c, hipMemcpyAsync(a.tp_psi, c->psi_n, nb3, hipMemcpyDeviceToDevice, c->s));
+            HIPCHK(c, hipMemcpyAsync(a.tp_h, c->h, nb2, hipMemcpyDeviceToDevice, c->s));
+            for (int s = n0; s < n1; ++s) {
+                dhara_step_stats st;
+                if ((rc = one_step(c, &st))) return rc;
+                const int l = s - n0 + 1;
+                HIPCHK(c, hipMemcpyAsync(a.tp_psi + l * N, c->psi_n, nb3, hipMemcpyDeviceToDevice, c->s));
+                HIPCHK(c, hipMemcpyAsync(a.tp_h + l * S, c->h, nb2, hipMemcpyDeviceToDevice, c->s));
+                HIPCHK(c, hipMemcpyAsync(a.tp_H + (l - 1) * S, c->H, nb2, hipMemcpyDeviceToDevice, c->s));
+            }
+            for (int s = n1 - 1; s >= n0; --s) {
+                const int l = s - n0;
+                const double* psin = a.tp_psi + l * N;
+                const double* hn = a.tp_h + l * S;
+                const double* psi1 = a.tp_psi + (l + 1) * N;
+                dh::launch_adj_vg(c->s, lin_args(c, psi1, psin));
+                dh::launch_adj_wmisfit(c->s, c->g, a.rw + s * S, a.C1, a.dtha1, a.dthn1, a.psib, a.ga, a.gn);
+                dh::launch_adj_surface_fwd(c->s, surf_adj(c, psin, hn));
+                LAUNCHCHK(c);
+                if (olf) {
+                    if ((rc = overland_reverse(c, a.tp_H + l * S, rq[s], tol, maxit, &olf_iters, &olf_conv)))
+                        return rc;
+                } else {
+                    HIPCHK(c, hipMemcpyAsync(a.hbp, a.hb, nb2, hipMemcpyDeviceToDevice, c->s));
+                }
+                dh::launch_adj_ponding(c->s, c->S, c->p.dt, a.hp, a.qq, a.hbp, a.hpb, a.qb);
+                dh::launch_adj_jdiag(c->s, c->g, psi1, a.K1, a.dK1, a.Ld, a.Jd);
+                LAUNCHCHK(c);
+                bool cv = true;
+                if ((rc = solve_jt(c, psi1, a.psib, a.lam, tol, maxit, &bicg_iters, &cv))) return rc;
+                bicg_conv = bicg_conv && cv;
+                dh::ParamArgs pa{};
+                pa.g = c->g;
+                pa.dt = c->p.dt;
+                pa.Ss = c->p.Ss;
+                pa.phi = c->p.porosity;
+                pa.lam = a.lam; pa.psi1 = psi1; pa.psin = psin; pa.th1 = a.th1; pa.K1 = a.K1;
+                pa.dtha1 = a.dtha1; pa.dthn1 = a.dthn1; pa.dKa1 = a.dKa1; pa.dKn1 = a.dKn1;
+                pa.gK = a.gK; pa.ga = a.ga; pa.gn = a.gn; pa.psinb = a.psinb; pa.thnb = a.thnb;
+                dh::launch_adj_param(c->s, pa);
+                dh::SurfRevArgs sr{};
+                sr.nxy = c->S;
+                sr.soil = soil;
+                sr.dt = c->p.dt;
+                sr.dz = c->grid.dz;
+                sr.surface_mode = surf ? 1 : 0;
+                sr.psin = psin; sr.hp = a.hp; sr.lam = a.lam; sr.qb = a.qb; sr.hbp = a.hbp; sr.br = a.qbr;
+                sr.hpb = a.hpb; sr.psinb = a.psinb; sr.thnb = a.thnb; sr.gK = a.gK; sr.ga = a.ga; sr.gn = a.gn;
+                sr.hnb = a.hnb;
+                dh::launch_adj_surface_rev(c->s, sr);
+                dh::launch_adj_theta_n(c->s, c->g, soil, psin, a.thnb, a.psinb, a.ga, a.gn);
+                LAUNCHCHK(c);
+                std::swap(a.psib, a.psinb);
+                std::swap(a.hb, a.hnb);
+            }
+        }
+        if ((rc = restore_state(c, a.fin_psi, a.fin_h))) return rc;
+        if (g_lnks) HIPCHK(c, hipMemcpyAsync(g_lnks, a.gK, nb3, hipMemcpyDeviceToHost, c->s));
+        if (g_alpha) HIPCHK(c, hipMemcpyAsync(g_alpha, a.ga, nb3, hipMemcpyDeviceToHost, c->s));
+        if (g_n) HIPCHK(c, hipMemcpyAsync(g_n, a.gn, nb3, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->lin_iters = (int32_t)bicg_iters;
+        stats->olf_iters = (int32_t)olf_iters;
+        stats->lin_converged = bicg_conv ? 1 : 0;
+        stats->olf_converged = olf_conv ? 1 : 0;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return DHARA_OK;
+}
+
+}  // extern "C"

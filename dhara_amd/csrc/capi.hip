@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "../../include/dhara_hip.h"
-#include "kernels.h"
+#include "ctx.h"
 
 using dh::FB;
 using dh::FE;
@@ -27,59 +27,8 @@ using dh::FS;
 using dh::FT;
 using dh::FW;
 
-struct dhara_ctx {
-    dhara_grid grid{};
-    dhara_params p{};
-    int dev = 0;
-    hipStream_t s = nullptr;
-    std::string err;
-    int N = 0, S = 0;
-    dh::Geo g{};
-    dh::Geo2 g2{};
-
-    // 3D FP64 state
-    double *psi_n = nullptr, *psi_m = nullptr, *x = nullptr;
-    double *theta_n = nullptr, *theta_m = nullptr, *theta_new = nullptr;
-    double *K = nullptr, *C = nullptr, *Ks = nullptr, *alpha_f = nullptr, *n_f = nullptr;
-    double *D = nullptr, *b = nullptr, *r = nullptr, *pv[2] = {nullptr, nullptr}, *q = nullptr;
-    // 3D FP32 solver arrays (mixed precision)
-    float *K32 = nullptr, *D32 = nullptr, *r32 = nullptr, *d32 = nullptr;
-    float *p32[2] = {nullptr, nullptr}, *q32 = nullptr;
-    uint8_t* flags = nullptr;
-    // surface
-    double *z = nullptr, *mann = nullptr, *h = nullptr, *H = nullptr;
-    double *rain = nullptr, *evap = nullptr, *transp = nullptr, *hpoten = nullptr, *qcapa = nullptr;
-    double *bcpsi[6] = {}, *bcq[6] = {};
-    int* bctype[6] = {};
-    int* procmap = nullptr;
-    double* root = nullptr;
-    // overland solver
-    double *D2 = nullptr, *b2 = nullptr, *r2 = nullptr, *p2[2] = {nullptr, nullptr}, *q2 = nullptr;
-    double *ke = nullptr, *kn = nullptr;
-    // reductions: partial buffers A (written by assembly/update kernels) and B (SpMV kernels)
-    double* part[2] = {nullptr, nullptr};
-    dh::PcgState* st_sub = nullptr;
-    dh::PcgState* st_olf = nullptr;
-    // pinned host mirrors
-    dh::PcgState* h_st = nullptr;
-    double* h_part = nullptr;
-
-    bool face_dirichlet[6] = {false, false, false, false, false, false};
-    bool has_alpha = false, has_n = false, has_transp = false;
-    bool initialized = false;
-    bool theta_m_current = true;
-    int pcg_hint = 8, olf_hint = 8;
-    std::vector<void*> allocs;
-    // SpMV tile shapes: [0] FP64 solves, [1] FP32 inner solves (tx = 0: untiled kernel)
-    dh::TileCfg tile[2] = {{32, 8, 0}, {32, 8, 0}};
-    // kernel profiling (dhara_profile_enable)
-    bool prof = false;
-    std::vector<hipEvent_t> ev;
-    dhara_kernel_profile prof_acc{};
-};
 
 namespace {
-constexpr int kMaxBatch = 64;   // Krylov iterations queued between two host checks
 
 struct TuneKey {
     int nx, ny, nz, elem;
@@ -103,41 +52,12 @@ dh::TileCfg default_tile(const dh::Geo& g)
 }
 }
 
-namespace {
+namespace dhc {
 
 int set_err(dhara_ctx* c, int code, const std::string& msg)
 {
     if (c) c->err = msg;
     return code;
-}
-
-#define HIPCHK(ctx, call)                                                                   \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return set_err((ctx), DHARA_E_DEVICE,                                           \
-                           std::string(#call) + ": " + hipGetErrorString(e_));              \
-    } while (0)
-
-#define LAUNCHCHK(ctx)                                                                      \
-    do {                                                                                    \
-        hipError_t e_ = hipGetLastError();                                                  \
-        if (e_ != hipSuccess)                                                               \
-            return set_err((ctx), DHARA_E_DEVICE,                                           \
-                           std::string("kernel launch: ") + hipGetErrorString(e_));         \
-    } while (0)
-
-template <typename T>
-int dalloc(dhara_ctx* c, T** ptr, size_t count)
-{
-    void* v = nullptr;
-    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
-    if (e != hipSuccess) return set_err(c, DHARA_E_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemsetAsync(v, 0, std::max<size_t>(count, 1) * sizeof(T), c->s);
-    if (e != hipSuccess) return set_err(c, DHARA_E_DEVICE, "hipMemset");
-    c->allocs.push_back(v);
-    *ptr = static_cast<T*>(v);
-    return DHARA_OK;
 }
 
 size_t face_size(const dhara_grid& g, int f)
@@ -335,56 +255,6 @@ dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bo
     return sc;
 }
 
-// Queue Krylov iterations in batches; the kernels stop doing work once the device-side
-// state says the solve is done, so over-queued iterations cost one launch each.  The SpMV
-// kernel of iteration `it` reads partials A and writes B, the update kernel reads B and
-// writes A.  With profiling on, events bracket each SpMV and update launch of the batch and
-// only the launches that did work (the first ones of the batch) are accumulated.
-template <class SpmvFn, class UpdFn>
-int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
-               SpmvFn spmv, UpdFn upd, int it_start = 0)
-{
-    int it = it_start;
-    // first batch: the previous solve's count (the next one is usually within +-2 of it);
-    // then small batches, since an over-queued iteration costs two empty launches and an
-    // under-queued one a host round trip
-    int batch = std::max(2, std::min(kMaxBatch, hint));
-    for (;;) {
-        const int it0 = it;
-        for (int b = 0; b < batch; ++b, ++it) {
-            if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b], c->s));
-            spmv(it);
-            if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b + 1], c->s));
-            upd(it);
-            if (prof) HIPCHK(c, hipEventRecord(c->ev[3 * b + 2], c->s));
-        }
-        LAUNCHCHK(c);
-        int rc = read_state(c, st);
-        if (rc) return rc;
-        const int done_iters = c->h_st->done ? c->h_st->iter : it;
-        if (prof) {
-            const int real = std::max(0, std::min(batch, done_iters - it0));
-            for (int b = 0; b < real; ++b) {
-                float m0 = 0.f, m1 = 0.f;
-                HIPCHK(c, hipEventElapsedTime(&m0, c->ev[3 * b], c->ev[3 * b + 1]));
-                HIPCHK(c, hipEventElapsedTime(&m1, c->ev[3 * b + 1], c->ev[3 * b + 2]));
-                c->prof_acc.spmv_ms += m0;
-                c->prof_acc.update_ms += m1;
-            }
-            c->prof_acc.spmv_launches += real;
-            c->prof_acc.update_launches += real;
-            c->prof_acc.spmv_bytes = 6.0 * elem * c->N;
-            c->prof_acc.update_bytes = 7.0 * elem * c->N;
-            c->prof_acc.elem_bytes = (int)elem;
-        }
-        if (c->h_st->done) break;
-        if (it > maxit + 1) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
-        batch = 3;
-    }
-    hint = std::max(1, c->h_st->iter - it_start);
-    return DHARA_OK;
-}
-
 int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
 {
     const bool dir = has_dirichlet(c);
@@ -397,19 +267,19 @@ int solve_fp64(dhara_ctx* c, dhara_step_stats* st)
     const int spmv_parts = tc.tx ? dh::tile_blocks(c->g, tc) : dh::grid_for(c->N);
     int rc = run_krylov(
         c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, c->prof, 8.0,
-        [&](int it) {
+        [&](int it, const dh::LaunchEv& ev) {
             dh::SolveCtl k = sc;
             k.it = it;
-            if (tc.tx) dh::launch_pcg3_spmv_tiled_f64(c->s, c->g, tc, fl, c->K, c->D, c->r, pp, c->q, k);
-            else dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, k);
+            if (tc.tx) dh::launch_pcg3_spmv_tiled_f64(c->s, c->g, tc, fl, c->K, c->D, c->r, pp, c->q, k, ev);
+            else dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, k, ev);
         },
-        [&](int it) {
+        [&](int it, const dh::LaunchEv& ev) {
             dh::SolveCtl k = sc;
             k.it = it;
             k.part_in = c->part[1];
             k.nparts = spmv_parts;
             k.part_out = c->part[0];
-            dh::launch_pcg_update_f64(c->s, c->N, c->x, c->r, c->D, pp[(it + 1) & 1], c->q, k);
+            dh::launch_pcg_update_f64(c->s, c->N, c->x, c->r, c->D, pp[(it + 1) & 1], c->q, k, ev);
         });
     if (rc) return rc;
     st->lin_iters += c->h_st->iter;
@@ -453,20 +323,20 @@ int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
     for (;;) {
         int rc = run_krylov(
             c, c->st_sub, maxit, c->pcg_hint, c->prof, 4.0,
-            [&](int k_it) {
+            [&](int k_it, const dh::LaunchEv& ev) {
                 dh::SolveCtl k = sc;
                 k.it = k_it;
                 k.segment_start = (k_it == it) ? 1 : 0;
-                if (tc.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, tc, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
-                else dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+                if (tc.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, tc, fl, c->K32, c->D32, c->r32, pp, c->q32, k, ev);
+                else dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k, ev);
             },
-            [&](int k_it) {
+            [&](int k_it, const dh::LaunchEv& ev) {
                 dh::SolveCtl k = sc;
                 k.it = k_it;
                 k.part_in = c->part[1];
                 k.nparts = spmv_parts;
                 k.part_out = c->part[0];
-                dh::launch_pcg_update_f32(c->s, c->N, c->d32, c->r32, c->D32, pp[(k_it + 1) & 1], c->q32, k);
+                dh::launch_pcg_update_f32(c->s, c->N, c->d32, c->r32, c->D32, pp[(k_it + 1) & 1], c->q32, k, ev);
             },
             it);
         if (rc) return rc;
@@ -612,12 +482,12 @@ int do_overland_step(dhara_ctx* c, dhara_step_stats* st)
     sc.lin_tol = c->p.olf_tol;
     int rc = run_krylov(
         c, c->st_olf, c->p.olf_maxiter, c->olf_hint, false, 8.0,
-        [&](int it) {
+        [&](int it, const dh::LaunchEv& ev) {
             dh::SolveCtl k = sc;
             k.it = it;
             dh::launch_pcg2_spmv(c->s, c->g2, c->ke, c->kn, c->D2, c->r2, pp, c->q2, k);
         },
-        [&](int it) {
+        [&](int it, const dh::LaunchEv& ev) {
             dh::SolveCtl k = sc;
             k.it = it;
             k.part_in = c->part[1];
@@ -693,7 +563,9 @@ void after_upload(dhara_ctx* c, int f)
         c->initialized = false;
 }
 
-}  // namespace
+}  // namespace dhc
+
+using namespace dhc;
 
 // =========================================================================================
 extern "C" {
@@ -801,6 +673,7 @@ int dhara_destroy(dhara_ctx* c)
 {
     if (!c) return DHARA_OK;
     if (c->s) (void)hipStreamSynchronize(c->s);
+    free_adjoint(c);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void* v : c->allocs) (void)hipFree(v);
     if (c->h_st) (void)hipHostFree(c->h_st);
@@ -1157,7 +1030,7 @@ int dhara_profile_enable(dhara_ctx* c, int32_t on)
 {
     if (!c) return DHARA_E_ARG;
     if (on && c->ev.empty()) {
-        c->ev.resize(3 * kMaxBatch);
+        c->ev.resize(4 * kMaxBatch);
         for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
     }
     c->prof = on != 0;

@@ -1,0 +1,168 @@
+// ctx.h -- the context object behind the C-ABI handle and the host-side helpers shared by
+// the forward solver (capi.hip) and the discrete adjoint (adjoint.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/dhara_hip.h"
+#include "kernels.h"
+
+struct dhara_adjoint;   // reverse-sweep buffers, allocated on the first adjoint call
+
+// One context = one HIP stream + every device buffer of one model.
+struct dhara_ctx {
+    dhara_grid grid{};
+    dhara_params p{};
+    int dev = 0;
+    hipStream_t s = nullptr;
+    std::string err;
+    int N = 0, S = 0;
+    dh::Geo g{};
+    dh::Geo2 g2{};
+
+    // 3D FP64 state
+    double *psi_n = nullptr, *psi_m = nullptr, *x = nullptr;
+    double *theta_n = nullptr, *theta_m = nullptr, *theta_new = nullptr;
+    double *K = nullptr, *C = nullptr, *Ks = nullptr, *alpha_f = nullptr, *n_f = nullptr;
+    double *D = nullptr, *b = nullptr, *r = nullptr, *pv[2] = {nullptr, nullptr}, *q = nullptr;
+    // 3D FP32 solver arrays (mixed precision)
+    float *K32 = nullptr, *D32 = nullptr, *r32 = nullptr, *d32 = nullptr;
+    float *p32[2] = {nullptr, nullptr}, *q32 = nullptr;
+    uint8_t* flags = nullptr;
+    // surface
+    double *z = nullptr, *mann = nullptr, *h = nullptr, *H = nullptr;
+    double *rain = nullptr, *evap = nullptr, *transp = nullptr, *hpoten = nullptr, *qcapa = nullptr;
+    double *bcpsi[6] = {}, *bcq[6] = {};
+    int* bctype[6] = {};
+    int* procmap = nullptr;
+    double* root = nullptr;
+    // overland solver
+    double *D2 = nullptr, *b2 = nullptr, *r2 = nullptr, *p2[2] = {nullptr, nullptr}, *q2 = nullptr;
+    double *ke = nullptr, *kn = nullptr;
+    // reductions: partial buffers A (written by assembly/update kernels) and B (SpMV kernels)
+    double* part[2] = {nullptr, nullptr};
+    dh::PcgState* st_sub = nullptr;
+    dh::PcgState* st_olf = nullptr;
+    // pinned host mirrors
+    dh::PcgState* h_st = nullptr;
+    double* h_part = nullptr;
+
+    bool face_dirichlet[6] = {false, false, false, false, false, false};
+    bool has_alpha = false, has_n = false, has_transp = false;
+    bool initialized = false;
+    bool theta_m_current = true;
+    int pcg_hint = 8, olf_hint = 8;
+    std::vector<void*> allocs;
+    // SpMV tile shapes: [0] FP64 solves, [1] FP32 inner solves (tx = 0: untiled kernel)
+    dh::TileCfg tile[2] = {{32, 8, 0}, {32, 8, 0}};
+    // kernel profiling (dhara_profile_enable)
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    dhara_kernel_profile prof_acc{};
+    dhara_adjoint* adj = nullptr;
+};
+
+namespace dhc {
+
+constexpr int kMaxBatch = 64;   // Krylov iterations queued between two host checks
+
+int set_err(dhara_ctx* c, int code, const std::string& msg);
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err((ctx), DHARA_E_DEVICE,                                           \
+                           std::string(#call) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+#define LAUNCHCHK(ctx)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess)                                                               \
+            return set_err((ctx), DHARA_E_DEVICE,                                           \
+                           std::string("kernel launch: ") + hipGetErrorString(e_));         \
+    } while (0)
+
+template <typename T>
+int dalloc(dhara_ctx* c, T** ptr, size_t count)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return set_err(c, DHARA_E_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(v, 0, std::max<size_t>(count, 1) * sizeof(T), c->s);
+    if (e != hipSuccess) return set_err(c, DHARA_E_DEVICE, "hipMemset");
+    c->allocs.push_back(v);
+    *ptr = static_cast<T*>(v);
+    return DHARA_OK;
+}
+
+int host_reduce(dhara_ctx* c, int nparts, int nv, bool take_max, double* out);
+int read_state(dhara_ctx* c, dh::PcgState* dev);
+bool has_dirichlet(const dhara_ctx* c);
+dh::Soil soil_of(const dhara_ctx* c);
+dh::VgArgs vg_args(dhara_ctx* c, const double* psi, double* theta);
+dh::OlfArgs olf_args(dhara_ctx* c);
+dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bool mixed_inner,
+                       bool first_sweep);
+int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st);
+int do_overland_step(dhara_ctx* c, dhara_step_stats* st);
+void free_adjoint(dhara_ctx* c);   // adjoint.hip
+
+// Queue Krylov iterations in batches; the kernels stop doing work once the device-side
+// state says the solve is done, so over-queued iterations cost one launch each.  The SpMV
+// kernel of iteration `it` reads partials A and writes B, the update kernel reads B and
+// writes A.  With profiling on, events bracket each SpMV and update launch of the batch and
+// only the launches that did work (the first ones of the batch) are accumulated.
+template <class SpmvFn, class UpdFn>
+int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
+               SpmvFn spmv, UpdFn upd, int it_start = 0)
+{
+    int it = it_start;
+    // first batch: the previous solve's count (the next one is usually within +-2 of it);
+    // then small batches, since an over-queued iteration costs two empty launches and an
+    // under-queued one a host round trip
+    int batch = std::max(2, std::min(kMaxBatch, hint));
+    for (;;) {
+        const int it0 = it;
+        for (int b = 0; b < batch; ++b, ++it) {
+            dh::LaunchEv e_spmv, e_upd;
+            if (prof) {
+                e_spmv = dh::LaunchEv{c->ev[4 * b], c->ev[4 * b + 1]};
+                e_upd = dh::LaunchEv{c->ev[4 * b + 2], c->ev[4 * b + 3]};
+            }
+            spmv(it, e_spmv);
+            upd(it, e_upd);
+        }
+        LAUNCHCHK(c);
+        int rc = read_state(c, st);
+        if (rc) return rc;
+        const int done_iters = c->h_st->done ? c->h_st->iter : it;
+        if (prof) {
+            const int real = std::max(0, std::min(batch, done_iters - it0));
+            for (int b = 0; b < real; ++b) {
+                float m0 = 0.f, m1 = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&m0, c->ev[4 * b], c->ev[4 * b + 1]));
+                HIPCHK(c, hipEventElapsedTime(&m1, c->ev[4 * b + 2], c->ev[4 * b + 3]));
+                c->prof_acc.spmv_ms += m0;
+                c->prof_acc.update_ms += m1;
+            }
+            c->prof_acc.spmv_launches += real;
+            c->prof_acc.update_launches += real;
+            c->prof_acc.spmv_bytes = 6.0 * elem * c->N;
+            c->prof_acc.update_bytes = 7.0 * elem * c->N;
+            c->prof_acc.elem_bytes = (int)elem;
+        }
+        if (c->h_st->done) break;
+        if (it > maxit + 1) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
+        batch = 3;
+    }
+    hint = std::max(1, c->h_st->iter - it_start);
+    return DHARA_OK;
+}
+
+}  // namespace dhc

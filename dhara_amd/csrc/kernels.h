@@ -146,6 +146,13 @@ struct OlfArgs {
 int grid_for(int n);
 
 // ---- launchers (all asynchronous on `s`) ----------------------------------------------
+// Optional kernel timestamps: with `start` set, a Krylov launch goes through
+// hipExtLaunchKernelGGL, which stamps the two events with the kernel's own begin and end
+// (what a kernel trace reports), rather than with markers queued around the launch.
+struct LaunchEv {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
 void launch_vg(hipStream_t s, const VgArgs& a);                 // theta, K, C of psi
 void launch_vg_picard(hipStream_t s, const VgArgs& a);          // + increment + relax
 void launch_vg_inverse(hipStream_t s, const Geo& g, const Soil& soil, const double* theta,
@@ -163,14 +170,14 @@ void launch_end_step(hipStream_t s, const EndArgs& a);
 //   update: x += alpha p, r -= alpha q, partials (r, M^-1 r), (r, r)
 void launch_pcg3_spmv_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                           const double* D, const double* r, double* const p[2], double* q,
-                          const SolveCtl& sc);
+                          const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 void launch_pcg3_spmv_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
                           const float* D, const float* r, float* const p[2], float* q,
-                          const SolveCtl& sc);
+                          const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 void launch_pcg_update_f64(hipStream_t s, int n, double* x, double* r, const double* D,
-                           const double* p, const double* q, const SolveCtl& sc);
+                           const double* p, const double* q, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float* D,
-                           const float* p, const float* q, const SolveCtl& sc);
+                           const float* p, const float* q, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 // LDS-tiled 2.5D variant of the SpMV kernel: a TX x TY tile of columns marches through a
 // chunk of kc planes; the p/K planes are staged in LDS with a one-cell halo, the vertical
 // neighbours stay in registers.  Returns the number of blocks (= partials for the update).
@@ -181,10 +188,10 @@ bool tile_supported(const TileCfg& t);
 int tile_blocks(const Geo& g, const TileCfg& t);
 void launch_pcg3_spmv_tiled_f64(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
                                 const double* K, const double* D, const double* r, double* const p[2],
-                                double* q, const SolveCtl& sc);
+                                double* q, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 void launch_pcg3_spmv_tiled_f32(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
                                 const float* K, const float* D, const float* r, float* const p[2],
-                                float* q, const SolveCtl& sc);
+                                float* q, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 // y = A x (tests)
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                        const double* D, const double* x, double* y);

@@ -6,6 +6,7 @@
 // them in block order (device_common.h).  The Krylov kernels read PcgState::done at entry and
 // return at once when the solve has finished, so the host can queue iterations in batches
 // without a round trip per iteration.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -881,48 +882,56 @@ void launch_end_step(hipStream_t s, const EndArgs& a)
     hipLaunchKernelGGL(k_end_step, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
 }
 
+// Krylov launches: through hipExtLaunchKernelGGL so that profiling events, when given, carry
+// the kernel's own begin/end timestamps.
+template <typename F, typename... Args>
+static void launch_ev(F kernel, dim3 gr, dim3 bl, hipStream_t s, const LaunchEv& ev, Args... args)
+{
+    hipExtLaunchKernelGGL(kernel, gr, bl, 0, s, ev.start, ev.stop, 0, args...);
+}
+
 template <typename T>
 static void pcg3_spmv(hipStream_t s, const Geo& g, const uint8_t* flags, const T* K, const T* D,
-                      const T* r, T* const p[2], T* q, const SolveCtl& sc)
+                      const T* r, T* const p[2], T* q, const SolveCtl& sc, const LaunchEv& ev)
 {
     const dim3 gr(grid_for(g.n)), bl(kBlock);
     const T* pold = p[sc.it & 1];
     T* pnew = p[(sc.it + 1) & 1];
-    if (flags) hipLaunchKernelGGL((k_pcg3_spmv<T, true>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, sc);
-    else hipLaunchKernelGGL((k_pcg3_spmv<T, false>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q, sc);
+    if (flags) launch_ev(k_pcg3_spmv<T, true>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q, sc);
+    else launch_ev(k_pcg3_spmv<T, false>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q, sc);
 }
 
 template <typename T>
 static void pcg_update(hipStream_t s, int n, T* x, T* r, const T* D, const T* p, const T* q,
-                       const SolveCtl& sc)
+                       const SolveCtl& sc, const LaunchEv& ev)
 {
-    hipLaunchKernelGGL((k_pcg_update<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, D, p, q, sc);
+    launch_ev(k_pcg_update<T>, dim3(grid_for(n)), dim3(kBlock), s, ev, n, x, r, D, p, q, sc);
 }
 
 void launch_pcg3_spmv_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                           const double* D, const double* r, double* const p[2], double* q,
-                          const SolveCtl& sc)
+                          const SolveCtl& sc, const LaunchEv& ev)
 {
-    pcg3_spmv<double>(s, g, flags, K, D, r, p, q, sc);
+    pcg3_spmv<double>(s, g, flags, K, D, r, p, q, sc, ev);
 }
 
 void launch_pcg3_spmv_f32(hipStream_t s, const Geo& g, const uint8_t* flags, const float* K,
                           const float* D, const float* r, float* const p[2], float* q,
-                          const SolveCtl& sc)
+                          const SolveCtl& sc, const LaunchEv& ev)
 {
-    pcg3_spmv<float>(s, g, flags, K, D, r, p, q, sc);
+    pcg3_spmv<float>(s, g, flags, K, D, r, p, q, sc, ev);
 }
 
 void launch_pcg_update_f64(hipStream_t s, int n, double* x, double* r, const double* D,
-                           const double* p, const double* q, const SolveCtl& sc)
+                           const double* p, const double* q, const SolveCtl& sc, const LaunchEv& ev)
 {
-    pcg_update<double>(s, n, x, r, D, p, q, sc);
+    pcg_update<double>(s, n, x, r, D, p, q, sc, ev);
 }
 
 void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float* D,
-                           const float* p, const float* q, const SolveCtl& sc)
+                           const float* p, const float* q, const SolveCtl& sc, const LaunchEv& ev)
 {
-    pcg_update<float>(s, n, x, r, D, p, q, sc);
+    pcg_update<float>(s, n, x, r, D, p, q, sc, ev);
 }
 
 bool tile_supported(const TileCfg& t)
@@ -940,43 +949,43 @@ int tile_blocks(const Geo& g, const TileCfg& t)
 
 template <typename T, int TX, int TY>
 static void tiled_launch(hipStream_t s, const Geo& g, int kc, const uint8_t* flags, const T* K, const T* D,
-                         const T* r, const T* pold, T* pnew, T* q, const SolveCtl& sc)
+                         const T* r, const T* pold, T* pnew, T* q, const SolveCtl& sc, const LaunchEv& ev)
 {
     const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, nkz = (g.nz + kc - 1) / kc;
     const dim3 gr(ntx * nty * nkz), bl(TX, TY);
     if (flags)
-        hipLaunchKernelGGL((k_pcg3_spmv_tiled<T, true, TX, TY>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q,
-                           sc, kc, ntx, nty, nkz);
+        launch_ev(k_pcg3_spmv_tiled<T, true, TX, TY>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q, sc, kc,
+                  ntx, nty, nkz);
     else
-        hipLaunchKernelGGL((k_pcg3_spmv_tiled<T, false, TX, TY>), gr, bl, 0, s, g, flags, K, D, r, pold, pnew, q,
-                           sc, kc, ntx, nty, nkz);
+        launch_ev(k_pcg3_spmv_tiled<T, false, TX, TY>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q, sc, kc,
+                  ntx, nty, nkz);
 }
 
 template <typename T>
 static void tiled_dispatch(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags, const T* K,
-                           const T* D, const T* r, T* const p[2], T* q, const SolveCtl& sc)
+                           const T* D, const T* r, T* const p[2], T* q, const SolveCtl& sc, const LaunchEv& ev)
 {
     const T* pold = p[sc.it & 1];
     T* pnew = p[(sc.it + 1) & 1];
-    if (t.tx == 32 && t.ty == 8) tiled_launch<T, 32, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
-    else if (t.tx == 64 && t.ty == 4) tiled_launch<T, 64, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
-    else if (t.tx == 16 && t.ty == 16) tiled_launch<T, 16, 16>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
-    else if (t.tx == 32 && t.ty == 4) tiled_launch<T, 32, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
-    else tiled_launch<T, 64, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc);
+    if (t.tx == 32 && t.ty == 8) tiled_launch<T, 32, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else if (t.tx == 64 && t.ty == 4) tiled_launch<T, 64, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else if (t.tx == 16 && t.ty == 16) tiled_launch<T, 16, 16>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else if (t.tx == 32 && t.ty == 4) tiled_launch<T, 32, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else tiled_launch<T, 64, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
 }
 
 void launch_pcg3_spmv_tiled_f64(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
                                 const double* K, const double* D, const double* r, double* const p[2],
-                                double* q, const SolveCtl& sc)
+                                double* q, const SolveCtl& sc, const LaunchEv& ev)
 {
-    tiled_dispatch<double>(s, g, t, flags, K, D, r, p, q, sc);
+    tiled_dispatch<double>(s, g, t, flags, K, D, r, p, q, sc, ev);
 }
 
 void launch_pcg3_spmv_tiled_f32(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
                                 const float* K, const float* D, const float* r, float* const p[2],
-                                float* q, const SolveCtl& sc)
+                                float* q, const SolveCtl& sc, const LaunchEv& ev)
 {
-    tiled_dispatch<float>(s, g, t, flags, K, D, r, p, q, sc);
+    tiled_dispatch<float>(s, g, t, flags, K, D, r, p, q, sc, ev);
 }
 
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,

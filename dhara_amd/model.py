@@ -128,6 +128,61 @@ class BaseModel:
         self._check(self._fn("step")(self._h, nsteps, arr), "step")
         return [arr[i].as_dict() for i in range(nsteps)]
 
+    # ---- discrete adjoint -------------------------------------------------------------
+    def observe(self, nsteps):
+        """Run nsteps steps; returns the outlet discharge q[t] and column storage w[t, col]."""
+        s = self.grid.nx * self.grid.ny
+        q, w = np.empty(nsteps), np.empty((nsteps, s))
+        self._check(self._fn("observe")(self._h, nsteps, _dptr(q), _dptr(w)), "observe")
+        return q, w
+
+    def adjoint_run(self, nsteps, checkpoint_every=10, w_q=1.0, w_w=1.0, q_target=None,
+                    w_target=None, adj_tol=1e-12, adj_maxiter=5000, gradients=True):
+        """Loss over a forward run of nsteps and, when gradients is true, its gradient with
+        respect to per-cell ln Ks, alpha and n.  Returns (loss, g_lnks, g_alpha, g_n, stats)."""
+        n = self.grid.nx * self.grid.ny * self.grid.nz
+        s = self.grid.nx * self.grid.ny
+        spec = _abi.LossSpec(nsteps=nsteps, checkpoint_every=checkpoint_every, w_q=w_q, w_w=w_w,
+                             adj_tol=adj_tol, adj_maxiter=adj_maxiter)
+        keep = []
+        if q_target is not None:
+            qt = np.ascontiguousarray(q_target, dtype=np.float64).ravel()
+            if qt.size != nsteps:
+                raise ValueError("q_target needs nsteps values")
+            keep.append(qt)
+            spec.q_target = _dptr(qt)
+        if w_target is not None:
+            wt = np.ascontiguousarray(w_target, dtype=np.float64).ravel()
+            if wt.size != nsteps * s:
+                raise ValueError("w_target needs nsteps * nx * ny values")
+            keep.append(wt)
+            spec.w_target = _dptr(wt)
+        loss = C.c_double()
+        st = _abi.StepStats()
+        if gradients:
+            g = [np.empty(n) for _ in range(3)]
+            ptrs = [_dptr(a) for a in g]
+        else:
+            g = [None, None, None]
+            ptrs = [None, None, None]
+        self._check(self._fn("adjoint_run")(self._h, C.byref(spec), C.byref(loss), *ptrs, C.byref(st)),
+                    "adjoint_run")
+        return loss.value, g[0], g[1], g[2], st.as_dict()
+
+    def adjoint_residual(self, psi_new):
+        x = np.ascontiguousarray(psi_new, dtype=np.float64)
+        out = np.empty_like(x)
+        self._check(self._fn("adjoint_residual")(self._h, _dptr(x), _dptr(out)), "adjoint_residual")
+        return out
+
+    def adjoint_jvp(self, psi_new, v, transpose=False):
+        x = np.ascontiguousarray(psi_new, dtype=np.float64)
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty_like(x)
+        self._check(self._fn("adjoint_jvp")(self._h, _dptr(x), 1 if transpose else 0, _dptr(vv), _dptr(out)),
+                    "adjoint_jvp")
+        return out
+
     # ---- kernel-level ---------------------------------------------------------------------
     def eval_constitutive(self, psi):
         psi = np.ascontiguousarray(psi, dtype=np.float64)

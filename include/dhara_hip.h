@@ -288,6 +288,50 @@ int dhara_apply_subsurface_operator_f32(dhara_ctx* ctx, const double* x, double*
  * from the non-ring cell of the face). */
 int dhara_overland_conductance(dhara_ctx* ctx, double* ke, double* kn);
 
+/* ---- discrete adjoint ------------------------------------------------------------------
+ * No reference counterpart (the reference has no sensitivities).  The adjoint is that of the
+ * converged implicit step: each step solves F(psi^{n+1}; psi^n, h^n, p) = 0 (the mixed-form
+ * Richards residual whose modified-Picard fixed point the forward iterates to), followed by
+ * the ponding update and the semi-implicit overland solve of that step.  The reverse sweep
+ * solves J^T lambda = dL/dpsi^{n+1} with J = dF/dpsi^{n+1} (BiCGStab on the transposed
+ * 7-point Jacobian stencil) and A2^T mu = dL/dH^{n+1} for the overland system, and recomputes
+ * the forward states of every checkpoint interval from its checkpoint.
+ * Requirements: FP64 precision, am = 0, DHARA_THETA_CARRY_CONSISTENT, no Dirichlet faces.
+ * The forcing fields in place at the call stay fixed for the whole run. */
+typedef struct dhara_loss_spec {
+    int32_t nsteps;           /* steps run from the current state                          */
+    int32_t checkpoint_every; /* a forward checkpoint (psi, ponding) every k steps, k >= 1  */
+    double w_q;               /* weight of sum_t (Q_t - Q*_t)^2, Q = outlet discharge
+                                 across the x = 0 edge after step t [m^3/s]               */
+    double w_w;               /* weight of sum_t sum_cols (W_t - W*_t)^2, W = column storage
+                                 sum_k theta dz after step t [m]                           */
+    const double* q_target;   /* nsteps host doubles, or NULL for zeros                    */
+    const double* w_target;   /* nsteps * nx*ny host doubles (step-major), or NULL         */
+    double adj_tol;           /* relative tolerance of the reverse linear solves           */
+    int32_t adj_maxiter;      /* iteration limit of each reverse linear solve              */
+    int32_t reserved;
+} dhara_loss_spec;
+
+/* Runs nsteps steps from the current state and records, after every step t, the outlet
+ * discharge q[t] (may be NULL) and the column storage w[t*nx*ny + col] (may be NULL). */
+int dhara_observe(dhara_ctx* ctx, int32_t nsteps, double* q, double* w);
+/* Forward run of spec->nsteps steps with checkpoints, the loss, and (when any gradient
+ * pointer is non-NULL) the reverse sweep.  Gradients are with respect to per-cell ln Ks,
+ * alpha [1/m] and n (nx*ny*nz host doubles each; NULL skips one).  On return the state is
+ * the one after the forward run.  stats (may be NULL) gets the reverse-solve totals:
+ * lin_iters = BiCGStab iterations, olf_iters = overland CG iterations, lin_converged and
+ * olf_converged = every reverse solve converged, wall_ms. */
+int dhara_adjoint_run(dhara_ctx* ctx, const dhara_loss_spec* spec, double* loss,
+                      double* grad_lnks, double* grad_alpha, double* grad_n,
+                      dhara_step_stats* stats);
+/* Residual F(psi_new) of the implicit subsurface step that starts from the current state
+ * (top flux from the current ponding depth), nx*ny*nz host doubles; F = 0 at the converged
+ * step.  Units: 1/s (volume balance per cell volume). */
+int dhara_adjoint_residual(dhara_ctx* ctx, const double* psi_new, double* F);
+/* out = J v (transpose = 0) or J^T v (transpose = 1), J = dF/dpsi_new at psi_new. */
+int dhara_adjoint_jvp(dhara_ctx* ctx, const double* psi_new, int32_t transpose, const double* v,
+                      double* out);
+
 #ifdef __cplusplus
 }
 #endif

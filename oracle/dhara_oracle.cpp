@@ -25,7 +25,7 @@
  *                                against ring outflow, hand-computed K on a 3x3 stencil.
  * Functions without such a check: none.
  */
-#include "dhara_oracle.h"
+#include "oracle_ctx.h"
 
 #include <algorithm>
 #include <cmath>
@@ -37,36 +37,7 @@
 #include <omp.h>
 #endif
 
-namespace {
-
-enum Face { FW = 0, FE, FS, FN, FT, FB };
-
-struct Soil {
-    double ts, tr, alpha, n, Ss, phi;
-};
-
-}  // namespace
-
-struct dor_ctx {
-    dhara_grid g;
-    dhara_params p;
-    int lin_solver, olf_solver;
-    size_t N, S;
-    bool initialized = false;
-    std::string err;
-
-    std::vector<double> psi_n, theta_n, ksat, alpha_f, n_f;
-    bool has_alpha = false, has_n = false;
-    std::vector<double> K, C, theta_m, theta_new, psi_m, x;
-    std::vector<double> ztopo, mann, h, H, rain, evap, transp, qtop, qbot, hpoten, qcapa;
-    std::vector<double> bcpsi[6], bcq[6];
-    std::vector<int> bctype[6];
-    std::vector<double> root;
-    std::vector<int> procmap;
-    std::vector<double> a3d, rhs3d;
-    std::vector<double> a2d, rhs2d;
-    std::vector<double> kw2, ke2, ks2, kn2;
-};
+using namespace dor_detail;
 
 namespace {
 
@@ -79,11 +50,6 @@ size_t face_size(const dhara_grid& g, int f)
     }
 }
 
-int fail(dor_ctx* c, int code, const char* msg)
-{
-    if (c) c->err = msg;
-    return code;
-}
 
 // ---------------------------------------------------------------------------------------
 // Mualem-van Genuchten.  src/global.cu:66-105 (vanGenuchten), SI: psi [m], alpha [1/m].
@@ -1230,3 +1196,24 @@ int dor_column_run(int32_t nz, double dz, const dhara_params* p, double Ks, doub
 }
 
 }  // extern "C"
+
+// ---- internal entry points shared with the adjoint (dhara_oracle_adjoint.cpp) ----------
+namespace dor_detail {
+
+int fail(dor_ctx* c, int code, const char* msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+void vg_cell(const dor_ctx& c, size_t i, double psi, double* theta, double* K, double* C)
+{
+    vg_stable(psi, c.ksat[i], cell_alpha(c, i), cell_n(c, i), c.p.theta_s, c.p.theta_r, theta, K, C);
+}
+
+double cell_alpha_of(const dor_ctx& c, size_t i) { return cell_alpha(c, i); }
+double cell_n_of(const dor_ctx& c, size_t i) { return cell_n(c, i); }
+
+int step(dor_ctx& c, dhara_step_stats* st) { return dor_step(&c, 1, st); }
+
+}  // namespace dor_detail

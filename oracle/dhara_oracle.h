@@ -62,6 +62,17 @@ int dor_overland_conductance(dor_ctx* ctx, double* ke, double* kn, double* kw, d
 int dor_assemble_overland(dor_ctx* ctx, double* a2d, double* rhs);
 int dor_set_threads(int32_t nthreads);
 
+/* ---- discrete adjoint (dhara_oracle_adjoint.cpp), same contracts as the dhara_ ones --- */
+int dor_observe(dor_ctx* ctx, int32_t nsteps, double* q, double* w);
+int dor_adjoint_run(dor_ctx* ctx, const dhara_loss_spec* spec, double* loss, double* grad_lnks,
+                    double* grad_alpha, double* grad_n, dhara_step_stats* stats);
+int dor_adjoint_residual(dor_ctx* ctx, const double* psi_new, double* F);
+int dor_adjoint_jvp(dor_ctx* ctx, const double* psi_new, int32_t transpose, const double* v,
+                    double* out);
+/* dK/dpsi, dtheta/dalpha, dtheta/dn, dK/dalpha, dK/dn of each cell at psi (any may be NULL). */
+int dor_vg_derivatives(dor_ctx* ctx, const double* psi, double* dK_dpsi, double* dth_da,
+                       double* dth_dn, double* dK_da, double* dK_dn);
+
 /* ---- CUSP-semantics building blocks (known-answer tests) ---------------------------- */
 /* cusp::gallery::poisson5pt(A, m, n) / poisson7pt(A, m, n, k) as DIA
  * (cusp/gallery/detail/poisson.inl:28-86, stencil.inl:143-186). */

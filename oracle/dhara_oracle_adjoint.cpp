@@ -1,0 +1,704 @@
+/*
+ * dhara_oracle_adjoint.cpp -- discrete adjoint of the oracle's coupled step (CPU, FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY (see dhara_oracle.h).  No reference counterpart: the reference
+ * computes no sensitivities.  The forward it differentiates is the oracle's restatement of
+ * src/subsurface.cu:686-847 (with src/global.cu:172-257) and src/overland.cu:387-419.
+ *
+ * Step n -> n+1 as differentiated here (SI units, flux boundaries only, am = 0, theta^n =
+ * theta(psi^n)):
+ *   surface   hp = h^n + (rain - evap) dt,  q = IdentifyTopBoundary(hp, qcapa(K(psi^n_top)),
+ *             theta^n_top, Ks_top)  (one branch of a min/max chain, src/global.cu:172-257)
+ *   Richards  F(psi^{n+1}) = 0, the fixed point of the modified Picard iteration:
+ *             F_c = (theta_c - theta^n_c)/dt + Ss theta_c (psi_c - psi^n_c)/(phi dt)
+ *                   + sum_{faces f of c} (+-) G_f - [top] q/dz + [bottom] K_c/dz - sources,
+ *             G_f = w_f K_f (psi_a - psi_b) + [vertical] K_f/dz,  K_f = (K_a + K_b)/2,
+ *             a the lower-index cell of face f (F_a += G_f, F_b -= G_f)
+ *   ponding   h' = max(hp - q dt, 0)
+ *   overland  H = h' + z, face conductances k_f(H, h', n_manning), one linear solve
+ *             (I + dt L(k)) H^{n+1} = H with the outer ring fixed, h^{n+1} = max(H^{n+1}-z, 0),
+ *             Q = outlet discharge across x = 0.
+ * Loss L = w_q sum_t (Q_t - Q*_t)^2 + w_w sum_t sum_cols (W_t - W*_t)^2, W = sum_k theta dz.
+ * Reverse: J^T lambda = dL/dpsi^{n+1} (J = dF/dpsi^{n+1}, BiCGStab, Jacobi preconditioner),
+ * A2^T mu = dL/dH^{n+1}; states of each checkpoint interval recomputed from its checkpoint.
+ *
+ * How it is pinned (tests/test_oracle_adjoint.py): gradient vs central finite differences of
+ * the oracle's forward loss; J v vs central differences of F; <J v, w> = <v, J^T w>;
+ * F(psi^{n+1}) ~ 0 after a converged forward step (residual formula == forward solver);
+ * van Genuchten parameter derivatives vs central differences; zero misfit -> zero gradient.
+ */
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "oracle_ctx.h"
+
+using namespace dor_detail;
+
+namespace {
+
+// ---- van Genuchten with derivatives -----------------------------------------------------
+// x = alpha |psi|, t = x^n, m = 1 - 1/n, Se = (1+t)^-m, u = t/(1+t) = 1 - Se^(1/m),
+// theta = tr + (ts - tr) Se, K = Ks sqrt(Se) (1 - u^m)^2.
+// A variation with d(ln t) and dm gives d ln Se = -dm ln(1+t) - m t/(1+t) d ln t,
+// d ln u = d ln t / (1+t), d(u^m) = u^m (dm ln u + m d ln u),
+// dK = K/2 d ln Se - 2 Ks sqrt(Se) (1 - u^m) d(u^m).
+// psi: d ln t = n/psi; alpha: d ln t = n/alpha; n: d ln t = ln x, dm = 1/n^2.
+struct VgD {
+    double th, K, C, dK, dth_a, dth_n, dK_a, dK_n;
+};
+
+VgD vg_d(const dor_ctx& c, size_t i, double psi)
+{
+    VgD o{};
+    vg_cell(c, i, psi, &o.th, &o.K, &o.C);
+    if (!(psi < 0.0)) return o;   // saturated: theta = ts, K = Ks, no dependence on alpha, n
+    const double a = cell_alpha_of(c, i), n = cell_n_of(c, i);
+    const double ts = c.p.theta_s, tr = c.p.theta_r, Ks = c.ksat[i];
+    const double m = 1.0 - 1.0 / n;
+    const double lx = std::log(-a * psi);
+    const double t = std::exp(n * lx);
+    const double l1t = std::log1p(t);
+    const double Se = std::exp(-m * l1t);
+    const double lu = n * lx - l1t;
+    const double um = std::exp(m * lu);
+    const double w = -std::expm1(m * lu);
+    const double sqSe = std::sqrt(Se);
+    auto var = [&](double dlt, double dm, double& dth, double& dK) {
+        const double dlnSe = -dm * l1t - m * t / (1.0 + t) * dlt;
+        dth = (ts - tr) * Se * dlnSe;
+        const double dum = um * (dm * lu + m * dlt / (1.0 + t));
+        dK = o.K * 0.5 * dlnSe - 2.0 * Ks * sqSe * w * dum;
+    };
+    double dth_psi;
+    var(n / psi, 0.0, dth_psi, o.dK);
+    var(n / a, 0.0, o.dth_a, o.dK_a);
+    var(lx, 1.0 / (n * n), o.dth_n, o.dK_n);
+    return o;
+}
+
+// ---- surface pass (EstimateFluxes + IdentifyTopBoundary) with its active branch ---------
+enum QBranch { Q_CAPA = 0, Q_KS, Q_HP, Q_DEFICIT };
+
+struct Surf {
+    double hp, q, hpost;
+    int br;
+};
+
+Surf surface_eval(const dor_ctx& c, size_t t, double psin, double thn, double Kn, double h)
+{
+    const double dt = c.p.dt, dz = c.g.dz, Ks = c.ksat[t];
+    Surf s{};
+    s.hp = h + c.rain[t] * dt - c.evap[t] * dt;
+    double qc = -Kn * ((psin - s.hp - 0.5 * dz) / (0.5 * dz));
+    int qc_br = Q_CAPA;
+    if (h / dt > Ks && qc < Ks) { qc = Ks; qc_br = Q_KS; }
+    if (s.hp > 0.0) {
+        if (s.hp / dt > qc) { s.q = qc; s.br = qc_br; }
+        else { s.q = s.hp / dt; s.br = Q_HP; }
+        const double def = (c.p.theta_s - thn) * dz / dt;
+        if (def < s.q) { s.q = def; s.br = Q_DEFICIT; }
+        if (Ks < s.q) { s.q = Ks; s.br = Q_KS; }
+    } else {
+        s.q = s.hp / dt;
+        s.br = Q_HP;
+    }
+    s.hpost = std::max(s.hp - s.q * dt, 0.0);
+    return s;
+}
+
+// ---- subsurface residual and Jacobian ---------------------------------------------------
+struct Lin {
+    std::vector<VgD> d1;      // constitutive state at psi^{n+1}
+    std::vector<double> Ld;   // d(local part of F_c)/d psi_c
+    std::vector<double> qtop;
+};
+
+void lin_at(const dor_ctx& c, const double* psi1, const double* psin, Lin& L)
+{
+    const long N = (long)c.N;
+    const int nz = c.g.nz;
+    const long nxy = (long)c.g.nx * c.g.ny;
+    const double dt = c.p.dt, dz = c.g.dz, Ss = c.p.Ss, phi = c.p.porosity;
+    L.d1.resize(N);
+    L.Ld.resize(N);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        const VgD d = vg_d(c, i, psi1[i]);
+        L.d1[i] = d;
+        double ld = d.C / dt + Ss * (d.C * (psi1[i] - psin[i]) + d.th) / (phi * dt);
+        if (i / nxy == nz - 1) ld += d.dK / dz;
+        L.Ld[i] = ld;
+    }
+}
+
+// Visit the (up to) 6 faces of cell i: f(nb, w, vertical, c_is_a).
+template <class Fn>
+inline void faces_of(const dor_ctx& c, long i, Fn f)
+{
+    const int nx = c.g.nx, ny = c.g.ny, nz = c.g.nz;
+    const long nxy = (long)nx * ny;
+    const int k = (int)(i / nxy), j = (int)((i % nxy) / nx), ii = (int)(i % nx);
+    const double wx = 1.0 / (c.g.dx * c.g.dx), wy = 1.0 / (c.g.dy * c.g.dy), wz = 1.0 / (c.g.dz * c.g.dz);
+    if (ii > 0) f(i - 1, wx, false, false);
+    if (ii < nx - 1) f(i + 1, wx, false, true);
+    if (j > 0) f(i - nx, wy, false, false);
+    if (j < ny - 1) f(i + nx, wy, false, true);
+    if (k > 0) f(i - nxy, wz, true, false);
+    if (k < nz - 1) f(i + nxy, wz, true, true);
+}
+
+// gamma_f = w (psi_a - psi_b) + [vertical] 1/dz
+inline double gamma_f(const dor_ctx& c, double w, bool vert, double psi_a, double psi_b)
+{
+    return w * (psi_a - psi_b) + (vert ? 1.0 / c.g.dz : 0.0);
+}
+
+void residual(const dor_ctx& c, const double* psi1, const double* psin, const double* thn,
+              const double* qtop, std::vector<double>& F)
+{
+    const long N = (long)c.N;
+    const int nx = c.g.nx, ny = c.g.ny, nz = c.g.nz;
+    const long nxy = (long)nx * ny;
+    const double dt = c.p.dt, dx = c.g.dx, dy = c.g.dy, dz = c.g.dz, Ss = c.p.Ss, phi = c.p.porosity;
+    std::vector<double> th(N), K(N), Cc(N);
+    for (long i = 0; i < N; ++i) vg_cell(c, i, psi1[i], &th[i], &K[i], &Cc[i]);
+    F.assign(N, 0.0);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        const int k = (int)(i / nxy), j = (int)((i % nxy) / nx), ii = (int)(i % nx);
+        const long t = i % nxy;
+        double f = (th[i] - thn[i]) / dt + Ss * th[i] * (psi1[i] - psin[i]) / (phi * dt);
+        faces_of(c, i, [&](long nb, double w, bool vert, bool c_is_a) {
+            const double Kf = 0.5 * (K[i] + K[nb]);
+            if (c_is_a) f += w * Kf * (psi1[i] - psi1[nb]) + (vert ? Kf / dz : 0.0);
+            else f -= w * Kf * (psi1[nb] - psi1[i]) + (vert ? Kf / dz : 0.0);
+        });
+        if (k == 0) f -= qtop[t] / dz;
+        if (k == nz - 1) f += K[i] / dz;
+        if (ii == 0) f -= c.bcq[FW][(size_t)k * ny + j] / dx;
+        if (ii == nx - 1) f += c.bcq[FE][(size_t)k * ny + j] / dx;
+        if (j == 0) f -= c.bcq[FS][(size_t)k * nx + ii] / dy;
+        if (j == ny - 1) f += c.bcq[FN][(size_t)k * nx + ii] / dy;
+        const double tr = c.transp[t];
+        if (tr > 0.0) f += tr * c.root[(size_t)c.procmap[t] * nz + k] / dz;
+        F[i] = f;
+    }
+}
+
+// out = J v
+void jv(const dor_ctx& c, const Lin& L, const double* psi1, const double* v, double* out)
+{
+    const long N = (long)c.N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        double acc = L.Ld[i] * v[i];
+        faces_of(c, i, [&](long nb, double w, bool vert, bool c_is_a) {
+            const double Kf = 0.5 * (L.d1[i].K + L.d1[nb].K);
+            const double g = c_is_a ? gamma_f(c, w, vert, psi1[i], psi1[nb])
+                                    : gamma_f(c, w, vert, psi1[nb], psi1[i]);
+            const double kk = 0.5 * g * (L.d1[i].dK * v[i] + L.d1[nb].dK * v[nb]);
+            acc += w * Kf * (v[i] - v[nb]) + (c_is_a ? kk : -kk);
+        });
+        out[i] = acc;
+    }
+}
+
+// out = J^T v
+void jtv(const dor_ctx& c, const Lin& L, const double* psi1, const double* v, double* out)
+{
+    const long N = (long)c.N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        double acc = L.Ld[i] * v[i];
+        faces_of(c, i, [&](long nb, double w, bool vert, bool c_is_a) {
+            const double Kf = 0.5 * (L.d1[i].K + L.d1[nb].K);
+            // (v_a - v_b) dG_f/dpsi_i
+            if (c_is_a) {
+                const double g = gamma_f(c, w, vert, psi1[i], psi1[nb]);
+                acc += (v[i] - v[nb]) * (w * Kf + 0.5 * L.d1[i].dK * g);
+            } else {
+                const double g = gamma_f(c, w, vert, psi1[nb], psi1[i]);
+                acc += (v[nb] - v[i]) * (-w * Kf + 0.5 * L.d1[i].dK * g);
+            }
+        });
+        out[i] = acc;
+    }
+}
+
+double dotv(const std::vector<double>& a, const std::vector<double>& b)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+// Jacobi-preconditioned BiCGStab (van der Vorst 1992) from x = 0; stops when
+// |r|_2 <= tol |b|_2 or after maxit iterations.  Returns the iteration count.
+template <class Op>
+int bicgstab(long n, Op A, const std::vector<double>& diag, const std::vector<double>& b,
+             std::vector<double>& x, double tol, int maxit, bool* conv)
+{
+    x.assign(n, 0.0);
+    std::vector<double> r = b, rh = b, p(n, 0.0), v(n, 0.0), s(n), t(n), y(n), z(n);
+    const double bn = std::sqrt(dotv(b, b));
+    *conv = true;
+    if (bn == 0.0) return 0;
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0;
+    while (it < maxit) {
+        const double rho1 = dotv(rh, r);
+        if (rho1 == 0.0) break;
+        const double beta = (rho1 / rho) * (alpha / omega);
+        rho = rho1;
+        for (long i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        for (long i = 0; i < n; ++i) y[i] = p[i] / diag[i];
+        A(y.data(), v.data());
+        alpha = rho / dotv(rh, v);
+        for (long i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+        ++it;
+        if (std::sqrt(dotv(s, s)) <= tol * bn) {
+            for (long i = 0; i < n; ++i) x[i] += alpha * y[i];
+            return it;
+        }
+        for (long i = 0; i < n; ++i) z[i] = s[i] / diag[i];
+        A(z.data(), t.data());
+        const double tt = dotv(t, t);
+        omega = tt > 0.0 ? dotv(t, s) / tt : 0.0;
+        for (long i = 0; i < n; ++i) x[i] += alpha * y[i] + omega * z[i];
+        for (long i = 0; i < n; ++i) r[i] = s[i] - omega * t[i];
+        if (std::sqrt(dotv(r, r)) <= tol * bn) return it;
+        if (omega == 0.0) break;
+    }
+    *conv = false;
+    return it;
+}
+
+// ---- overland face conductance with its derivatives (the row's own view) ----------------
+struct FaceD {
+    double k;
+    bool active;
+    double dgd, dgc, dh;   // dk/d(direct gradient), dk/d(cross gradient), dk/dh (each side)
+};
+
+FaceD face_d(const dor_ctx& c, double Hnb, double Hc, double hnb, double hc, double mnb, double mc,
+             double gd, double gc)
+{
+    FaceD o{};
+    const double S = std::sqrt(gd * gd + gc * gc);
+    const double hup = (Hnb > Hc) ? hnb : hc;
+    const double hav = (hnb + hc) / 2.;
+    if (hup > c.p.hmin && hav > c.p.hmin && std::fabs(S) > c.p.delta) {
+        o.k = std::pow(hav, 5. / 3.) / (0.5 * (mnb + mc) * std::sqrt(S));
+        o.active = true;
+        o.dgd = -0.5 * o.k * gd / (S * S);
+        o.dgc = -0.5 * o.k * gc / (S * S);
+        o.dh = (5. / 6.) * o.k / hav;
+    } else {
+        o.k = c.p.K0;
+    }
+    return o;
+}
+
+// One interior row's view of its 4 faces: the face values, and the cells each depends on.
+struct RowFaces {
+    FaceD f[4];            // E, W, N, S
+    long nb[4];            // neighbour across each face
+    long gcell[4][2];      // direct-gradient cells (+, -) with coefficient 1/spacing
+    long ccell[4][4];      // cross-gradient cells (+, +, -, -) with coefficient 1/(4 spacing)
+    double gsp[4], csp[4];
+};
+
+RowFaces row_faces(const dor_ctx& c, const std::vector<double>& H, const std::vector<double>& h, long I)
+{
+    const long N = c.g.nx;
+    const double dx = c.g.dx, dy = c.g.dy;
+    RowFaces r;
+    const long nb[4] = {I + 1, I - 1, I + N, I - N};
+    const long g[4][2] = {{I + 1, I}, {I, I - 1}, {I + N, I}, {I, I - N}};
+    const long cc[4][4] = {{I + N + 1, I + N, I - N + 1, I - N},
+                           {I + N, I + N - 1, I - N, I - N - 1},
+                           {I + N + 1, I + 1, I + N - 1, I - 1},
+                           {I + 1, I - N + 1, I - 1, I - N - 1}};
+    const double gs[4] = {dx, dx, dy, dy}, cs[4] = {4 * dy, 4 * dy, 4 * dx, 4 * dx};
+    for (int d = 0; d < 4; ++d) {
+        r.nb[d] = nb[d];
+        r.gcell[d][0] = g[d][0];
+        r.gcell[d][1] = g[d][1];
+        for (int q = 0; q < 4; ++q) r.ccell[d][q] = cc[d][q];
+        r.gsp[d] = gs[d];
+        r.csp[d] = cs[d];
+        const double gd = (H[g[d][0]] - H[g[d][1]]) / gs[d];
+        const double gc = (H[cc[d][0]] + H[cc[d][1]] - H[cc[d][2]] - H[cc[d][3]]) / cs[d];
+        r.f[d] = face_d(c, H[nb[d]], H[I], h[nb[d]], h[I], c.mann[nb[d]], c.mann[I], gd, gc);
+    }
+    return r;
+}
+
+inline bool ring(const dor_ctx& c, long t)
+{
+    const int i = (int)(t % c.g.nx), j = (int)(t / c.g.nx);
+    return i == 0 || j == 0 || i == c.g.nx - 1 || j == c.g.ny - 1;
+}
+
+struct AdjStats {
+    long bicg_iters = 0, olf_iters = 0;
+    bool bicg_conv = true, olf_conv = true;
+};
+
+// Reverse of the overland step.  In: hpost (ponding before the overland step), H1 (its
+// solution), hbar1 = dL/dh^{n+1}, rq = dL/dQ.  Out: dL/dhpost.
+void overland_reverse(const dor_ctx& c, const std::vector<double>& hpost, const std::vector<double>& H1,
+                      const std::vector<double>& hbar1, double rq, double tol, int maxit,
+                      std::vector<double>& hbar_post, AdjStats& as)
+{
+    const long S = (long)c.S;
+    const int nx = c.g.nx, ny = c.g.ny;
+    const double cxy[4] = {c.p.dt / (c.g.dx * c.g.dx), c.p.dt / (c.g.dx * c.g.dx),
+                           c.p.dt / (c.g.dy * c.g.dy), c.p.dt / (c.g.dy * c.g.dy)};
+    std::vector<double> H(S);
+    for (long t = 0; t < S; ++t) H[t] = hpost[t] + c.ztopo[t];
+    std::vector<RowFaces> rows(S);
+    for (long t = 0; t < S; ++t)
+        if (!ring(c, t)) rows[t] = row_faces(c, H, hpost, t);
+    // dL/dH^{n+1}: depth clip and outlet discharge (west face of the i = 1 rows)
+    std::vector<double> Hb1(S, 0.0);
+    for (long t = 0; t < S; ++t)
+        if (H1[t] - c.ztopo[t] > 0.0) Hb1[t] = hbar1[t];
+    for (int j = 1; j < ny - 1; ++j) {
+        const long I = (long)j * nx + 1;
+        const double kq = rows[I].f[1].k * c.g.dy / c.g.dx;
+        Hb1[I] += rq * kq;
+        Hb1[I - 1] -= rq * kq;
+    }
+    // A2^T mu = Hb1 on the interior; the transposed coupling of row I to J is row J's view
+    std::vector<double> diag(S, 1.0), rhs(S, 0.0), mu;
+    for (long t = 0; t < S; ++t) {
+        if (ring(c, t)) continue;
+        double d = 1.0;
+        for (int q = 0; q < 4; ++q) d += cxy[q] * rows[t].f[q].k;
+        diag[t] = d;
+        rhs[t] = Hb1[t];
+    }
+    auto opp = [](int q) { return q ^ 1; };   // E<->W, N<->S
+    auto A2T = [&](const double* v, double* y) {
+        for (long t = 0; t < S; ++t) {
+            if (ring(c, t)) { y[t] = v[t]; continue; }
+            double acc = diag[t] * v[t];
+            for (int q = 0; q < 4; ++q) {
+                const long J = rows[t].nb[q];
+                if (ring(c, J)) continue;
+                acc -= cxy[q] * rows[J].f[opp(q)].k * v[J];
+            }
+            y[t] = acc;
+        }
+    };
+    bool conv = true;
+    as.olf_iters += bicgstab(S, A2T, diag, rhs, mu, tol, maxit, &conv);
+    as.olf_conv = as.olf_conv && conv;
+    // ring cells: mu_r = Hb1_r + sum over interior neighbours of c k mu
+    for (long t = 0; t < S; ++t)
+        if (ring(c, t)) mu[t] = Hb1[t];
+    for (long t = 0; t < S; ++t) {
+        if (ring(c, t)) continue;
+        for (int q = 0; q < 4; ++q) {
+            const long J = rows[t].nb[q];
+            if (ring(c, J)) mu[J] += cxy[q] * rows[t].f[q].k * mu[t];
+        }
+    }
+    // dL/dH (H = hpost + z) and dL/dhpost through the face conductances
+    std::vector<double> Hb(mu), hb(S, 0.0);
+    for (long t = 0; t < S; ++t) {
+        if (ring(c, t)) continue;
+        const RowFaces& r = rows[t];
+        for (int q = 0; q < 4; ++q) {
+            double kb = -mu[t] * cxy[q] * (H1[t] - H1[r.nb[q]]);
+            if (q == 1 && (t % nx) == 1) kb += rq * (H1[t] - H1[t - 1]) * c.g.dy / c.g.dx;
+            const FaceD& f = r.f[q];
+            if (!f.active || kb == 0.0) continue;
+            const double a = kb * f.dgd / r.gsp[q];
+            Hb[r.gcell[q][0]] += a;
+            Hb[r.gcell[q][1]] -= a;
+            const double b = kb * f.dgc / r.csp[q];
+            Hb[r.ccell[q][0]] += b;
+            Hb[r.ccell[q][1]] += b;
+            Hb[r.ccell[q][2]] -= b;
+            Hb[r.ccell[q][3]] -= b;
+            hb[t] += kb * f.dh;
+            hb[r.nb[q]] += kb * f.dh;
+        }
+    }
+    hbar_post.resize(S);
+    for (long t = 0; t < S; ++t) hbar_post[t] = hb[t] + Hb[t];
+}
+
+struct Tape {
+    std::vector<std::vector<double>> psi, h, H1;
+};
+
+void restore(dor_ctx& c, const std::vector<double>& psi, const std::vector<double>& h)
+{
+    const long N = (long)c.N;
+    c.psi_n = psi;
+    c.x = psi;
+    c.psi_m = psi;
+    for (long i = 0; i < N; ++i) vg_cell(c, i, psi[i], &c.theta_n[i], &c.K[i], &c.C[i]);
+    c.theta_m = c.theta_n;
+    c.theta_new = c.theta_n;
+    c.h = h;
+}
+
+int check_supported(dor_ctx* c)
+{
+    if (!c->initialized) return fail(c, DHARA_E_STATE, "dor_init_state not called");
+    if (c->p.am != 0.0) return fail(c, DHARA_E_UNSUPPORTED, "adjoint needs am = 0");
+    if (c->p.theta_carry != DHARA_THETA_CARRY_CONSISTENT)
+        return fail(c, DHARA_E_UNSUPPORTED, "adjoint needs the consistent theta carry");
+    for (int f = 0; f < 6; ++f) {
+        if (f == FT && c->p.top_bc == DHARA_TOP_SURFACE) continue;
+        for (int t : c->bctype[f])
+            if (t == DHARA_BC_DIRICHLET) return fail(c, DHARA_E_UNSUPPORTED, "adjoint needs flux faces");
+    }
+    return DHARA_OK;
+}
+
+double column_storage(const dor_ctx& c, long t)
+{
+    double w = 0.0;
+    for (int k = 0; k < c.g.nz; ++k) w += c.theta_n[(size_t)k * c.S + t] * c.g.dz;
+    return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dor_observe(dor_ctx* c, int32_t nsteps, double* q, double* w)
+{
+    if (!c || nsteps < 0) return DHARA_E_ARG;
+    if (!c->initialized) return fail(c, DHARA_E_STATE, "dor_init_state not called");
+    for (int s = 0; s < nsteps; ++s) {
+        dhara_step_stats st;
+        std::memset(&st, 0, sizeof(st));
+        int rc = step(*c, &st);
+        if (rc) return rc;
+        if (q) q[s] = c->p.overland ? st.outlet_discharge : 0.0;
+        if (w)
+            for (long t = 0; t < (long)c->S; ++t) w[(size_t)s * c->S + t] = column_storage(*c, t);
+    }
+    return DHARA_OK;
+}
+
+int dor_adjoint_residual(dor_ctx* c, const double* psi_new, double* F)
+{
+    if (!c || !psi_new || !F) return DHARA_E_ARG;
+    int rc = check_supported(c);
+    if (rc) return rc;
+    std::vector<double> qtop(c->S);
+    for (long t = 0; t < (long)c->S; ++t)
+        qtop[t] = c->p.top_bc == DHARA_TOP_SURFACE
+                      ? surface_eval(*c, t, c->psi_n[t], c->theta_n[t], c->K[t], c->h[t]).q
+                      : c->bcq[FT][t];
+    std::vector<double> f;
+    residual(*c, psi_new, c->psi_n.data(), c->theta_n.data(), qtop.data(), f);
+    std::memcpy(F, f.data(), c->N * sizeof(double));
+    return DHARA_OK;
+}
+
+int dor_adjoint_jvp(dor_ctx* c, const double* psi_new, int32_t transpose, const double* v, double* out)
+{
+    if (!c || !psi_new || !v || !out) return DHARA_E_ARG;
+    int rc = check_supported(c);
+    if (rc) return rc;
+    Lin L;
+    lin_at(*c, psi_new, c->psi_n.data(), L);
+    if (transpose) jtv(*c, L, psi_new, v, out);
+    else jv(*c, L, psi_new, v, out);
+    return DHARA_OK;
+}
+
+int dor_adjoint_run(dor_ctx* c, const dhara_loss_spec* spec, double* loss, double* g_lnks,
+                    double* g_alpha, double* g_n, dhara_step_stats* stats)
+{
+    if (!c || !spec || spec->nsteps < 1 || spec->checkpoint_every < 1) return DHARA_E_ARG;
+    int rc = check_supported(c);
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    const long N = (long)c->N, S = (long)c->S;
+    const int T = spec->nsteps, kc = spec->checkpoint_every, nz = c->g.nz;
+    const double dt = c->p.dt, dz = c->g.dz, Ss = c->p.Ss, phi = c->p.porosity;
+    const bool surf = c->p.top_bc == DHARA_TOP_SURFACE;
+    const bool olf = c->p.overland != 0;
+    const double tol = spec->adj_tol > 0 ? spec->adj_tol : 1e-12;
+    const int maxit = spec->adj_maxiter > 0 ? spec->adj_maxiter : 5000;
+    // forward with checkpoints; misfits kept for the reverse sweep
+    std::vector<std::vector<double>> ck_psi, ck_h;
+    std::vector<double> rq(T, 0.0), rw((size_t)T * S, 0.0);
+    double L = 0.0;
+    for (int s = 0; s < T; ++s) {
+        if (s % kc == 0) { ck_psi.push_back(c->psi_n); ck_h.push_back(c->h); }
+        dhara_step_stats st;
+        std::memset(&st, 0, sizeof(st));
+        if ((rc = step(*c, &st))) return rc;
+        const double Q = olf ? st.outlet_discharge : 0.0;
+        const double dq = Q - (spec->q_target ? spec->q_target[s] : 0.0);
+        L += spec->w_q * dq * dq;
+        rq[s] = 2.0 * spec->w_q * dq;
+        for (long t = 0; t < S; ++t) {
+            const double dw = column_storage(*c, t) - (spec->w_target ? spec->w_target[(size_t)s * S + t] : 0.0);
+            L += spec->w_w * dw * dw;
+            rw[(size_t)s * S + t] = 2.0 * spec->w_w * dw;
+        }
+    }
+    if (loss) *loss = L;
+    AdjStats as;
+    if (!g_lnks && !g_alpha && !g_n) return DHARA_OK;
+    const std::vector<double> fin_psi = c->psi_n, fin_h = c->h;
+    std::vector<double> gK(N, 0.0), ga(N, 0.0), gn(N, 0.0);
+    std::vector<double> psib(N, 0.0), hb(S, 0.0);
+    const int nseg = (T + kc - 1) / kc;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int n0 = seg * kc, n1 = std::min(T, n0 + kc);
+        Tape tp;
+        restore(*c, ck_psi[seg], ck_h[seg]);
+        tp.psi.push_back(c->psi_n);
+        tp.h.push_back(c->h);
+        for (int s = n0; s < n1; ++s) {
+            if ((rc = step(*c, nullptr))) return rc;
+            tp.psi.push_back(c->psi_n);
+            tp.h.push_back(c->h);
+            tp.H1.push_back(c->H);
+        }
+        for (int s = n1 - 1; s >= n0; --s) {
+            const std::vector<double>& psin = tp.psi[s - n0];
+            const std::vector<double>& hn = tp.h[s - n0];
+            const std::vector<double>& psi1 = tp.psi[s - n0 + 1];
+            Lin lin;
+            lin_at(*c, psi1.data(), psin.data(), lin);
+            // column-storage misfit after step s
+            for (long i = 0; i < N; ++i) {
+                const double r = rw[(size_t)s * S + i % S] * dz;
+                psib[i] += r * lin.d1[i].C;
+                ga[i] += r * lin.d1[i].dth_a;
+                gn[i] += r * lin.d1[i].dth_n;
+            }
+            // surface pass of step s, recomputed at psi^n
+            std::vector<VgD> dn(N);
+            for (long i = 0; i < N; ++i) dn[i] = vg_d(*c, i, psin[i]);
+            std::vector<Surf> sf(S);
+            std::vector<double> hpost(S);
+            for (long t = 0; t < S; ++t) {
+                if (surf) {
+                    sf[t] = surface_eval(*c, t, psin[t], dn[t].th, dn[t].K, hn[t]);
+                    hpost[t] = sf[t].hpost;
+                } else {
+                    hpost[t] = hn[t];
+                }
+            }
+            std::vector<double> hbp;
+            if (olf) overland_reverse(*c, hpost, tp.H1[s - n0], hb, rq[s], tol, maxit, hbp, as);
+            else hbp = hb;
+            std::vector<double> hpb(S, 0.0), qb(S, 0.0), hnb(S, 0.0);
+            for (long t = 0; t < S; ++t) {
+                if (!surf) { hnb[t] = hbp[t]; continue; }
+                if (sf[t].hp - sf[t].q * dt > 0.0) { hpb[t] += hbp[t]; qb[t] -= dt * hbp[t]; }
+            }
+            // subsurface: J^T lambda = psib
+            std::vector<double> Jd(N), lam;
+            for (long i = 0; i < N; ++i) {
+                double d = lin.Ld[i];
+                faces_of(*c, i, [&](long nb, double w, bool vert, bool c_is_a) {
+                    const double Kf = 0.5 * (lin.d1[i].K + lin.d1[nb].K);
+                    const double g = c_is_a ? gamma_f(*c, w, vert, psi1[i], psi1[nb])
+                                            : gamma_f(*c, w, vert, psi1[nb], psi1[i]);
+                    d += w * Kf + (c_is_a ? 0.5 : -0.5) * lin.d1[i].dK * g;
+                });
+                Jd[i] = d;
+            }
+            bool conv = true;
+            as.bicg_iters += bicgstab(
+                N, [&](const double* v, double* y) { jtv(*c, lin, psi1.data(), v, y); }, Jd, psib, lam, tol,
+                maxit, &conv);
+            as.bicg_conv = as.bicg_conv && conv;
+            std::vector<double> psinb(N, 0.0), thnb(N, 0.0);
+            for (long i = 0; i < N; ++i) {
+                const VgD& d = lin.d1[i];
+                double Kb = (i / (long)S == nz - 1) ? lam[i] / dz : 0.0;
+                faces_of(*c, i, [&](long nb, double w, bool vert, bool c_is_a) {
+                    const double g = c_is_a ? gamma_f(*c, w, vert, psi1[i], psi1[nb])
+                                            : gamma_f(*c, w, vert, psi1[nb], psi1[i]);
+                    Kb += (c_is_a ? lam[i] - lam[nb] : lam[nb] - lam[i]) * 0.5 * g;
+                });
+                const double cth = 1.0 / dt + Ss * (psi1[i] - psin[i]) / (phi * dt);
+                gK[i] -= Kb * d.K;
+                ga[i] -= lam[i] * cth * d.dth_a + Kb * d.dK_a;
+                gn[i] -= lam[i] * cth * d.dth_n + Kb * d.dK_n;
+                psinb[i] = lam[i] * Ss * d.th / (phi * dt);
+                thnb[i] = lam[i] / dt;
+            }
+            if (surf) {
+                for (long t = 0; t < S; ++t) qb[t] += lam[t] / dz;
+                for (long t = 0; t < S; ++t) {
+                    const double q = qb[t];
+                    const Surf& s_ = sf[t];
+                    double Knb = 0.0;
+                    switch (s_.br) {
+                    case Q_CAPA:
+                        Knb = -q * (psin[t] - s_.hp - 0.5 * dz) / (0.5 * dz);
+                        psinb[t] += -q * dn[t].K / (0.5 * dz);
+                        hpb[t] += q * dn[t].K / (0.5 * dz);
+                        break;
+                    case Q_KS: gK[t] += q * c->ksat[t]; break;
+                    case Q_HP: hpb[t] += q / dt; break;
+                    default: thnb[t] += -q * dz / dt; break;
+                    }
+                    psinb[t] += Knb * dn[t].dK;
+                    gK[t] += Knb * dn[t].K;
+                    ga[t] += Knb * dn[t].dK_a;
+                    gn[t] += Knb * dn[t].dK_n;
+                    hnb[t] = hpb[t];
+                }
+            }
+            for (long i = 0; i < N; ++i) {
+                psinb[i] += thnb[i] * dn[i].C;
+                ga[i] += thnb[i] * dn[i].dth_a;
+                gn[i] += thnb[i] * dn[i].dth_n;
+            }
+            psib.swap(psinb);
+            hb.swap(hnb);
+        }
+    }
+    restore(*c, fin_psi, fin_h);
+    if (g_lnks) std::memcpy(g_lnks, gK.data(), N * sizeof(double));
+    if (g_alpha) std::memcpy(g_alpha, ga.data(), N * sizeof(double));
+    if (g_n) std::memcpy(g_n, gn.data(), N * sizeof(double));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->lin_iters = (int32_t)as.bicg_iters;
+        stats->olf_iters = (int32_t)as.olf_iters;
+        stats->lin_converged = as.bicg_conv ? 1 : 0;
+        stats->olf_converged = as.olf_conv ? 1 : 0;
+        stats->wall_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return DHARA_OK;
+}
+
+int dor_vg_derivatives(dor_ctx* c, const double* psi, double* dK_dpsi, double* dth_da, double* dth_dn,
+                       double* dK_da, double* dK_dn)
+{
+    if (!c || !psi) return DHARA_E_ARG;
+    for (size_t i = 0; i < c->N; ++i) {
+        const VgD d = vg_d(*c, i, psi[i]);
+        if (dK_dpsi) dK_dpsi[i] = d.dK;
+        if (dth_da) dth_da[i] = d.dth_a;
+        if (dth_dn) dth_dn[i] = d.dth_n;
+        if (dK_da) dK_da[i] = d.dK_a;
+        if (dK_dn) dK_dn[i] = d.dK_n;
+    }
+    return DHARA_OK;
+}
+
+}  // extern "C"

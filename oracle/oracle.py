@@ -40,6 +40,7 @@ def lib():
         L.dor_overland_conductance.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.dor_assemble_overland.argtypes = [C.c_void_p, dp, dp]
         L.dor_set_threads.argtypes = [C.c_int32]
+        L.dor_vg_derivatives.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp]
         L.dor_poisson5pt.argtypes = [C.c_int32, C.c_int32, ip, dp]
         L.dor_poisson7pt.argtypes = [C.c_int32, C.c_int32, C.c_int32, ip, dp]
         L.dor_dia_spmv.argtypes = [C.c_int32, C.c_int32, ip, dp, dp, dp]
@@ -85,6 +86,13 @@ class OracleModel(BaseModel):
         self._check(self._lib.dor_eval_constitutive_refform(self._h, _dp(psi), _dp(th), _dp(k), _dp(c)),
                     "refform")
         return th, k, c
+
+    def vg_derivatives(self, psi):
+        """dK/dpsi, dtheta/dalpha, dtheta/dn, dK/dalpha, dK/dn of every cell at psi."""
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        out = [np.empty(psi.size) for _ in range(5)]
+        self._check(self._lib.dor_vg_derivatives(self._h, _dp(psi), *[_dp(a) for a in out]), "vg_derivatives")
+        return out
 
     def assemble(self):
         n = self.size("PSI")

@@ -1,0 +1,50 @@
+/*
+ * oracle_ctx.h -- state of the CPU oracle, shared by the forward restatement
+ * (dhara_oracle.cpp) and its discrete adjoint (dhara_oracle_adjoint.cpp).
+ *
+ * TEST INFRASTRUCTURE ONLY (see dhara_oracle.h).
+ */
+#ifndef DHARA_ORACLE_CTX_H
+#define DHARA_ORACLE_CTX_H
+
+#include <string>
+#include <vector>
+
+#include "dhara_oracle.h"
+
+namespace dor_detail {
+enum Face { FW = 0, FE, FS, FN, FT, FB };
+}
+
+struct dor_ctx {
+    dhara_grid g;
+    dhara_params p;
+    int lin_solver, olf_solver;
+    size_t N, S;
+    bool initialized = false;
+    std::string err;
+
+    std::vector<double> psi_n, theta_n, ksat, alpha_f, n_f;
+    bool has_alpha = false, has_n = false;
+    std::vector<double> K, C, theta_m, theta_new, psi_m, x;
+    std::vector<double> ztopo, mann, h, H, rain, evap, transp, qtop, qbot, hpoten, qcapa;
+    std::vector<double> bcpsi[6], bcq[6];
+    std::vector<int> bctype[6];
+    std::vector<double> root;
+    std::vector<int> procmap;
+    std::vector<double> a3d, rhs3d;
+    std::vector<double> a2d, rhs2d;
+    std::vector<double> kw2, ke2, ks2, kn2;
+};
+
+namespace dor_detail {
+// theta, K, C of cell i's soil at psi (the forward's stable Mualem-van Genuchten form).
+void vg_cell(const dor_ctx& c, size_t i, double psi, double* theta, double* K, double* C);
+double cell_alpha_of(const dor_ctx& c, size_t i);
+double cell_n_of(const dor_ctx& c, size_t i);
+// One coupled step (subsurface, then overland when enabled), as dor_step.
+int step(dor_ctx& c, dhara_step_stats* st);
+int fail(dor_ctx* c, int code, const char* msg);
+}  // namespace dor_detail
+
+#endif

@@ -1,0 +1,204 @@
+"""Pins of the oracle's discrete adjoint (CPU).
+
+The adjoint has no reference counterpart; it is pinned by what the mathematics fixes:
+  * van Genuchten parameter derivatives vs central differences of the constitutive law;
+  * F(psi^{n+1}) = 0 after a converged forward step (the residual the adjoint differentiates
+    is the equation the forward solver solves), and F(psi^n) != 0 as the control;
+  * J v vs central differences of F;  <J v, w> = <v, J^T w>;
+  * loss gradient vs central finite differences of the forward loss;
+  * zero misfit -> zero gradient;  the checkpoint interval does not change the gradient.
+"""
+import numpy as np
+import pytest
+
+from dhara_amd import _abi, inputs
+from oracle import oracle
+
+T_STEPS = 12
+
+
+def small_workload(perturbed=False, nx=10, ny=8, nz=6):
+    """C4-type inputs (per-cell alpha, n, GRF ln Ks) on a small grid with heavy rain and a
+    shallow water table, so ponding and outlet discharge occur within a few steps."""
+    w = inputs.c4_adjoint(nx=nx, ny=ny, nz=nz, perturbed=perturbed)
+    p = _abi.Params.from_buffer_copy(w.params)
+    p.picard_tol, p.picard_maxiter = 1e-12, 200
+    p.lin_tol, p.olf_tol, p.lin_maxiter, p.olf_maxiter = 1e-14, 1e-14, 5000, 5000
+    w.params = p
+    w.rain_schedule = [(0, 80.0 * inputs.MM_PER_H)]
+    w.fields["PSI"] = inputs.hydrostatic_psi(nx, ny, nz, 0.05, 0.2)
+    w.fields["KSAT"] = w.fields["KSAT"] * 0.3
+    return w
+
+
+def make(w, **override):
+    m = oracle.OracleModel(w.grid, _abi.Params.from_buffer_copy(w.params))
+    f = dict(w.fields)
+    f.update(override)
+    m.upload_many(f)
+    m.upload("RAIN", w.rain_at(0))
+    m.init_state(_abi.INIT_FROM_PSI)
+    return m
+
+
+@pytest.fixture(scope="module")
+def one_thread():
+    old = oracle.set_threads(0)
+    oracle.set_threads(1)   # deterministic reductions: recomputed states are bit-identical
+    yield
+    oracle.set_threads(old)
+
+
+@pytest.fixture(scope="module")
+def targets(one_thread):
+    return make(small_workload(perturbed=True)).observe(T_STEPS)
+
+
+def test_vg_parameter_derivatives(one_thread):
+    w = small_workload()
+    m = make(w)
+    n = w.grid.nx * w.grid.ny * w.grid.nz
+    rng = np.random.default_rng(3)
+    psi = -np.exp(rng.uniform(np.log(1e-3), np.log(20.0), n))
+    dK, dth_a, dth_n, dK_a, dK_n = m.vg_derivatives(psi)
+    th0, K0, C0 = m.eval_constitutive(psi)
+    # psi derivative of theta is C (closed form in the forward); of K from the adjoint
+    e = 1e-6 * np.abs(psi)
+    _, Kp, _ = m.eval_constitutive(psi + e)
+    _, Km, _ = m.eval_constitutive(psi - e)
+    np.testing.assert_allclose(dK, (Kp - Km) / (2 * e), rtol=1e-6, atol=1e-30)
+    for key, dth, dk in (("ALPHA", dth_a, dK_a), ("NVG", dth_n, dK_n)):
+        base = w.fields[key]
+        h = 1e-6 * base
+        m.upload(key, base + h)
+        thp, Kp, _ = m.eval_constitutive(psi)
+        m.upload(key, base - h)
+        thm, Km, _ = m.eval_constitutive(psi)
+        m.upload(key, base)
+        np.testing.assert_allclose(dth, (thp - thm) / (2 * h), rtol=1e-6, atol=1e-14)
+        np.testing.assert_allclose(dk, (Kp - Km) / (2 * h), rtol=1e-6, atol=1e-24)
+    # saturated cells do not depend on alpha, n
+    d = m.vg_derivatives(np.full(n, 0.1))
+    assert all(np.all(a == 0.0) for a in d)
+
+
+def _state_pair(w, steps_before):
+    a, b = make(w), make(w)
+    a.step(steps_before)
+    b.step(steps_before)
+    a.step(1)
+    return a, b
+
+
+def test_residual_vanishes_at_converged_step(one_thread):
+    w = small_workload()
+    a, b = _state_pair(w, 4)
+    psi1 = a.download("PSI")
+    F = b.adjoint_residual(psi1)
+    F0 = b.adjoint_residual(b.download("PSI"))
+    # scale: storage-change term of the step, (theta^{n+1} - theta^n)/dt
+    th1, _, _ = b.eval_constitutive(psi1)
+    scale = np.max(np.abs(th1 - b.download("THETA"))) / w.params.dt
+    assert np.max(np.abs(F0)) > 1e-3 * scale        # control: the old state is no solution
+    assert np.max(np.abs(F)) < 1e-9 * scale
+
+
+def test_jvp_matches_residual_differences(one_thread):
+    w = small_workload()
+    a, b = _state_pair(w, 3)
+    psi1 = a.download("PSI")
+    rng = np.random.default_rng(11)
+    v = rng.standard_normal(psi1.size)
+    Jv = b.adjoint_jvp(psi1, v)
+    eps = 1e-6
+    fd = (b.adjoint_residual(psi1 + eps * v) - b.adjoint_residual(psi1 - eps * v)) / (2 * eps)
+    assert np.max(np.abs(Jv - fd)) <= 1e-6 * np.max(np.abs(Jv))
+
+
+def test_jtv_is_the_transpose(one_thread):
+    w = small_workload()
+    a, b = _state_pair(w, 3)
+    psi1 = a.download("PSI")
+    rng = np.random.default_rng(12)
+    v, u = rng.standard_normal(psi1.size), rng.standard_normal(psi1.size)
+    lhs = np.dot(b.adjoint_jvp(psi1, v), u)
+    rhs = np.dot(v, b.adjoint_jvp(psi1, u, transpose=True))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1e-300) * 10
+
+
+def _loss(w, targets, **override):
+    q, wt = targets
+    m = make(w, **override)
+    loss, *_ = m.adjoint_run(T_STEPS, checkpoint_every=5, q_target=q, w_target=wt, w_q=1e6, w_w=1.0,
+                             gradients=False)
+    return loss
+
+
+def test_gradient_matches_finite_differences(one_thread, targets):
+    w = small_workload()
+    q, wt = targets
+    m = make(w)
+    loss, gk, ga, gn, st = m.adjoint_run(T_STEPS, checkpoint_every=5, q_target=q, w_target=wt,
+                                         w_q=1e6, w_w=1.0)
+    assert loss > 0 and st["lin_converged"] and st["olf_converged"]
+    qm, _ = make(w).observe(T_STEPS)
+    assert np.all(qm[2:] > 0)            # the overland reverse path is exercised
+    S = w.grid.nx * w.grid.ny
+    rng = np.random.default_rng(7)
+    cells = np.concatenate([rng.choice(S, 4, replace=False),             # top layer
+                            S + rng.choice(S, 2, replace=False)])        # second layer
+    def fd(key, c, log, h):
+        vals = []
+        for sgn in (1, -1):
+            f = w.fields[key].copy()
+            f[c] = np.exp(np.log(f[c]) + sgn * h) if log else f[c] + sgn * h
+            vals.append(_loss(w, targets, **{key: f}))
+        return (vals[0] - vals[1]) / (2 * h)
+
+    strict = 0
+    for key, g, log in (("KSAT", gk, True), ("ALPHA", ga, False), ("NVG", gn, False)):
+        for c in cells:
+            # FP64 solver rounding leaves ~5e-16 of noise in the loss, which is not negligible
+            # against 2 h |g| for small gradients; the spread of two step sizes measures it
+            fd1, fd2 = fd(key, c, log, 1e-6), fd(key, c, log, 1e-5)
+            noise = abs(fd1 - fd2)
+            assert abs(g[c] - fd2) <= 1e-4 * abs(fd2) + 4 * noise, (key, c, g[c], fd1, fd2)
+            strict += abs(g[c] - fd1) <= 1e-4 * abs(fd1)
+    assert strict >= len(cells)        # at least a third within 1e-4 at h = 1e-6 outright
+
+
+def test_zero_misfit_gives_zero_gradient(one_thread):
+    w = small_workload()
+    q, wt = make(w).observe(T_STEPS)
+    loss, gk, ga, gn, _ = make(w).adjoint_run(T_STEPS, checkpoint_every=4, q_target=q, w_target=wt)
+    assert loss == 0.0
+    assert not np.any(gk) and not np.any(ga) and not np.any(gn)
+
+
+def test_checkpoint_interval_and_final_state(one_thread, targets):
+    w = small_workload()
+    q, wt = targets
+    res = []
+    for k in (1, 5, T_STEPS):
+        m = make(w)
+        res.append(m.adjoint_run(T_STEPS, checkpoint_every=k, q_target=q, w_target=wt, w_q=1e6)[:4])
+        if k == 5:
+            ref = make(w)
+            ref.step(T_STEPS)
+            np.testing.assert_array_equal(m.download("PSI"), ref.download("PSI"))
+            np.testing.assert_array_equal(m.download("PONDING"), ref.download("PONDING"))
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        for a, b in zip(r[1:], res[0][1:]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_adjoint_rejects_unsupported_configurations(one_thread):
+    w = small_workload()
+    p = _abi.Params.from_buffer_copy(w.params)
+    p.theta_carry = _abi.THETA_CARRY_REFERENCE
+    m = oracle.OracleModel(w.grid, p)
+    m.upload_many(w.fields)
+    m.init_state(_abi.INIT_FROM_PSI)
+    with pytest.raises(Exception):
+        m.adjoint_run(2)
