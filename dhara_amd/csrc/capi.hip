@@ -471,28 +471,70 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
     return st->nan_seen ? set_err(c, DHARA_E_NUMERIC, "NaN/Inf in the subsurface state") : DHARA_OK;
 }
 
+// Level arrays of the overland multigrid: level 0 shares D2 (diagonal), r2 (right-hand side
+// of the V-cycle) and z2 (its result) with the CG.
+int ensure_mg(dhara_ctx* c)
+{
+    if (c->mg_ready) return DHARA_OK;
+    int dims[dh::kMgMaxLevels][2];
+    const int nl = dh::mg_levels(c->grid.nx, c->grid.ny, dims);
+    if (nl > dh::kMgMaxLevels) return set_err(c, DHARA_E_UNSUPPORTED, "surface grid too large for the multigrid");
+    int rc;
+    if ((rc = dalloc(c, &c->z2, c->S))) return rc;
+    c->mg.nlev = nl;
+    for (int l = 0; l < nl; ++l) {
+        dh::MgLevel& L = c->mg.lv[l];
+        L.nx = dims[l][0];
+        L.ny = dims[l][1];
+        const size_t n = (size_t)L.nx * L.ny;
+        if ((rc = dalloc(c, &L.we, n)) || (rc = dalloc(c, &L.wn, n)) || (rc = dalloc(c, &L.x, n))) return rc;
+        if (l == 0) {
+            L.d = c->D2;
+            L.b = c->r2;
+            L.y = c->z2;
+        } else if ((rc = dalloc(c, &L.d, n)) || (rc = dalloc(c, &L.b, n)) || (rc = dalloc(c, &L.y, n))) {
+            return rc;
+        }
+    }
+    c->mg_ready = true;
+    return DHARA_OK;
+}
+
 int do_overland_step(dhara_ctx* c, dhara_step_stats* st)
 {
+    const bool mg = c->p.olf_precond == DHARA_OLF_PRECOND_MG;
+    if (mg) {
+        int rc = ensure_mg(c);
+        if (rc) return rc;
+    }
     dh::OlfArgs a = olf_args(c);
     dh::launch_olf_faces(c->s, a);
     dh::launch_olf_assemble(c->s, a);
-    LAUNCHCHK(c);
     double* const pp[2] = {c->p2[0], c->p2[1]};
     dh::SolveCtl sc = solve_ctl(c, c->st_olf, c->p.olf_maxiter, dh::grid_for(c->S), false, true);
     sc.lin_tol = c->p.olf_tol;
+    if (mg) {
+        // z0 = M^-1 r0; its kernel writes the [rz, rr, bb, rr] partials the first CG kernel reads
+        dh::launch_mg_setup(c->s, c->g2, c->ke, c->kn, c->mg);
+        dh::launch_mg_vcycle(c->s, c->mg, c->st_olf, c->part[0], c->b2);
+        sc.nparts = dh::mg_partials(c->mg);
+    }
+    LAUNCHCHK(c);
     int rc = run_krylov(
         c, c->st_olf, c->p.olf_maxiter, c->olf_hint, false, 8.0,
         [&](int it, const dh::LaunchEv& ev) {
             dh::SolveCtl k = sc;
             k.it = it;
-            dh::launch_pcg2_spmv(c->s, c->g2, c->ke, c->kn, c->D2, c->r2, pp, c->q2, k);
+            dh::launch_pcg2_spmv(c->s, c->g2, c->ke, c->kn, c->D2, c->r2, pp, c->q2, k, mg ? c->z2 : nullptr);
         },
         [&](int it, const dh::LaunchEv& ev) {
             dh::SolveCtl k = sc;
             k.it = it;
             k.part_in = c->part[1];
+            k.nparts = dh::grid_for(c->S);
             k.part_out = c->part[0];
             dh::launch_pcg_update_f64(c->s, c->S, c->H, c->r2, c->D2, pp[(it + 1) & 1], c->q2, k);
+            if (mg) dh::launch_mg_vcycle(c->s, c->mg, c->st_olf, c->part[0], nullptr);
         });
     if (rc) return rc;
     st->olf_iters = c->h_st->iter;
@@ -1127,6 +1169,24 @@ int dhara_apply_subsurface_operator_f32(dhara_ctx* c, const double* xin, double*
     dh::launch_cast_f64(c->s, c->N, c->p32[1], c->pv[1]);
     LAUNCHCHK(c);
     HIPCHK(c, hipMemcpyAsync(y, c->pv[1], nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return DHARA_OK;
+}
+
+int dhara_overland_precond(dhara_ctx* c, const double* r, double* z)
+{
+    if (!c || !r || !z) return DHARA_E_ARG;
+    int rc = ensure_mg(c);
+    if (rc) return rc;
+    dh::OlfArgs a = olf_args(c);
+    dh::launch_olf_faces(c->s, a);
+    dh::launch_olf_assemble(c->s, a);
+    dh::launch_mg_setup(c->s, c->g2, c->ke, c->kn, c->mg);
+    const size_t nb = (size_t)c->S * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->r2, r, nb, hipMemcpyHostToDevice, c->s));
+    dh::launch_mg_vcycle(c->s, c->mg, nullptr, nullptr, nullptr);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(z, c->z2, nb, hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return DHARA_OK;
 }

@@ -43,6 +43,10 @@ struct dhara_ctx {
     // overland solver
     double *D2 = nullptr, *b2 = nullptr, *r2 = nullptr, *p2[2] = {nullptr, nullptr}, *q2 = nullptr;
     double *ke = nullptr, *kn = nullptr;
+    // overland multigrid hierarchy (allocated on first use) and the preconditioned residual
+    dh::MgHier mg{};
+    bool mg_ready = false;
+    double* z2 = nullptr;
     // reductions: partial buffers A (written by assembly/update kernels) and B (SpMV kernels)
     double* part[2] = {nullptr, nullptr};
     dh::PcgState* st_sub = nullptr;
@@ -111,6 +115,7 @@ dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bo
                        bool first_sweep);
 int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st);
 int do_overland_step(dhara_ctx* c, dhara_step_stats* st);
+int ensure_mg(dhara_ctx* c);   // allocates the overland multigrid hierarchy
 void free_adjoint(dhara_ctx* c);   // adjoint.hip
 
 // Queue Krylov iterations in batches; the kernels stop doing work once the device-side

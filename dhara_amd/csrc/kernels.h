@@ -203,9 +203,31 @@ void launch_cast_f64(hipStream_t s, int n, const float* in, double* out);
 // Overland
 void launch_olf_faces(hipStream_t s, const OlfArgs& a);
 void launch_olf_assemble(hipStream_t s, const OlfArgs& a);
+// z = NULL: Jacobi (p = r/D + beta p); otherwise p = z + beta p with z the preconditioned residual.
 void launch_pcg2_spmv(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
                       const double* D, const double* r, double* const p[2], double* q,
-                      const SolveCtl& sc);
+                      const SolveCtl& sc, const double* z = nullptr);
 void launch_olf_post(hipStream_t s, const OlfArgs& a);
+
+// ---- overland multigrid preconditioner (olf_mg.hip) ---------------------------------------
+// One level of the 2 x 2 aggregation hierarchy: operator A = d - weights (we: cell to its east
+// neighbour, wn: to its north neighbour), right-hand side b, pre-smoothed x, V-cycle result y.
+struct MgLevel {
+    int nx, ny;
+    double *d, *we, *wn, *b, *x, *y;
+};
+constexpr int kMgMaxLevels = 24;
+struct MgHier {
+    int nlev;
+    MgLevel lv[kMgMaxLevels];
+};
+int mg_levels(int nx, int ny, int dims[][2]);   // level sizes down to 1 x 1; returns the count
+int mg_coarse_level(const MgHier& h);           // first level handled by the single-block kernel
+int mg_partials(const MgHier& h);               // partial sums the V-cycle's last kernel writes
+// Level-0 weights from the face conductances and the Galerkin operators of all coarse levels.
+void launch_mg_setup(hipStream_t s, const Geo2& g, const double* ke, const double* kn, const MgHier& h);
+// y_0 = V-cycle(b_0); with `part`, partial sums [(b0, y0), (b0, b0), (bb_src, bb_src), (b0, b0)].
+// Every kernel returns at once when st->done is set (st may be NULL).
+void launch_mg_vcycle(hipStream_t s, const MgHier& h, const PcgState* st, double* part, const double* bb_src);
 
 }  // namespace dh

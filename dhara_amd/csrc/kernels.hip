@@ -763,19 +763,21 @@ __global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
     block_partials<4, false>(acc, a.part_out);
 }
 
+// PRECZ: p = z + beta p with z the preconditioned residual (multigrid); else z = r / D.
+template <bool PRECZ>
 __global__ __launch_bounds__(kBlock) void k_pcg2_spmv(Geo2 g, const double* __restrict__ ke,
                                                       const double* __restrict__ kn,
                                                       const double* __restrict__ D,
                                                       const double* __restrict__ r,
                                                       const double* __restrict__ pold,
                                                       double* __restrict__ pnew, double* __restrict__ q,
-                                                      SolveCtl sc)
+                                                      SolveCtl sc, const double* __restrict__ zp)
 {
     double beta;
     if (!pcg_begin(sc, beta)) return;
     const bool first = (sc.it == 0);
     auto pf = [&](int c) -> double {
-        const double z = r[c] / D[c];
+        const double z = PRECZ ? zp[c] : r[c] / D[c];
         return first ? z : z + beta * pold[c];
     };
     double acc[1] = {0.0};
@@ -1027,11 +1029,13 @@ void launch_olf_assemble(hipStream_t s, const OlfArgs& a)
 
 void launch_pcg2_spmv(hipStream_t s, const Geo2& g, const double* ke, const double* kn,
                       const double* D, const double* r, double* const p[2], double* q,
-                      const SolveCtl& sc)
+                      const SolveCtl& sc, const double* z)
 {
     const double* pold = p[sc.it & 1];
     double* pnew = p[(sc.it + 1) & 1];
-    hipLaunchKernelGGL(k_pcg2_spmv, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, ke, kn, D, r, pold, pnew, q, sc);
+    const dim3 gr(grid_for(g.n)), bl(kBlock);
+    if (z) hipLaunchKernelGGL(k_pcg2_spmv<true>, gr, bl, 0, s, g, ke, kn, D, r, pold, pnew, q, sc, z);
+    else hipLaunchKernelGGL(k_pcg2_spmv<false>, gr, bl, 0, s, g, ke, kn, D, r, pold, pnew, q, sc, z);
 }
 
 void launch_olf_post(hipStream_t s, const OlfArgs& a)

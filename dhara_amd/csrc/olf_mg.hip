@@ -1,0 +1,331 @@
+// olf_mg.hip -- multigrid preconditioner of the overland solve (gfx950).
+//
+// The overland system of one step (src/overland.cu:65-189, Dirichlet ring eliminated) is a
+// symmetric 5-point operator A = d - (east/north face weights) whose coefficients jump by six
+// orders of magnitude between dry faces (K0) and ponded ones; Jacobi-preconditioned CG needs
+// ~1500 iterations per step on a 400 x 400 surface.  This preconditioner is one V-cycle of
+// unsmoothed-aggregation multigrid:
+//   * levels: 2 x 2 aggregates down to 1 x 1, Galerkin coarse operators P^T A P with P
+//     piecewise constant -- on a structured grid these stay 5-point (d, we, wn), so every
+//     level uses the same kernels;
+//   * smoothing: two red-black Gauss-Seidel sweeps before (red first) and after (black first)
+//     the coarse correction, which keeps the V-cycle symmetric, so CG stays valid;
+//   * coarse correction scaled by 1.5 (over-correction for piecewise-constant interpolation);
+//   * large levels: one tiled kernel down (smooth, residual, restrict) and one up (prolong,
+//     smooth) per level; a tile of T x T cells works in LDS with a halo of 5 (down) or 4 (up)
+//     cells recomputed redundantly, so four half-sweeps need no grid-wide synchronisation and
+//     the tile interior equals a global sweep exactly;
+//   * levels of at most 32 x 32 cells: the rest of the V-cycle in one single-block kernel.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace dh {
+
+constexpr double kMgOver = 1.5;   // coarse-correction scaling
+constexpr int kMgT = 28;          // tile edge of the large-level kernels (even)
+constexpr int kMgHd = 5;          // halo: 4 half-sweeps + the residual
+constexpr int kMgHu = 4;          // halo: 4 half-sweeps
+constexpr int kMgThreads = 256;
+constexpr int kMgCoarseCells = 1366;   // cells of all levels from 32 x 32 down to 1 x 1
+
+namespace {
+
+__device__ __forceinline__ int mg_coarse_first(const MgHier& h)
+{
+    int l = 0;
+    while (l < h.nlev - 1 && !(h.lv[l].nx <= 32 && h.lv[l].ny <= 32)) ++l;
+    return l;
+}
+
+// Level-0 operator: the eliminated overland matrix.  Diagonal = the assembled D (ring rows 1);
+// weights only between two interior cells (couplings to the ring moved to the right-hand side).
+__global__ __launch_bounds__(kBlock) void k_mg_fine(Geo2 g, const double* ke, const double* kn, double* we,
+                                                    double* wn)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        const int j = c / g.nx, i = c - j * g.nx;
+        const bool in = i > 0 && j > 0 && i < g.nx - 1 && j < g.ny - 1;
+        we[c] = (in && i + 1 < g.nx - 1) ? g.cx * ke[c] : 0.0;
+        wn[c] = (in && j + 1 < g.ny - 1) ? g.cy * kn[c] : 0.0;
+    }
+}
+
+// Galerkin coarse operator of 2 x 2 aggregates: d_C = sum d_f - 2 (couplings inside C),
+// weight between C and its east/north neighbour = sum of the fine couplings that cross.
+__global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel F, MgLevel C)
+{
+    for (int cc = blockIdx.x * blockDim.x + threadIdx.x; cc < C.nx * C.ny; cc += gridDim.x * blockDim.x) {
+        const int J = cc / C.nx, I = cc - J * C.nx;
+        const int i0 = 2 * I, j0 = 2 * J;
+        const bool i1 = i0 + 1 < F.nx, j1 = j0 + 1 < F.ny;
+        double d = 0.0, e = 0.0, n = 0.0;
+        for (int dj = 0; dj < (j1 ? 2 : 1); ++dj)
+            for (int di = 0; di < (i1 ? 2 : 1); ++di) {
+                const int f = (j0 + dj) * F.nx + i0 + di;
+                d += F.d[f];
+                if (di == 0 && i1) d -= 2.0 * F.we[f];        // (i0, i0+1) inside C
+                if (di == 1 || !i1) e += F.we[f];             // crosses to the east aggregate
+                if (dj == 0 && j1) d -= 2.0 * F.wn[f];
+                if (dj == 1 || !j1) n += F.wn[f];
+            }
+        C.d[cc] = d;
+        C.we[cc] = e;
+        C.wn[cc] = n;
+    }
+}
+
+// One red-black Gauss-Seidel half-sweep over an LDS grid of w x w cells (global origin ox, oy),
+// updating the cells of `color` ((gi + gj) & 1) that are not on the LDS border.
+__device__ __forceinline__ void lds_half_sweep(int w, int ox, int oy, int color, const double* sb, const double* sd,
+                                               const double* swe, const double* swn, double* sx)
+{
+    for (int t = threadIdx.x; t < w * w; t += blockDim.x) {
+        const int lj = t / w, li = t - lj * w;
+        if (li == 0 || lj == 0 || li == w - 1 || lj == w - 1) continue;
+        if (((ox + li + oy + lj) & 1) != color) continue;
+        const double s = sb[t] + swe[t] * sx[t + 1] + swe[t - 1] * sx[t - 1] + swn[t] * sx[t + w] + swn[t - w] * sx[t - w];
+        sx[t] = s / sd[t];
+    }
+    __syncthreads();
+}
+
+// Down: x = 2 forward RBGS sweeps from 0 on b, r = b - A x, b_coarse = P^T r.
+__global__ __launch_bounds__(kMgThreads) void k_mg_down(MgHier h, int l, const PcgState* st)
+{
+    if (st && st->done) return;
+    constexpr int W = kMgT + 2 * kMgHd;
+    __shared__ double sb[W * W], sd[W * W], swe[W * W], swn[W * W], sx[W * W], sr[kMgT * kMgT];
+    const MgLevel L = h.lv[l], C = h.lv[l + 1];
+    const int ntx = (L.nx + kMgT - 1) / kMgT;
+    const int gx0 = (blockIdx.x % ntx) * kMgT, gy0 = (blockIdx.x / ntx) * kMgT;
+    const int ox = gx0 - kMgHd, oy = gy0 - kMgHd;
+    for (int t = threadIdx.x; t < W * W; t += blockDim.x) {
+        const int lj = t / W, li = t - lj * W;
+        const int gi = ox + li, gj = oy + lj;
+        const bool in = gi >= 0 && gj >= 0 && gi < L.nx && gj < L.ny;
+        const int g = gj * L.nx + gi;
+        sb[t] = in ? L.b[g] : 0.0;
+        sd[t] = in ? L.d[g] : 1.0;
+        swe[t] = in ? L.we[g] : 0.0;
+        swn[t] = in ? L.wn[g] : 0.0;
+        sx[t] = 0.0;
+    }
+    __syncthreads();
+    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
+    // residual of the tile interior (exact: the halo absorbed the four half-sweeps)
+    for (int t = threadIdx.x; t < kMgT * kMgT; t += blockDim.x) {
+        const int tj = t / kMgT, ti = t - tj * kMgT;
+        const int q = (tj + kMgHd) * W + ti + kMgHd;
+        const double Ax = sd[q] * sx[q] - swe[q] * sx[q + 1] - swe[q - 1] * sx[q - 1] - swn[q] * sx[q + W] -
+                          swn[q - W] * sx[q - W];
+        const int gi = gx0 + ti, gj = gy0 + tj;
+        if (gi < L.nx && gj < L.ny) L.x[gj * L.nx + gi] = sx[q];
+        sr[t] = sb[q] - Ax;
+    }
+    __syncthreads();
+    constexpr int TC = kMgT / 2;
+    for (int t = threadIdx.x; t < TC * TC; t += blockDim.x) {
+        const int cj = t / TC, ci = t - cj * TC;
+        const int I = gx0 / 2 + ci, J = gy0 / 2 + cj;
+        if (I >= C.nx || J >= C.ny) continue;
+        const int q = 2 * cj * kMgT + 2 * ci;
+        C.b[J * C.nx + I] = sr[q] + sr[q + 1] + sr[q + kMgT] + sr[q + kMgT + 1];
+    }
+}
+
+// Up: x = x_l + 1.5 P y_{l+1}, 2 backward RBGS sweeps, written to out.  On level 0 also the
+// partial sums (b, x), (b, b), (bb_src, bb_src), (b, b) the CG kernels read.
+__global__ __launch_bounds__(kMgThreads) void k_mg_up(MgHier h, int l, double* out, const PcgState* st,
+                                                      double* part, const double* bb_src)
+{
+    if (st && st->done) return;
+    constexpr int W = kMgT + 2 * kMgHu;
+    __shared__ double sb[W * W], sd[W * W], swe[W * W], swn[W * W], sx[W * W];
+    const MgLevel L = h.lv[l], C = h.lv[l + 1];
+    const int ntx = (L.nx + kMgT - 1) / kMgT;
+    const int gx0 = (blockIdx.x % ntx) * kMgT, gy0 = (blockIdx.x / ntx) * kMgT;
+    const int ox = gx0 - kMgHu, oy = gy0 - kMgHu;
+    for (int t = threadIdx.x; t < W * W; t += blockDim.x) {
+        const int lj = t / W, li = t - lj * W;
+        const int gi = ox + li, gj = oy + lj;
+        const bool in = gi >= 0 && gj >= 0 && gi < L.nx && gj < L.ny;
+        const int g = gj * L.nx + gi;
+        sb[t] = in ? L.b[g] : 0.0;
+        sd[t] = in ? L.d[g] : 1.0;
+        swe[t] = in ? L.we[g] : 0.0;
+        swn[t] = in ? L.wn[g] : 0.0;
+        sx[t] = in ? L.x[g] + kMgOver * C.y[(gj >> 1) * C.nx + (gi >> 1)] : 0.0;
+    }
+    __syncthreads();
+    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < kMgT * kMgT; t += blockDim.x) {
+        const int tj = t / kMgT, ti = t - tj * kMgT;
+        const int gi = gx0 + ti, gj = gy0 + tj;
+        if (gi >= L.nx || gj >= L.ny) continue;
+        const int q = (tj + kMgHu) * W + ti + kMgHu;
+        const int g = gj * L.nx + gi;
+        out[g] = sx[q];
+        acc[0] += sb[q] * sx[q];
+        acc[1] += sb[q] * sb[q];
+        if (bb_src) acc[2] += bb_src[g] * bb_src[g];
+    }
+    acc[3] = acc[1];
+    if (part) block_partials<4, false>(acc, part);
+}
+
+// The V-cycle from level lc (<= 32 x 32) down to 1 x 1 and back, in one block, in LDS:
+// reads b of level lc, writes y of level lc (and on level 0 the CG partial sums).
+__global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgState* st, double* part,
+                                                          const double* bb_src)
+{
+    if (st && st->done) return;
+    __shared__ double sb[kMgCoarseCells], sx[kMgCoarseCells], sd[kMgCoarseCells], swe[kMgCoarseCells],
+        swn[kMgCoarseCells], sr[1024];
+    const int lc = mg_coarse_first(h);
+    const int nl = h.nlev;
+    int off[kMgMaxLevels];
+    {
+        int o = 0;
+        for (int l = lc; l < nl; ++l) { off[l] = o; o += h.lv[l].nx * h.lv[l].ny; }
+    }
+    for (int l = lc; l < nl; ++l) {
+        const MgLevel& L = h.lv[l];
+        const int n = L.nx * L.ny;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            sd[off[l] + t] = L.d[t];
+            swe[off[l] + t] = L.we[t];
+            swn[off[l] + t] = L.wn[t];
+            sb[off[l] + t] = (l == lc) ? L.b[t] : 0.0;
+            sx[off[l] + t] = 0.0;
+        }
+    }
+    __syncthreads();
+    auto half = [&](int l, int color) {
+        const int nx = h.lv[l].nx, ny = h.lv[l].ny, o = off[l];
+        for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
+            const int j = t / nx, i = t - j * nx;
+            if (((i + j) & 1) != color) continue;
+            double s = sb[o + t];
+            if (i + 1 < nx) s += swe[o + t] * sx[o + t + 1];
+            if (i > 0) s += swe[o + t - 1] * sx[o + t - 1];
+            if (j + 1 < ny) s += swn[o + t] * sx[o + t + nx];
+            if (j > 0) s += swn[o + t - nx] * sx[o + t - nx];
+            sx[o + t] = s / sd[o + t];
+        }
+        __syncthreads();
+    };
+    for (int l = lc; l < nl - 1; ++l) {
+        half(l, 0); half(l, 1); half(l, 0); half(l, 1);
+        const int nx = h.lv[l].nx, ny = h.lv[l].ny, o = off[l];
+        for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
+            const int j = t / nx, i = t - j * nx;
+            double Ax = sd[o + t] * sx[o + t];
+            if (i + 1 < nx) Ax -= swe[o + t] * sx[o + t + 1];
+            if (i > 0) Ax -= swe[o + t - 1] * sx[o + t - 1];
+            if (j + 1 < ny) Ax -= swn[o + t] * sx[o + t + nx];
+            if (j > 0) Ax -= swn[o + t - nx] * sx[o + t - nx];
+            sr[t] = sb[o + t] - Ax;
+        }
+        __syncthreads();
+        const int cnx = h.lv[l + 1].nx, cny = h.lv[l + 1].ny, co = off[l + 1];
+        for (int t = threadIdx.x; t < cnx * cny; t += blockDim.x) {
+            const int J = t / cnx, I = t - J * cnx;
+            double s = 0.0;
+            for (int dj = 0; dj < 2; ++dj)
+                for (int di = 0; di < 2; ++di) {
+                    const int i = 2 * I + di, j = 2 * J + dj;
+                    if (i < nx && j < ny) s += sr[j * nx + i];
+                }
+            sb[co + t] = s;
+        }
+        __syncthreads();
+    }
+    {
+        const int o = off[nl - 1], n = h.lv[nl - 1].nx * h.lv[nl - 1].ny;   // 1 x 1: exact
+        for (int t = threadIdx.x; t < n; t += blockDim.x) sx[o + t] = sb[o + t] / sd[o + t];
+        __syncthreads();
+    }
+    for (int l = nl - 2; l >= lc; --l) {
+        const int nx = h.lv[l].nx, ny = h.lv[l].ny, o = off[l];
+        const int cnx = h.lv[l + 1].nx, co = off[l + 1];
+        for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
+            const int j = t / nx, i = t - j * nx;
+            sx[o + t] += kMgOver * sx[co + (j >> 1) * cnx + (i >> 1)];
+        }
+        __syncthreads();
+        half(l, 1); half(l, 0); half(l, 1); half(l, 0);
+    }
+    const MgLevel& L = h.lv[lc];
+    const int n = L.nx * L.ny, o = off[lc];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        L.y[t] = sx[o + t];
+        acc[0] += sb[o + t] * sx[o + t];
+        acc[1] += sb[o + t] * sb[o + t];
+        if (bb_src) acc[2] += bb_src[t] * bb_src[t];
+    }
+    acc[3] = acc[1];
+    if (part) block_partials<4, false>(acc, part);
+}
+
+inline int mg_tiles(const MgLevel& L) { return ((L.nx + kMgT - 1) / kMgT) * ((L.ny + kMgT - 1) / kMgT); }
+
+}  // namespace
+
+int mg_levels(int nx, int ny, int dims[][2])
+{
+    int n = 0;
+    for (;;) {
+        dims[n][0] = nx;
+        dims[n][1] = ny;
+        ++n;
+        if (nx == 1 && ny == 1) break;
+        nx = (nx + 1) / 2;
+        ny = (ny + 1) / 2;
+    }
+    return n;
+}
+
+int mg_coarse_level(const MgHier& h)
+{
+    int l = 0;
+    while (l < h.nlev - 1 && !(h.lv[l].nx <= 32 && h.lv[l].ny <= 32)) ++l;
+    return l;
+}
+
+int mg_partials(const MgHier& h)
+{
+    const int lc = mg_coarse_level(h);
+    return lc == 0 ? 1 : mg_tiles(h.lv[0]);
+}
+
+void launch_mg_setup(hipStream_t s, const Geo2& g, const double* ke, const double* kn, const MgHier& h)
+{
+    hipLaunchKernelGGL(k_mg_fine, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, ke, kn, h.lv[0].we, h.lv[0].wn);
+    for (int l = 0; l + 1 < h.nlev; ++l) {
+        const int nc = h.lv[l + 1].nx * h.lv[l + 1].ny;
+        hipLaunchKernelGGL(k_mg_coarsen, dim3(grid_for(nc)), dim3(kBlock), 0, s, h.lv[l], h.lv[l + 1]);
+    }
+}
+
+void launch_mg_vcycle(hipStream_t s, const MgHier& h, const PcgState* st, double* part, const double* bb_src)
+{
+    const int lc = mg_coarse_level(h);
+    for (int l = 0; l < lc; ++l)
+        hipLaunchKernelGGL(k_mg_down, dim3(mg_tiles(h.lv[l])), dim3(kMgThreads), 0, s, h, l, st);
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kMgThreads), 0, s, h, st, lc == 0 ? part : nullptr,
+                       lc == 0 ? bb_src : nullptr);
+    for (int l = lc - 1; l >= 0; --l)
+        hipLaunchKernelGGL(k_mg_up, dim3(mg_tiles(h.lv[l])), dim3(kMgThreads), 0, s, h, l, h.lv[l].y, st,
+                           l == 0 ? part : nullptr, l == 0 ? bb_src : nullptr);
+}
+
+}  // namespace dh

@@ -234,6 +234,8 @@ def load_hip_library(path=HIP_LIB_PATH):
         lib.dhara_apply_subsurface_operator_f32.restype = C.c_int
         lib.dhara_overland_conductance.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         lib.dhara_overland_conductance.restype = C.c_int
+        lib.dhara_overland_precond.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        lib.dhara_overland_precond.restype = C.c_int
         lib.dhara_profile_enable.argtypes = [C.c_void_p, C.c_int32]
         lib.dhara_profile_enable.restype = C.c_int
         lib.dhara_profile_read.argtypes = [C.c_void_p, P(_abi.KernelProfile)]
@@ -292,6 +294,13 @@ class FlowModel(BaseModel):
         self._check(self._lib.dhara_overland_conductance(self._h, _dptr(ke), _dptr(kn)),
                     "overland_conductance")
         return ke, kn
+
+    def overland_precond(self, r):
+        """One V-cycle of the overland multigrid preconditioner at the current ponding depth."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        self._check(self._lib.dhara_overland_precond(self._h, _dptr(r), _dptr(z)), "overland_precond")
+        return z
 
     def set_tile(self, elem_bytes, tx, ty=0, kc=0):
         """Tile shape of the subsurface SpMV for FP64 (8) or FP32 (4) solves; tx = 0: untiled."""

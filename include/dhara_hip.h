@@ -67,6 +67,11 @@ extern "C" {
 #define DHARA_TOP_PRESCRIBED 1   /* use the uploaded DHARA_F_BCTYPE_T/BCPSI_T/BCQ_T; the
                                     ponding depth is left unchanged                       */
 
+/* Preconditioner of the overland CG (the reference's CUSP CG had none, src/overland.cu:410) */
+#define DHARA_OLF_PRECOND_JACOBI 0
+#define DHARA_OLF_PRECOND_MG     1   /* aggregation-multigrid V-cycle (same solution, far fewer
+                                        iterations on ponded surfaces)                         */
+
 /* State initialisation. */
 #define DHARA_INIT_FROM_THETA 0   /* psi <- vG^-1(theta), then theta,K,C <- vG(psi)
                                      (src/device.cu:323-330, src/global.cu:289)          */
@@ -143,7 +148,7 @@ typedef struct dhara_params {
                                                      inner Dirichlet rows
                                                      (src/subsurface.cu:223,259,294,328) */
     int32_t top_bc;                               /* DHARA_TOP_*                          */
-    int32_t reserved0;
+    int32_t olf_precond;                          /* DHARA_OLF_PRECOND_* (overland CG)    */
 } dhara_params;
 
 typedef struct dhara_step_stats {
@@ -195,7 +200,7 @@ static inline void dhara_params_defaults(dhara_params* p)
     p->theta_carry = DHARA_THETA_CARRY_CONSISTENT;
     p->dirichlet_face_quirk = 1;
     p->top_bc = DHARA_TOP_SURFACE;
-    p->reserved0 = 0;
+    p->olf_precond = DHARA_OLF_PRECOND_MG;
 }
 
 /* ---- lifecycle ------------------------------------------------------------------------ */
@@ -287,6 +292,11 @@ int dhara_apply_subsurface_operator_f32(dhara_ctx* ctx, const double* x, double*
  * of each cell, kn[nx*ny] the north face (UpdateKOverland, src/overland.cu:193-368, seen
  * from the non-ring cell of the face). */
 int dhara_overland_conductance(dhara_ctx* ctx, double* ke, double* kn);
+/* z = M^-1 r for the overland system at the current ponding depth, M^-1 one V-cycle of the
+ * overland multigrid preconditioner (no reference counterpart: the reference's CG was
+ * unpreconditioned, src/overland.cu:410).  r and z: nx*ny host doubles; r must be zero on the
+ * outer ring (the Dirichlet rows). */
+int dhara_overland_precond(dhara_ctx* ctx, const double* r, double* z);
 
 /* ---- discrete adjoint ------------------------------------------------------------------
  * No reference counterpart (the reference has no sensitivities).  The adjoint is that of the
