@@ -50,16 +50,20 @@ def parse():
     return ap.parse_args()
 
 
-def counted_bytes(stats, n3, n2, mixed):
+def counted_bytes(stats, n3, n2, mixed, olf_mg=True):
     """Algorithmic HBM bytes of the steps (per-kernel byte model of the solver)."""
     w = 4 if mixed else 8
+    # overland CG iteration per surface cell: SpMV 6 and update 7 arrays; with the multigrid,
+    # one V-cycle more: level 0 down (read b, d, we, wn; write x, 1/4 coarse b) and up (read
+    # b, d, we, wn, x, 1/4 coarse y; write y), x 4/3 for the coarser levels
+    olf_iter = 13 * 8 + (int(round((4 + 1.25 + 5.25 + 1) * 8 * 4 / 3)) if olf_mg else 0)
     total = 0.0
     for s in stats:
         P, I, R, Io = s["picard_iters"], s["lin_iters"], s["refine_sweeps"], s["olf_iters"]
         vg = 56 + (4 if mixed else 0)                  # read x, psi_m, Ks; write theta, K, C, psi_m
         asm = 80 if not mixed else 84                  # read 7 fields; write D, b, r (or D, b, D32, r32, d32)
         per_cell = P * (vg + asm) + I * 13 * w + R * 64 + 48
-        per_surf = 13 * 8 * Io + 200                   # overland PCG + surface/overland passes
+        per_surf = olf_iter * Io + 200                 # overland CG + surface/overland passes
         total += n3 * per_cell + n2 * per_surf
     return total
 

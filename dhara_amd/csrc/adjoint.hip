@@ -13,6 +13,8 @@
 //   ponding   h' = max(hp - q dt, 0)
 //   overland  H = h' + z, face conductances k_f(H, h', n_manning), one solve A2 H^{n+1} = H
 //             with the ring fixed, h^{n+1} = max(H^{n+1} - z, 0), outlet discharge Q.
+// The Picard relaxation am changes only how the fixed point is reached, so the adjoint of the
+// converged step does not depend on it.
 // Reverse step: J^T lambda = dL/dpsi^{n+1} (J = dF/dpsi^{n+1}: BiCGStab with Jacobi
 // preconditioning on the transposed 7-point Jacobian stencil), A2 mu = dL/dH^{n+1} (A2 is
 // symmetric after ring elimination, so the forward's own CG kernels solve it), then
@@ -807,7 +809,6 @@ int check_adjoint_config(dhara_ctx* c)
 {
     if (!c->initialized) return set_err(c, DHARA_E_STATE, "dhara_init_state not called after a state upload");
     if (c->p.precision != DHARA_FP64) return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs FP64 precision");
-    if (c->p.am != 0.0) return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs am = 0");
     if (c->p.theta_carry != DHARA_THETA_CARRY_CONSISTENT)
         return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs the consistent theta carry");
     if (has_dirichlet(c)) return set_err(c, DHARA_E_UNSUPPORTED, "the adjoint needs flux faces");

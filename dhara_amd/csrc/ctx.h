@@ -111,6 +111,9 @@ bool has_dirichlet(const dhara_ctx* c);
 dh::Soil soil_of(const dhara_ctx* c);
 dh::VgArgs vg_args(dhara_ctx* c, const double* psi, double* theta);
 dh::OlfArgs olf_args(dhara_ctx* c);
+dh::SurfArgs surf_args(dhara_ctx* c);
+dh::AsmArgs asm_args(dhara_ctx* c, bool dir);
+int alloc_mixed(dhara_ctx* c);
 dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bool mixed_inner,
                        bool first_sweep);
 int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st);

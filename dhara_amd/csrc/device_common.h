@@ -117,17 +117,30 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* lds)
     }
 }
 
-// Producer side: reduce NV values over the block and store them at part[i*kMaxBlocks + block].
+// The block a kernel body works as: its index among the blocks of the launch and their
+// number.  A single-model launch uses the hardware values; a batched launch gives every model
+// its own share of blocks (id 0..n-1 within that model).
+struct Blk {
+    int id, n;
+};
+__device__ __forceinline__ Blk hw_blk() { return Blk{lin_bid(), (int)(gridDim.x * gridDim.y)}; }
+
+// Producer side: reduce NV values over the block and store them at part[i*kMaxBlocks + bid].
 template <int NV, bool MAX>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* part, int bid)
 {
     __shared__ double lds[NV * 16];
     block_reduce<NV, MAX>(v, lds);
     if (lin_tid() == 0) {
-        const int bid = lin_bid();
 #pragma unroll
         for (int i = 0; i < NV; ++i) part[i * kMaxBlocks + bid] = v[i];
     }
+}
+
+template <int NV, bool MAX>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
+{
+    block_partials<NV, MAX>(v, part, lin_bid());
 }
 
 // Consumer side: the first nv (<= NV) totals of the nparts partials of the previous kernel,

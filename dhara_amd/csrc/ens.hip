@@ -1,0 +1,553 @@
+// ens.hip -- a batched ensemble: many independent models of uneven size stepped together.
+//
+// No reference counterpart (the reference runs one model per process).  The members are
+// ordinary contexts (created, filled and initialised through the C-ABI); the ensemble steps
+// all of them at once on its own stream:
+//   * every kernel is launched once for all members: a block map sends physical block b to
+//     block map[b].id of model map[b].m, which reads that model's argument struct, so one
+//     launch covers ~sum N cells (>> 256 workgroups even when each member is small);
+//   * members run in lockstep through the step -- surface pass, Picard iterations, Krylov
+//     solves, end of step, overland solve -- and a member that has converged drops out through
+//     its active flag (its blocks return at once); every member keeps its own Picard and
+//     Krylov stopping decisions, so each member's result is its own, not a batch average;
+//   * a batch of Krylov iterations (SpMV + update kernels for every member) is captured once
+//     as a hipGraph and replayed; the kernels take the iteration index from a device counter
+//     that the graph's last node advances, so every replay is the same graph;
+//   * per-member reductions go through one kernel writing all members' sums, one copy to the
+//     host; the members' solver states sit in one array, one copy per batch.
+// Mixed precision here is FP32 Jacobi-PCG with FP64 iterative refinement that restarts the
+// FP32 recurrence after each refinement, so all members of a round share the iteration index.
+// The overland solve uses Jacobi-PCG (the multigrid preconditioner is per model).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace dhc;
+
+struct dhara_ensemble {
+    int M = 0;
+    std::vector<dhara_ctx*> mem;
+    hipStream_t s = nullptr;
+    bool mixed = false, use_graph = true, overland = false;
+    std::string err;
+    std::vector<void*> allocs;
+    dh::BlkMap *map3 = nullptr, *map2 = nullptr;
+    int nb3 = 0, nb2 = 0;
+    int* active = nullptr;        // device, M
+    int* h_active = nullptr;      // pinned
+    int* it_base = nullptr;
+    dh::PcgState* st = nullptr;   // device, 2M: subsurface then overland
+    dh::PcgState* h_st = nullptr; // pinned
+    double** parts = nullptr;     // device, M (each member's partial buffer A)
+    int *np3 = nullptr, *np2 = nullptr;
+    double* red = nullptr;        // device, M * kMaxSums
+    double* h_red = nullptr;      // pinned
+    // argument arrays (device) and their host images
+    dh::VgArgs* a_vg = nullptr;
+    dh::SurfArgs* a_surf = nullptr;
+    dh::AsmArgs* a_asm = nullptr;
+    dh::RefineArgs* a_ref = nullptr;
+    dh::EndArgs* a_end = nullptr;
+    dh::OlfArgs* a_olf = nullptr;
+    void* a_k3 = nullptr;
+    dh::KrylovArgs<double>* a_k2 = nullptr;
+    std::vector<dh::VgArgs> h_vg;
+    std::vector<dh::SurfArgs> h_surf;
+    std::vector<dh::AsmArgs> h_asm;
+    std::vector<dh::RefineArgs> h_ref;
+    std::vector<dh::EndArgs> h_end;
+    std::vector<dh::OlfArgs> h_olf;
+    std::vector<dh::KrylovArgs<double>> h_k3d, h_k2;
+    std::vector<dh::KrylovArgs<float>> h_k3f;
+    hipGraphExec_t graph3 = nullptr, graph2 = nullptr;
+    int batch = 8;
+};
+
+namespace {
+
+int eset(dhara_ensemble* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    return code;
+}
+
+#define EHIP(e, call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return eset((e), DHARA_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int ealloc(dhara_ensemble* e, T** p, size_t n)
+{
+    void* v = nullptr;
+    if (hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return eset(e, DHARA_E_ALLOC, "hipMalloc");
+    e->allocs.push_back(v);
+    *p = static_cast<T*>(v);
+    return DHARA_OK;
+}
+
+template <typename T>
+int upload(dhara_ensemble* e, T* dev, const std::vector<T>& host)
+{
+    EHIP(e, hipMemcpyAsync(dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, e->s));
+    EHIP(e, hipStreamSynchronize(e->s));   // the host image may change after this
+    return DHARA_OK;
+}
+
+int set_active(dhara_ensemble* e, const std::vector<int>& on)
+{
+    std::copy(on.begin(), on.end(), e->h_active);
+    EHIP(e, hipMemcpyAsync(e->active, e->h_active, e->M * sizeof(int), hipMemcpyHostToDevice, e->s));
+    EHIP(e, hipStreamSynchronize(e->s));
+    return DHARA_OK;
+}
+
+int read_states(dhara_ensemble* e)
+{
+    EHIP(e, hipMemcpyAsync(e->h_st, e->st, 2 * e->M * sizeof(dh::PcgState), hipMemcpyDeviceToHost, e->s));
+    EHIP(e, hipStreamSynchronize(e->s));
+    return DHARA_OK;
+}
+
+int reduce(dhara_ensemble* e, bool two_d, int nv, bool take_max)
+{
+    dh::launch_reduce_models(e->s, e->M, e->parts, two_d ? e->np2 : e->np3, nv, take_max, e->red);
+    EHIP(e, hipGetLastError());
+    EHIP(e, hipMemcpyAsync(e->h_red, e->red, (size_t)e->M * nv * sizeof(double), hipMemcpyDeviceToHost, e->s));
+    EHIP(e, hipStreamSynchronize(e->s));
+    return DHARA_OK;
+}
+
+// One batch of Krylov iterations for every member whose active flag is set.
+void krylov_batch(dhara_ensemble* e, bool two_d)
+{
+    for (int k = 0; k < e->batch; ++k) {
+        if (two_d) {
+            dh::launch_batched_spmv2(e->s, e->a_k2, e->map2, e->nb2, e->active, e->it_base, k);
+            dh::launch_batched_update2(e->s, e->a_k2, e->map2, e->nb2, e->active, e->it_base, k);
+        } else {
+            dh::launch_batched_spmv3(e->s, e->mixed, e->a_k3, e->map3, e->nb3, e->active, e->it_base, k);
+            dh::launch_batched_update3(e->s, e->mixed, e->a_k3, e->map3, e->nb3, e->active, e->it_base, k);
+        }
+    }
+    dh::launch_advance(e->s, e->it_base, e->batch);
+}
+
+int ensure_graph(dhara_ensemble* e, bool two_d)
+{
+    hipGraphExec_t& ex = two_d ? e->graph2 : e->graph3;
+    if (ex) return DHARA_OK;
+    hipGraph_t g = nullptr;
+    EHIP(e, hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+    krylov_batch(e, two_d);
+    EHIP(e, hipStreamEndCapture(e->s, &g));
+    EHIP(e, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    return DHARA_OK;
+}
+
+// Krylov iterations until every active member's solve is done.  `which` selects the state
+// block (0 subsurface, 1 overland).
+int krylov_solve(dhara_ensemble* e, bool two_d, const std::vector<int>& on, int maxit)
+{
+    EHIP(e, hipMemsetAsync(e->it_base, 0, sizeof(int), e->s));   // outside any captured graph
+    if (e->use_graph) {
+        int rc = ensure_graph(e, two_d);
+        if (rc) return rc;
+    }
+    const int off = two_d ? e->M : 0;
+    for (int launched = 0;; launched += e->batch) {
+        if (e->use_graph) EHIP(e, hipGraphLaunch(two_d ? e->graph2 : e->graph3, e->s));
+        else krylov_batch(e, two_d);
+        EHIP(e, hipGetLastError());
+        int rc = read_states(e);
+        if (rc) return rc;
+        bool all = true;
+        for (int m = 0; m < e->M; ++m)
+            if (on[m] && !e->h_st[off + m].done) all = false;
+        if (all) break;
+        if (launched > maxit + 2 * e->batch) return eset(e, DHARA_E_STATE, "batched Krylov loop did not terminate");
+    }
+    return DHARA_OK;
+}
+
+dh::SolveCtl ctl(dhara_ctx* c, dh::PcgState* st, int maxit, double tol, bool mixed_inner, bool first_sweep,
+                 double* part_in, int nparts, double* part_out)
+{
+    dh::SolveCtl sc{};
+    sc.maxit = maxit;
+    sc.mixed_inner = mixed_inner ? 1 : 0;
+    sc.first_sweep = first_sweep ? 1 : 0;
+    sc.lin_tol = tol;
+    sc.inner_tol = c->p.inner_tol;
+    sc.part_in = part_in;
+    sc.nparts = nparts;
+    sc.part_out = part_out;
+    sc.st = st;
+    return sc;
+}
+
+void build_k3(dhara_ensemble* e, bool first_sweep)
+{
+    for (int m = 0; m < e->M; ++m) {
+        dhara_ctx* c = e->mem[m];
+        const int np = dh::grid_for(c->N);
+        if (e->mixed) {
+            dh::KrylovArgs<float>& k = e->h_k3f[m];
+            k = dh::KrylovArgs<float>{};
+            k.g = c->g;
+            k.K = c->K32; k.D = c->D32; k.r = c->r32;
+            k.x = c->d32; k.res = c->r32; k.p[0] = c->p32[0]; k.p[1] = c->p32[1]; k.q = c->q32;
+            k.spmv = ctl(c, &e->st[m], c->p.inner_maxiter, c->p.lin_tol, true, first_sweep, c->part[0], np, c->part[1]);
+            k.upd = ctl(c, &e->st[m], c->p.inner_maxiter, c->p.lin_tol, true, first_sweep, c->part[1], np, c->part[0]);
+        } else {
+            dh::KrylovArgs<double>& k = e->h_k3d[m];
+            k = dh::KrylovArgs<double>{};
+            k.g = c->g;
+            k.K = c->K; k.D = c->D; k.r = c->r;
+            k.x = c->x; k.res = c->r; k.p[0] = c->pv[0]; k.p[1] = c->pv[1]; k.q = c->q;
+            k.spmv = ctl(c, &e->st[m], c->p.lin_maxiter, c->p.lin_tol, false, true, c->part[0], np, c->part[1]);
+            k.upd = ctl(c, &e->st[m], c->p.lin_maxiter, c->p.lin_tol, false, true, c->part[1], np, c->part[0]);
+        }
+    }
+}
+
+int upload_k3(dhara_ensemble* e)
+{
+    return e->mixed ? upload(e, (dh::KrylovArgs<float>*)e->a_k3, e->h_k3f)
+                    : upload(e, (dh::KrylovArgs<double>*)e->a_k3, e->h_k3d);
+}
+
+// Subsurface step of every member (do_subsurface_step in lockstep).
+int subsurface_step(dhara_ensemble* e, std::vector<dhara_step_stats>& st)
+{
+    const int M = e->M;
+    int rc;
+    for (int m = 0; m < M; ++m) {
+        dhara_ctx* c = e->mem[m];
+        if (c->p.am != 0.0)
+            EHIP(e, hipMemcpyAsync(c->psi_m, c->psi_n, (size_t)c->N * 8, hipMemcpyDeviceToDevice, e->s));
+        e->h_surf[m] = surf_args(c);
+    }
+    std::vector<int> all(M, 1), surf(M, 0);
+    for (int m = 0; m < M; ++m) surf[m] = e->mem[m]->p.top_bc == DHARA_TOP_SURFACE;
+    if ((rc = upload(e, e->a_surf, e->h_surf)) || (rc = set_active(e, surf))) return rc;
+    dh::launch_batched(e->s, dh::BK_SURF_PRE, e->a_surf, e->map2, e->nb2, e->active);
+    for (int m = 0; m < M; ++m) {
+        dhara_ctx* c = e->mem[m];
+        if (!c->theta_m_current) {
+            std::swap(c->theta_m, c->theta_new);
+            c->theta_m_current = true;
+        }
+        st[m].lin_converged = 1;
+    }
+    std::vector<int> pic(M, 1), niter(M, 0), conv(M, 0);
+    std::vector<double> delta(M, 0.0);
+    auto any = [](const std::vector<int>& v) { return std::any_of(v.begin(), v.end(), [](int x) { return x != 0; }); };
+    while (any(pic)) {
+        // assembly (D, b, r0 = b - A x) of every member still iterating
+        for (int m = 0; m < M; ++m) {
+            dhara_ctx* c = e->mem[m];
+            e->h_asm[m] = asm_args(c, false);
+            e->h_asm[m].st = &e->st[m];
+        }
+        if ((rc = upload(e, e->a_asm, e->h_asm)) || (rc = set_active(e, pic))) return rc;
+        dh::launch_batched(e->s, e->mixed ? dh::BK_ASM_MIXED : dh::BK_ASM_F64, e->a_asm, e->map3, e->nb3, e->active);
+        build_k3(e, true);
+        if ((rc = upload_k3(e))) return rc;
+        std::vector<int> solving = pic;
+        int round = 0;
+        for (;;) {
+            if ((rc = set_active(e, solving))) return rc;
+            const int maxit = e->mixed ? e->mem[0]->p.inner_maxiter : e->mem[0]->p.lin_maxiter;
+            if ((rc = krylov_solve(e, false, solving, maxit))) return rc;
+            std::vector<int> refine(M, 0);
+            for (int m = 0; m < M; ++m) {
+                if (!solving[m]) continue;
+                const dh::PcgState& s = e->h_st[m];
+                st[m].lin_iters += s.iter;
+                if (s.nan) st[m].nan_seen = 1;
+                if (!e->mixed) {
+                    st[m].lin_converged &= s.conv;
+                    continue;
+                }
+                if (s.outer_conv || s.nan) {
+                    if (!s.outer_conv) st[m].lin_converged = 0;
+                    continue;
+                }
+                refine[m] = 1;   // fold d into x and restart from the FP64 residual
+            }
+            if (!e->mixed || !any(refine)) break;
+            for (int m = 0; m < M; ++m) {
+                dhara_ctx* c = e->mem[m];
+                dh::RefineArgs& ra = e->h_ref[m];
+                ra = dh::RefineArgs{};
+                ra.g = c->g; ra.K = c->K; ra.D = c->D; ra.b = c->b; ra.x = c->x; ra.flags = nullptr;
+                ra.r32 = c->r32; ra.D32 = c->D32; ra.d32 = c->d32; ra.part_out = c->part[0]; ra.st = &e->st[m];
+                if (refine[m]) st[m].refine_sweeps += 1;
+            }
+            if ((rc = upload(e, e->a_ref, e->h_ref)) || (rc = set_active(e, refine))) return rc;
+            dh::launch_batched(e->s, dh::BK_REFINE_ADD, e->a_ref, e->map3, e->nb3, e->active);
+            dh::launch_batched(e->s, dh::BK_REFINE_RES, e->a_ref, e->map3, e->nb3, e->active);
+            ++round;
+            if (round > e->mem[0]->p.refine_maxiter) {
+                for (int m = 0; m < M; ++m)
+                    if (refine[m]) st[m].lin_converged = 0;
+                break;
+            }
+            if (round == 1) {
+                build_k3(e, false);   // later rounds keep the first round's ||b||
+                if ((rc = upload_k3(e))) return rc;
+            }
+            solving = refine;
+        }
+        // theta, K, C of the new iterate, Picard increment (k_vg_picard) for every member
+        for (int m = 0; m < M; ++m) e->h_vg[m] = vg_args(e->mem[m], e->mem[m]->x, e->mem[m]->theta_new);
+        if ((rc = upload(e, e->a_vg, e->h_vg)) || (rc = set_active(e, pic))) return rc;
+        dh::launch_batched(e->s, dh::BK_VG_PICARD, e->a_vg, e->map3, e->nb3, e->active);
+        if ((rc = reduce(e, false, 2, true))) return rc;
+        std::vector<int> relax(M, 0);
+        for (int m = 0; m < M; ++m) {
+            if (!pic[m]) continue;
+            dhara_ctx* c = e->mem[m];
+            ++niter[m];
+            delta[m] = e->h_red[2 * m];
+            if (e->h_red[2 * m + 1] > 0.0 || st[m].nan_seen) {
+                st[m].nan_seen = 1;
+                pic[m] = 0;
+                c->theta_m_current = false;
+            } else if (delta[m] < c->p.picard_tol) {
+                conv[m] = 1;
+                pic[m] = 0;
+                c->theta_m_current = false;
+            } else if (niter[m] < c->p.picard_maxiter) {
+                if (c->p.am == 0.0) std::swap(c->theta_m, c->theta_new);
+                else relax[m] = 1;
+                c->theta_m_current = true;
+            } else {
+                pic[m] = 0;
+                c->theta_m_current = false;
+            }
+        }
+        if (any(relax)) {   // theta, K, C at the relaxed iterate psi_m
+            for (int m = 0; m < M; ++m) e->h_vg[m] = vg_args(e->mem[m], e->mem[m]->psi_m, e->mem[m]->theta_m);
+            if ((rc = upload(e, e->a_vg, e->h_vg)) || (rc = set_active(e, relax))) return rc;
+            dh::launch_batched(e->s, dh::BK_VG, e->a_vg, e->map3, e->nb3, e->active);
+        }
+    }
+    // ponding update and end of step for every member
+    if ((rc = set_active(e, surf))) return rc;
+    dh::launch_batched(e->s, dh::BK_SURF_POST, e->a_surf, e->map2, e->nb2, e->active);
+    for (int m = 0; m < M; ++m) {
+        dhara_ctx* c = e->mem[m];
+        dh::EndArgs& ea = e->h_end[m];
+        ea = dh::EndArgs{};
+        ea.g = c->g;
+        ea.dt = c->p.dt;
+        ea.Ss = c->p.Ss;
+        ea.poros = c->p.porosity;
+        ea.x = c->x;
+        ea.theta_new = c->theta_new;
+        ea.theta_m = c->theta_m;
+        ea.psi_n = c->psi_n;
+        ea.theta_n = c->theta_n;
+        ea.transp = c->transp;
+        ea.root = c->root;
+        ea.procmap = c->procmap;
+        ea.has_transp = c->has_transp ? 1 : 0;
+        ea.carry_reference = c->p.theta_carry == DHARA_THETA_CARRY_REFERENCE ? 1 : 0;
+        for (int f = 0; f < 6; ++f) {
+            ea.bctype[f] = c->bctype[f];
+            ea.bcq[f] = c->bcq[f];
+        }
+        ea.part_out = c->part[0];
+    }
+    if ((rc = upload(e, e->a_end, e->h_end)) || (rc = set_active(e, all))) return rc;
+    dh::launch_batched(e->s, dh::BK_END, e->a_end, e->map3, e->nb3, e->active);
+    if ((rc = reduce(e, false, 9, false))) return rc;
+    for (int m = 0; m < M; ++m) {
+        dhara_ctx* c = e->mem[m];
+        c->theta_m_current = false;
+        dhara_step_stats& o = st[m];
+        o.picard_iters = niter[m];
+        o.picard_converged = conv[m];
+        o.picard_delta = o.nan_seen ? NAN : delta[m];
+        const double* s = e->h_red + 9 * m;
+        const double dx = c->grid.dx, dy = c->grid.dy, dz = c->grid.dz, dt = c->p.dt;
+        const double qt = s[3], qb = s[4], qw = s[5], qe = s[6], qs = s[7], qn = s[8];
+        o.mb_ref = s[0] * dz + ((qe - qw) + (qn - qs) + (qb - qt)) * dt + s[2] + s[1];
+        o.storage_change = (s[0] * dz + s[1]) * dx * dy;
+        const double inflow = (qt - qb) * dx * dy * dt + (qw - qe) * dy * dz * dt + (qs - qn) * dx * dz * dt -
+                              s[2] * dx * dy;
+        o.boundary_inflow = inflow;
+        const double flux_scale = (std::fabs(qt) + std::fabs(qb)) * dx * dy * dt +
+                                  (std::fabs(qw) + std::fabs(qe)) * dy * dz * dt +
+                                  (std::fabs(qs) + std::fabs(qn)) * dx * dz * dt + s[2] * dx * dy;
+        const double scale = std::max({std::fabs(o.storage_change), std::fabs(inflow), flux_scale, 1e-300});
+        o.mb_rel = std::fabs(o.storage_change - inflow) / scale;
+    }
+    return DHARA_OK;
+}
+
+int overland_step(dhara_ensemble* e, std::vector<dhara_step_stats>& st)
+{
+    const int M = e->M;
+    int rc;
+    std::vector<int> on(M, 0);
+    for (int m = 0; m < M; ++m) {
+        dhara_ctx* c = e->mem[m];
+        on[m] = c->p.overland ? 1 : 0;
+        e->h_olf[m] = olf_args(c);
+        e->h_olf[m].st = &e->st[M + m];
+        dh::KrylovArgs<double>& k = e->h_k2[m];
+        k = dh::KrylovArgs<double>{};
+        k.g2 = c->g2;
+        k.D = c->D2; k.r = c->r2; k.x = c->H; k.res = c->r2; k.p[0] = c->p2[0]; k.p[1] = c->p2[1]; k.q = c->q2;
+        k.ke = c->ke; k.kn = c->kn;
+        const int np = dh::grid_for(c->S);
+        k.spmv = ctl(c, &e->st[M + m], c->p.olf_maxiter, c->p.olf_tol, false, true, c->part[0], np, c->part[1]);
+        k.upd = ctl(c, &e->st[M + m], c->p.olf_maxiter, c->p.olf_tol, false, true, c->part[1], np, c->part[0]);
+    }
+    if ((rc = upload(e, e->a_olf, e->h_olf)) || (rc = upload(e, e->a_k2, e->h_k2)) || (rc = set_active(e, on)))
+        return rc;
+    dh::launch_batched(e->s, dh::BK_OLF_H, e->a_olf, e->map2, e->nb2, e->active);
+    dh::launch_batched(e->s, dh::BK_OLF_FACES, e->a_olf, e->map2, e->nb2, e->active);
+    dh::launch_batched(e->s, dh::BK_OLF_ASM, e->a_olf, e->map2, e->nb2, e->active);
+    if ((rc = krylov_solve(e, true, on, e->mem[0]->p.olf_maxiter))) return rc;
+    for (int m = 0; m < M; ++m) {
+        if (!on[m]) continue;
+        st[m].olf_iters = e->h_st[M + m].iter;
+        st[m].olf_converged = e->h_st[M + m].conv;
+        if (e->h_st[M + m].nan) st[m].nan_seen = 1;
+    }
+    dh::launch_batched(e->s, dh::BK_OLF_POST, e->a_olf, e->map2, e->nb2, e->active);
+    if ((rc = reduce(e, true, 2, false))) return rc;
+    for (int m = 0; m < M; ++m) {
+        if (!on[m]) continue;
+        st[m].outlet_discharge = e->h_red[2 * m];
+        st[m].ring_outflow = e->h_red[2 * m + 1];
+    }
+    return DHARA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble** out)
+{
+    if (n < 1 || !members || !out) return DHARA_E_ARG;
+    *out = nullptr;
+    dhara_ensemble* e = new dhara_ensemble;
+    e->M = n;
+    e->mem.assign(members, members + n);
+    for (dhara_ctx* c : e->mem) {
+        if (!c || !c->initialized) { delete e; return DHARA_E_STATE; }
+        if (has_dirichlet(c)) { delete e; return DHARA_E_UNSUPPORTED; }
+    }
+    e->mixed = e->mem[0]->p.precision == DHARA_MIXED;
+    for (dhara_ctx* c : e->mem) {
+        if ((c->p.precision == DHARA_MIXED) != e->mixed) { delete e; return DHARA_E_ARG; }
+        if (e->mixed && alloc_mixed(c)) { delete e; return DHARA_E_ALLOC; }
+        if (hipStreamSynchronize(c->s) != hipSuccess) { delete e; return DHARA_E_DEVICE; }
+    }
+    if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess) { delete e; return DHARA_E_DEVICE; }
+    std::vector<dh::BlkMap> m3, m2;
+    std::vector<double*> parts(n);
+    std::vector<int> np3(n), np2(n);
+    for (int m = 0; m < n; ++m) {
+        dhara_ctx* c = e->mem[m];
+        np3[m] = dh::grid_for(c->N);
+        np2[m] = dh::grid_for(c->S);
+        for (int b = 0; b < np3[m]; ++b) m3.push_back(dh::BlkMap{m, b, np3[m]});
+        for (int b = 0; b < np2[m]; ++b) m2.push_back(dh::BlkMap{m, b, np2[m]});
+        parts[m] = c->part[0];
+    }
+    e->nb3 = (int)m3.size();
+    e->nb2 = (int)m2.size();
+    int rc = 0;
+    if ((rc = ealloc(e, &e->map3, m3.size())) || (rc = ealloc(e, &e->map2, m2.size())) ||
+        (rc = ealloc(e, &e->active, n)) || (rc = ealloc(e, &e->it_base, 1)) || (rc = ealloc(e, &e->st, 2 * n)) ||
+        (rc = ealloc(e, &e->parts, n)) || (rc = ealloc(e, &e->np3, n)) || (rc = ealloc(e, &e->np2, n)) ||
+        (rc = ealloc(e, &e->red, (size_t)n * dh::kMaxSums)) || (rc = ealloc(e, &e->a_vg, n)) ||
+        (rc = ealloc(e, &e->a_surf, n)) || (rc = ealloc(e, &e->a_asm, n)) || (rc = ealloc(e, &e->a_ref, n)) ||
+        (rc = ealloc(e, &e->a_end, n)) || (rc = ealloc(e, &e->a_olf, n)) || (rc = ealloc(e, &e->a_k2, n))) {
+        dhara_ensemble_destroy(e);
+        return rc;
+    }
+    if (e->mixed) rc = ealloc(e, (dh::KrylovArgs<float>**)&e->a_k3, n);
+    else rc = ealloc(e, (dh::KrylovArgs<double>**)&e->a_k3, n);
+    if (rc) { dhara_ensemble_destroy(e); return rc; }
+    if (hipHostMalloc((void**)&e->h_active, n * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void**)&e->h_st, 2 * n * sizeof(dh::PcgState)) != hipSuccess ||
+        hipHostMalloc((void**)&e->h_red, (size_t)n * dh::kMaxSums * sizeof(double)) != hipSuccess) {
+        dhara_ensemble_destroy(e);
+        return DHARA_E_ALLOC;
+    }
+    if ((rc = upload(e, e->map3, m3)) || (rc = upload(e, e->map2, m2)) || (rc = upload(e, e->parts, parts)) ||
+        (rc = upload(e, e->np3, np3)) || (rc = upload(e, e->np2, np2))) {
+        dhara_ensemble_destroy(e);
+        return rc;
+    }
+    std::vector<dh::PcgState> zero(2 * n);
+    if ((rc = upload(e, e->st, zero))) { dhara_ensemble_destroy(e); return rc; }
+    e->h_vg.resize(n); e->h_surf.resize(n); e->h_asm.resize(n); e->h_ref.resize(n); e->h_end.resize(n);
+    e->h_olf.resize(n); e->h_k3d.resize(n); e->h_k3f.resize(n); e->h_k2.resize(n);
+    *out = e;
+    return DHARA_OK;
+}
+
+int dhara_ensemble_set_graph(dhara_ensemble* e, int32_t on)
+{
+    if (!e) return DHARA_E_ARG;
+    e->use_graph = on != 0;
+    return DHARA_OK;
+}
+
+const char* dhara_ensemble_last_error(const dhara_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int dhara_ensemble_step(dhara_ensemble* e, int32_t nsteps, dhara_step_stats* stats)
+{
+    if (!e || nsteps < 0) return DHARA_E_ARG;
+    for (int s = 0; s < nsteps; ++s) {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<dhara_step_stats> st(e->M);
+        for (auto& x : st) std::memset(&x, 0, sizeof(x));
+        int rc = subsurface_step(e, st);
+        if (rc == DHARA_OK) rc = overland_step(e, st);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int m = 0; m < e->M; ++m) {
+            st[m].wall_ms = ms;
+            if (stats) stats[(size_t)s * e->M + m] = st[m];
+        }
+        if (rc) return rc;
+        for (int m = 0; m < e->M; ++m)
+            if (st[m].nan_seen) return eset(e, DHARA_E_NUMERIC, "NaN/Inf in member " + std::to_string(m));
+    }
+    return DHARA_OK;
+}
+
+int dhara_ensemble_destroy(dhara_ensemble* e)
+{
+    if (!e) return DHARA_OK;
+    if (e->s) (void)hipStreamSynchronize(e->s);
+    if (e->graph3) (void)hipGraphExecDestroy(e->graph3);
+    if (e->graph2) (void)hipGraphExecDestroy(e->graph2);
+    for (void* v : e->allocs) (void)hipFree(v);
+    if (e->h_active) (void)hipHostFree(e->h_active);
+    if (e->h_st) (void)hipHostFree(e->h_st);
+    if (e->h_red) (void)hipHostFree(e->h_red);
+    if (e->s) (void)hipStreamDestroy(e->s);
+    delete e;
+    return DHARA_OK;
+}
+
+}  // extern "C"

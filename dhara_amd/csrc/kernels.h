@@ -209,6 +209,44 @@ void launch_pcg2_spmv(hipStream_t s, const Geo2& g, const double* ke, const doub
                       const SolveCtl& sc, const double* z = nullptr);
 void launch_olf_post(hipStream_t s, const OlfArgs& a);
 
+// ---- batched launches over an ensemble of models (kernels.hip) ----------------------------
+// A batched launch runs one kernel body for many models at once: physical block b works as
+// block map[b].id of map[b].n blocks of model map[b].m, with that model's argument struct.
+// Models whose active flag is 0 return at once.
+struct BlkMap {
+    int m, id, n;
+};
+// Krylov arguments of one model for batched solves (the iteration index comes from *it_base
+// plus the launch's position k in the batch, so a captured batch can be replayed unchanged).
+template <typename T>
+struct KrylovArgs {
+    Geo g;
+    Geo2 g2;
+    const T *K, *D, *r;
+    T *x, *res, *p[2], *q;
+    const double *ke, *kn;       // overland operator (2D solves)
+    SolveCtl spmv, upd;          // part_in/part_out/st of the two kernels; `it` is ignored
+};
+enum BatchKind { BK_VG = 0, BK_VG_PICARD, BK_SURF_PRE, BK_SURF_POST, BK_ASM_F64, BK_ASM_MIXED,
+                 BK_REFINE_ADD, BK_REFINE_RES, BK_END, BK_OLF_H, BK_OLF_FACES, BK_OLF_ASM, BK_OLF_POST };
+// args points to an array of the argument struct of the kind (VgArgs, SurfArgs, AsmArgs,
+// RefineArgs, EndArgs or OlfArgs), one per model.
+void launch_batched(hipStream_t s, BatchKind kind, const void* args, const BlkMap* map, int nblocks,
+                    const int* active);
+void launch_batched_spmv3(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,
+                          const int* active, const int* it_base, int k);
+void launch_batched_update3(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,
+                            const int* active, const int* it_base, int k);
+void launch_batched_spmv2(hipStream_t s, const void* kargs, const BlkMap* map, int nblocks, const int* active,
+                          const int* it_base, int k);
+void launch_batched_update2(hipStream_t s, const void* kargs, const BlkMap* map, int nblocks, const int* active,
+                            const int* it_base, int k);
+void launch_advance(hipStream_t s, int* it_base, int by);
+// out[m*nv + v] = sum (or max) of row v of model m's partials part[m] (nparts[m] entries),
+// in block order; one block per model.
+void launch_reduce_models(hipStream_t s, int nmodels, const double* const* part, const int* nparts, int nv,
+                          bool take_max, double* out);
+
 // ---- overland multigrid preconditioner (olf_mg.hip) ---------------------------------------
 // One level of the 2 x 2 aggregation hierarchy: operator A = d - weights (we: cell to its east
 // neighbour, wn: to its north neighbour), right-hand side b, pre-smoothed x, V-cycle result y.

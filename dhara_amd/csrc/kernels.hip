@@ -80,9 +80,9 @@ __device__ __forceinline__ double op2_row(const Geo2& g, int c, double Dc, doubl
 // ---------------------------------------------------------------------------------------
 // Constitutive kernels
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_vg(VgArgs a)
+__device__ __forceinline__ void b_vg(VgArgs a, Blk bk)
 {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.g.n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < a.g.n; c += bk.n * blockDim.x) {
         double th, K, C;
         vg_eval(a.psi[c], a.soil.Ks[c], cell_alpha(a.soil, c), cell_n(a.soil, c), a.soil.ts,
                 a.soil.tr, th, K, C);
@@ -93,12 +93,17 @@ __global__ __launch_bounds__(kBlock) void k_vg(VgArgs a)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_vg(VgArgs a)
+{
+    b_vg(a, hw_blk());
+}
+
 // vanGenuchten on psi^{m+1} (src/subsurface.cu:754) fused with GetIterationDifference
 // (432-446), the max reduction (777) and ModifiedPicardUpdate (450-461).
-__global__ __launch_bounds__(kBlock) void k_vg_picard(VgArgs a)
+__device__ __forceinline__ void b_vg_picard(VgArgs a, Blk bk)
 {
     double mx[2] = {0.0, 0.0};   // increment, NaN flag
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.g.n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < a.g.n; c += bk.n * blockDim.x) {
         const double xc = a.psi[c];
         const double pm = a.psi_m[c];
         double th, K, C;
@@ -116,7 +121,12 @@ __global__ __launch_bounds__(kBlock) void k_vg_picard(VgArgs a)
         if (a.K32) a.K32[c] = (float)K;
         a.psi_m[c] = a.am * pm + (1.0 - a.am) * xc;
     }
-    block_partials<2, true>(mx, a.part_out);
+    block_partials<2, true>(mx, a.part_out, bk.id);
+}
+
+__global__ __launch_bounds__(kBlock) void k_vg_picard(VgArgs a)
+{
+    b_vg_picard(a, hw_blk());
 }
 
 __global__ __launch_bounds__(kBlock) void k_vg_inverse(Geo g, Soil s, const double* theta, double* psi)
@@ -157,9 +167,9 @@ __global__ __launch_bounds__(kBlock) void k_group_scatter(int nxy, const int* pr
 
 // EstimateFluxes (src/global.cu:172-191) + IdentifyTopBoundary (206-257).  Both only use
 // quantities fixed within a step, so the top flux is computed once per step.
-__global__ __launch_bounds__(kBlock) void k_surface_pre(SurfArgs a)
+__device__ __forceinline__ void b_surface_pre(SurfArgs a, Blk bk)
 {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nxy; t += gridDim.x * blockDim.x) {
+    for (int t = bk.id * blockDim.x + threadIdx.x; t < a.nxy; t += bk.n * blockDim.x) {
         const double ph = a.h[t];
         const double hp = ph + a.rain[t] * a.dt - a.evap[t] * a.dt;
         double qc = -a.K[t] * ((a.psi_n[t] - hp - 0.5 * a.dz) / (0.5 * a.dz));
@@ -179,11 +189,21 @@ __global__ __launch_bounds__(kBlock) void k_surface_pre(SurfArgs a)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_surface_pre(SurfArgs a)
+{
+    b_surface_pre(a, hw_blk());
+}
+
 // SubsurfaceEstimateInfiltrationPonding, flux branch (src/subsurface.cu:474-481).
+__device__ __forceinline__ void b_surface_post(SurfArgs a, Blk bk)
+{
+    for (int t = bk.id * blockDim.x + threadIdx.x; t < a.nxy; t += bk.n * blockDim.x)
+        a.h[t] = fmax(a.hpoten[t] - a.qtop[t] * a.dt, 0.0);
+}
+
 __global__ __launch_bounds__(kBlock) void k_surface_post(SurfArgs a)
 {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nxy; t += gridDim.x * blockDim.x)
-        a.h[t] = fmax(a.hpoten[t] - a.qtop[t] * a.dt, 0.0);
+    b_surface_post(a, hw_blk());
 }
 
 // ---------------------------------------------------------------------------------------
@@ -192,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_surface_post(SurfArgs a)
 // warm-started solve r0 = b - A x0 and its dot products.
 // ---------------------------------------------------------------------------------------
 template <bool DIR, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
+__device__ __forceinline__ void b_assemble(AsmArgs a, Blk bk)
 {
     const Geo g = a.g;
     const double dx = g.dx, dy = g.dy, dz = g.dz, dt = a.dt;
@@ -200,8 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
     const double* __restrict__ K = a.K;
     const double* __restrict__ x = a.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz, rr (solver precision), bb, rr (FP64)
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;   // a new solve starts
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    if (bk.id == 0 && threadIdx.x == 0) a.st->done = 0;   // a new solve starts
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         int i, j, k;
         decompose(g, c, i, j, k);
         const double Kc = K[c];
@@ -302,25 +322,36 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
             acc[1] += r * r;
         }
     }
-    block_partials<4, false>(acc, a.part_out);
+    block_partials<4, false>(acc, a.part_out, bk.id);
+}
+
+template <bool DIR, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
+{
+    b_assemble<DIR, MIXED>(a, hw_blk());
 }
 
 // FP64 iterative refinement: x <- x + d (pointwise), then r = b - A x in FP64 with the
 // FP32 copy of r as the next inner right-hand side.
-__global__ __launch_bounds__(kBlock) void k_refine_add(int n, double* x, const float* d)
+__device__ __forceinline__ void b_refine_add(int n, double* x, const float* d, Blk bk)
 {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < n; c += bk.n * blockDim.x)
         x[c] += (double)d[c];
 }
 
+__global__ __launch_bounds__(kBlock) void k_refine_add(int n, double* x, const float* d)
+{
+    b_refine_add(n, x, d, hw_blk());
+}
+
 template <bool DIR>
-__global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
+__device__ __forceinline__ void b_refine_residual(RefineArgs a, Blk bk)
 {
     const Geo g = a.g;
     const double* __restrict__ x = a.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz32, rr32, (unused), rr64
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;   // next inner solve starts
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    if (bk.id == 0 && threadIdx.x == 0) a.st->done = 0;   // next inner solve starts
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         int i, j, k;
         decompose(g, c, i, j, k);
         const unsigned fl = DIR ? a.flags[c] : 0u;
@@ -336,7 +367,13 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
         acc[0] += (double)r32 * (double)z32;
         acc[1] += (double)r32 * (double)r32;
     }
-    block_partials<4, false>(acc, a.part_out);
+    block_partials<4, false>(acc, a.part_out, bk.id);
+}
+
+template <bool DIR>
+__global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
+{
+    b_refine_residual<DIR>(a, hw_blk());
 }
 
 // ---------------------------------------------------------------------------------------
@@ -349,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(RefineArgs a)
 // Start of the SpMV kernel of iteration sc.it: sum the residual partials of the previous
 // kernel, decide whether the solve is over, and return beta.  Every block takes the same
 // decision from the same fixed-order sums; block 0 records it.
-__device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta)
+__device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta, int bid)
 {
     PcgState* st = sc.st;
     if (st->done) return false;
@@ -357,7 +394,7 @@ __device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta)
     double in[4];
     sum_partials<4>(sc.part_in, sc.nparts, fresh ? 4 : 2, in);
     const double rz = in[0], rr = in[1];
-    const bool lead = lin_bid() == 0 && lin_tid() == 0;
+    const bool lead = bid == 0 && lin_tid() == 0;
     double tol2;
     int outer = 0, nan;
     if (fresh) {
@@ -399,7 +436,7 @@ __device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta)
 }
 
 // Start of the update kernel: alpha = (r, z) / (p, A p).
-__device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha)
+__device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha, int bid)
 {
     PcgState* st = sc.st;
     if (st->done) return false;
@@ -407,7 +444,7 @@ __device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha)
     sum_partials<1>(sc.part_in, sc.nparts, 1, pq);
     const double a = st->rz[sc.it & 1] / pq[0];
     if (!(pq[0] != 0.0) || !isfinite(a)) {
-        if (lin_bid() == 0 && lin_tid() == 0) {
+        if (bid == 0 && lin_tid() == 0) {
             st->iter = sc.it;
             st->conv = 0;
             st->nan = 1;
@@ -420,13 +457,13 @@ __device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha)
 }
 
 template <typename T, bool DIR>
-__global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __restrict__ flags,
+__device__ __forceinline__ void b_pcg3_spmv(Geo g, const uint8_t* __restrict__ flags,
                                                       const T* __restrict__ K, const T* __restrict__ D,
                                                       const T* __restrict__ r, const T* __restrict__ pold,
-                                                      T* __restrict__ pnew, T* __restrict__ q, SolveCtl sc)
+                                                      T* __restrict__ pnew, T* __restrict__ q, SolveCtl sc, Blk bk)
 {
     double beta_d;
-    if (!pcg_begin(sc, beta_d)) return;
+    if (!pcg_begin(sc, beta_d, bk.id)) return;
     const bool first = (sc.it == 0);
     const T beta = (T)beta_d;
     auto pf = [&](int c) -> T {
@@ -434,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __re
         return first ? z : z + beta * pold[c];
     };
     double acc[1] = {0.0};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         int i, j, k;
         decompose(g, c, i, j, k);
         const T pc = pf(c);
@@ -444,19 +481,28 @@ __global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __re
         q[c] = qc;
         acc[0] += (double)pc * (double)qc;
     }
-    block_partials<1, false>(acc, sc.part_out);
+    block_partials<1, false>(acc, sc.part_out, bk.id);
+}
+
+template <typename T, bool DIR>
+__global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __restrict__ flags,
+                                                      const T* __restrict__ K, const T* __restrict__ D,
+                                                      const T* __restrict__ r, const T* __restrict__ pold,
+                                                      T* __restrict__ pnew, T* __restrict__ q, SolveCtl sc)
+{
+    b_pcg3_spmv<T, DIR>(g, flags, K, D, r, pold, pnew, q, sc, hw_blk());
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x, T* __restrict__ r,
+__device__ __forceinline__ void b_pcg_update(int n, T* __restrict__ x, T* __restrict__ r,
                                                        const T* __restrict__ D, const T* __restrict__ p,
-                                                       const T* __restrict__ q, SolveCtl sc)
+                                                       const T* __restrict__ q, SolveCtl sc, Blk bk)
 {
     double alpha_d;
-    if (!update_begin(sc, alpha_d)) return;
+    if (!update_begin(sc, alpha_d, bk.id)) return;
     const T alpha = (T)alpha_d;
     double acc[2] = {0.0, 0.0};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < n; c += bk.n * blockDim.x) {
         const T xc = x[c] + alpha * p[c];
         const T rc = r[c] - alpha * q[c];
         x[c] = xc;
@@ -465,7 +511,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
         acc[0] += (double)rc * (double)zc;
         acc[1] += (double)rc * (double)rc;
     }
-    block_partials<2, false>(acc, sc.part_out);
+    block_partials<2, false>(acc, sc.part_out, bk.id);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x, T* __restrict__ r,
+                                                       const T* __restrict__ D, const T* __restrict__ p,
+                                                       const T* __restrict__ q, SolveCtl sc)
+{
+    b_pcg_update<T>(n, x, r, D, p, q, sc, hw_blk());
 }
 
 // LDS-tiled 2.5D SpMV.  Block = TX x TY threads owning TX x TY columns; the block marches
@@ -494,7 +548,7 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
                                                              SolveCtl sc, int kc, int ntx, int nty, int nkz)
 {
     double beta_d;
-    if (!pcg_begin(sc, beta_d)) return;
+    if (!pcg_begin(sc, beta_d, lin_bid())) return;
     const bool first = (sc.it == 0);
     const T beta = (T)beta_d;
     __shared__ T sp[2][TY + 2][TX + 2];
@@ -625,12 +679,12 @@ __global__ __launch_bounds__(kBlock) void k_cast_f64(int n, const float* in, dou
 // End of step: mass-balance sums (src/subsurface.cu:494-629, 808-826) and the state copies
 // psi^n <- psi^{n+1}, theta^n <- theta carry (836-840).
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_end_step(EndArgs a)
+__device__ __forceinline__ void b_end_step(EndArgs a, Blk bk)
 {
     const Geo g = a.g;
     // 0 sum dtheta, 1 sum Ss th/phi dpsi dz, 2 sink volume/area, 3 qt, 4 qb, 5 qw, 6 qe, 7 qs, 8 qn
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         int i, j, k;
         decompose(g, c, i, j, k);
         const double thn = a.theta_new[c];
@@ -652,7 +706,12 @@ __global__ __launch_bounds__(kBlock) void k_end_step(EndArgs a)
         a.psi_n[c] = xc;
         a.theta_n[c] = a.carry_reference ? a.theta_m[c] : thn;
     }
-    block_partials<9, false>(acc, a.part_out);
+    block_partials<9, false>(acc, a.part_out, bk.id);
+}
+
+__global__ __launch_bounds__(kBlock) void k_end_step(EndArgs a)
+{
+    b_end_step(a, hw_blk());
 }
 
 // ---------------------------------------------------------------------------------------
@@ -682,20 +741,25 @@ __device__ __forceinline__ double olf_face_k(double gdir, double gcross, double 
 // H = h + z, then the east and north face conductances of every face the eliminated
 // system uses, each evaluated from its non-ring cell (the row that owns the coupling in
 // the reference matrix).
-__global__ __launch_bounds__(kBlock) void k_olf_H(OlfArgs a)
+__device__ __forceinline__ void b_olf_H(OlfArgs a, Blk bk)
 {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.g.n; c += gridDim.x * blockDim.x)
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < a.g.n; c += bk.n * blockDim.x)
         a.H[c] = a.h[c] + a.z[c];
 }
 
-__global__ __launch_bounds__(kBlock) void k_olf_faces(OlfArgs a)
+__global__ __launch_bounds__(kBlock) void k_olf_H(OlfArgs a)
+{
+    b_olf_H(a, hw_blk());
+}
+
+__device__ __forceinline__ void b_olf_faces(OlfArgs a, Blk bk)
 {
     const Geo2 g = a.g;
     const int nx = g.nx;
     const double* __restrict__ H = a.H;
     const double* __restrict__ h = a.h;
     const double* __restrict__ m = a.mann;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         const int j = c / nx, i = c - j * nx;
         double ke = 0.0, kn = 0.0;
         // east face (i,j)-(i+1,j): used when row j is interior
@@ -721,9 +785,14 @@ __global__ __launch_bounds__(kBlock) void k_olf_faces(OlfArgs a)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_olf_faces(OlfArgs a)
+{
+    b_olf_faces(a, hw_blk());
+}
+
 // SetUpLinearSystemsOverland (src/overland.cu:65-189) with the ring eliminated, fused with
 // r0 = b - A H.
-__global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
+__device__ __forceinline__ void b_olf_assemble(OlfArgs a, Blk bk)
 {
     const Geo2 g = a.g;
     const int nx = g.nx;
@@ -732,8 +801,8 @@ __global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
     const double* __restrict__ ke = a.ke;
     const double* __restrict__ kn = a.kn;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz, rr, bb, rr
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;   // a new solve starts
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    if (bk.id == 0 && threadIdx.x == 0) a.st->done = 0;   // a new solve starts
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         const int j = c / nx, i = c - j * nx;
         double D, b, r;
         if (is_ring(g, i, j)) {
@@ -760,10 +829,42 @@ __global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
         acc[2] += b * b;
         acc[3] += r * r;
     }
-    block_partials<4, false>(acc, a.part_out);
+    block_partials<4, false>(acc, a.part_out, bk.id);
+}
+
+__global__ __launch_bounds__(kBlock) void k_olf_assemble(OlfArgs a)
+{
+    b_olf_assemble(a, hw_blk());
 }
 
 // PRECZ: p = z + beta p with z the preconditioned residual (multigrid); else z = r / D.
+template <bool PRECZ>
+__device__ __forceinline__ void b_pcg2_spmv(Geo2 g, const double* __restrict__ ke,
+                                                      const double* __restrict__ kn,
+                                                      const double* __restrict__ D,
+                                                      const double* __restrict__ r,
+                                                      const double* __restrict__ pold,
+                                                      double* __restrict__ pnew, double* __restrict__ q,
+                                                      SolveCtl sc, const double* __restrict__ zp, Blk bk)
+{
+    double beta;
+    if (!pcg_begin(sc, beta, bk.id)) return;
+    const bool first = (sc.it == 0);
+    auto pf = [&](int c) -> double {
+        const double z = PRECZ ? zp[c] : r[c] / D[c];
+        return first ? z : z + beta * pold[c];
+    };
+    double acc[1] = {0.0};
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
+        const double pc = pf(c);
+        pnew[c] = pc;
+        const double qc = op2_row(g, c, D[c], pc, pf, ke, kn);
+        q[c] = qc;
+        acc[0] += pc * qc;
+    }
+    block_partials<1, false>(acc, sc.part_out, bk.id);
+}
+
 template <bool PRECZ>
 __global__ __launch_bounds__(kBlock) void k_pcg2_spmv(Geo2 g, const double* __restrict__ ke,
                                                       const double* __restrict__ kn,
@@ -773,32 +874,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg2_spmv(Geo2 g, const double* __re
                                                       double* __restrict__ pnew, double* __restrict__ q,
                                                       SolveCtl sc, const double* __restrict__ zp)
 {
-    double beta;
-    if (!pcg_begin(sc, beta)) return;
-    const bool first = (sc.it == 0);
-    auto pf = [&](int c) -> double {
-        const double z = PRECZ ? zp[c] : r[c] / D[c];
-        return first ? z : z + beta * pold[c];
-    };
-    double acc[1] = {0.0};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
-        const double pc = pf(c);
-        pnew[c] = pc;
-        const double qc = op2_row(g, c, D[c], pc, pf, ke, kn);
-        q[c] = qc;
-        acc[0] += pc * qc;
-    }
-    block_partials<1, false>(acc, sc.part_out);
+    b_pcg2_spmv<PRECZ>(g, ke, kn, D, r, pold, pnew, q, sc, zp, hw_blk());
 }
 
 // WaterDepthOverland (src/overland.cu:371-383) and the discharge into the Dirichlet ring.
-__global__ __launch_bounds__(kBlock) void k_olf_post(OlfArgs a)
+__device__ __forceinline__ void b_olf_post(OlfArgs a, Blk bk)
 {
     const Geo2 g = a.g;
     const int nx = g.nx;
     const double* __restrict__ H = a.H;
     double acc[2] = {0.0, 0.0};   // x = 0 edge, whole ring
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         const int j = c / nx, i = c - j * nx;
         if (!is_ring(g, i, j)) {
             const double Hc = H[c];
@@ -813,7 +899,104 @@ __global__ __launch_bounds__(kBlock) void k_olf_post(OlfArgs a)
         }
         a.h[c] = fmax(H[c] - a.z[c], 0.0);
     }
-    block_partials<2, false>(acc, a.part_out);
+    block_partials<2, false>(acc, a.part_out, bk.id);
+}
+
+__global__ __launch_bounds__(kBlock) void k_olf_post(OlfArgs a)
+{
+    b_olf_post(a, hw_blk());
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched launches over an ensemble of models
+// ---------------------------------------------------------------------------------------
+template <class A, void (*F)(A, Blk)>
+__global__ __launch_bounds__(kBlock) void kb_args(const A* __restrict__ args, const BlkMap* __restrict__ map,
+                                                  const int* __restrict__ active)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (active && !active[e.m]) return;
+    F(args[e.m], Blk{e.id, e.n});
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void b_assemble_flux(AsmArgs a, Blk bk) { b_assemble<false, MIXED>(a, bk); }
+__device__ __forceinline__ void b_refine_add_args(RefineArgs a, Blk bk) { b_refine_add(a.g.n, a.x, a.d32, bk); }
+__device__ __forceinline__ void b_refine_res_args(RefineArgs a, Blk bk) { b_refine_residual<false>(a, bk); }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kb_spmv3(const KrylovArgs<T>* __restrict__ args,
+                                                   const BlkMap* __restrict__ map, const int* __restrict__ active,
+                                                   const int* __restrict__ it_base, int k)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const KrylovArgs<T>& a = args[e.m];
+    SolveCtl sc = a.spmv;
+    sc.it = *it_base + k;
+    b_pcg3_spmv<T, false>(a.g, nullptr, a.K, a.D, a.r, a.p[sc.it & 1], a.p[(sc.it + 1) & 1], a.q, sc,
+                          Blk{e.id, e.n});
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kb_update3(const KrylovArgs<T>* __restrict__ args,
+                                                     const BlkMap* __restrict__ map, const int* __restrict__ active,
+                                                     const int* __restrict__ it_base, int k)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const KrylovArgs<T>& a = args[e.m];
+    SolveCtl sc = a.upd;
+    sc.it = *it_base + k;
+    b_pcg_update<T>(a.g.n, a.x, a.res, a.D, a.p[(sc.it + 1) & 1], a.q, sc, Blk{e.id, e.n});
+}
+
+__global__ __launch_bounds__(kBlock) void kb_spmv2(const KrylovArgs<double>* __restrict__ args,
+                                                   const BlkMap* __restrict__ map, const int* __restrict__ active,
+                                                   const int* __restrict__ it_base, int k)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const KrylovArgs<double>& a = args[e.m];
+    SolveCtl sc = a.spmv;
+    sc.it = *it_base + k;
+    b_pcg2_spmv<false>(a.g2, a.ke, a.kn, a.D, a.r, a.p[sc.it & 1], a.p[(sc.it + 1) & 1], a.q, sc, nullptr,
+                       Blk{e.id, e.n});
+}
+
+__global__ __launch_bounds__(kBlock) void kb_update2(const KrylovArgs<double>* __restrict__ args,
+                                                     const BlkMap* __restrict__ map, const int* __restrict__ active,
+                                                     const int* __restrict__ it_base, int k)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const KrylovArgs<double>& a = args[e.m];
+    SolveCtl sc = a.upd;
+    sc.it = *it_base + k;
+    b_pcg_update<double>(a.g2.n, a.x, a.res, a.D, a.p[(sc.it + 1) & 1], a.q, sc, Blk{e.id, e.n});
+}
+
+__global__ void k_advance(int* it_base, int by)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *it_base += by;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_models(const double* const* __restrict__ part,
+                                                          const int* __restrict__ nparts, int nv, int take_max,
+                                                          double* __restrict__ out)
+{
+    const int m = blockIdx.x;
+    __shared__ double lds[16];
+    for (int v = 0; v < nv; ++v) {
+        const double* row = part[m] + (size_t)v * kMaxBlocks;
+        double acc[1] = {take_max ? -INFINITY : 0.0};
+        for (int b = threadIdx.x; b < nparts[m]; b += blockDim.x)
+            acc[0] = take_max ? fmax(acc[0], row[b]) : acc[0] + row[b];
+        if (take_max) block_reduce<1, true>(acc, lds);
+        else block_reduce<1, false>(acc, lds);
+        if (threadIdx.x == 0) out[m * nv + v] = acc[0];
+        __syncthreads();
+    }
 }
 
 }  // namespace
@@ -1041,6 +1224,101 @@ void launch_pcg2_spmv(hipStream_t s, const Geo2& g, const double* ke, const doub
 void launch_olf_post(hipStream_t s, const OlfArgs& a)
 {
     hipLaunchKernelGGL(k_olf_post, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_batched(hipStream_t s, BatchKind kind, const void* args, const BlkMap* map, int nblocks,
+                    const int* active)
+{
+    const dim3 gr(nblocks), bl(kBlock);
+    switch (kind) {
+    case BK_VG:
+        hipLaunchKernelGGL((kb_args<VgArgs, b_vg>), gr, bl, 0, s, (const VgArgs*)args, map, active); break;
+    case BK_VG_PICARD:
+        hipLaunchKernelGGL((kb_args<VgArgs, b_vg_picard>), gr, bl, 0, s, (const VgArgs*)args, map, active); break;
+    case BK_SURF_PRE:
+        hipLaunchKernelGGL((kb_args<SurfArgs, b_surface_pre>), gr, bl, 0, s, (const SurfArgs*)args, map, active);
+        break;
+    case BK_SURF_POST:
+        hipLaunchKernelGGL((kb_args<SurfArgs, b_surface_post>), gr, bl, 0, s, (const SurfArgs*)args, map, active);
+        break;
+    case BK_ASM_F64:
+        hipLaunchKernelGGL((kb_args<AsmArgs, b_assemble_flux<false>>), gr, bl, 0, s, (const AsmArgs*)args, map,
+                           active);
+        break;
+    case BK_ASM_MIXED:
+        hipLaunchKernelGGL((kb_args<AsmArgs, b_assemble_flux<true>>), gr, bl, 0, s, (const AsmArgs*)args, map,
+                           active);
+        break;
+    case BK_REFINE_ADD:
+        hipLaunchKernelGGL((kb_args<RefineArgs, b_refine_add_args>), gr, bl, 0, s, (const RefineArgs*)args, map,
+                           active);
+        break;
+    case BK_REFINE_RES:
+        hipLaunchKernelGGL((kb_args<RefineArgs, b_refine_res_args>), gr, bl, 0, s, (const RefineArgs*)args, map,
+                           active);
+        break;
+    case BK_END:
+        hipLaunchKernelGGL((kb_args<EndArgs, b_end_step>), gr, bl, 0, s, (const EndArgs*)args, map, active); break;
+    case BK_OLF_H:
+        hipLaunchKernelGGL((kb_args<OlfArgs, b_olf_H>), gr, bl, 0, s, (const OlfArgs*)args, map, active); break;
+    case BK_OLF_FACES:
+        hipLaunchKernelGGL((kb_args<OlfArgs, b_olf_faces>), gr, bl, 0, s, (const OlfArgs*)args, map, active);
+        break;
+    case BK_OLF_ASM:
+        hipLaunchKernelGGL((kb_args<OlfArgs, b_olf_assemble>), gr, bl, 0, s, (const OlfArgs*)args, map, active);
+        break;
+    case BK_OLF_POST:
+        hipLaunchKernelGGL((kb_args<OlfArgs, b_olf_post>), gr, bl, 0, s, (const OlfArgs*)args, map, active);
+        break;
+    }
+}
+
+void launch_batched_spmv3(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,
+                          const int* active, const int* it_base, int k)
+{
+    if (fp32)
+        hipLaunchKernelGGL(kb_spmv3<float>, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<float>*)kargs, map,
+                           active, it_base, k);
+    else
+        hipLaunchKernelGGL(kb_spmv3<double>, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<double>*)kargs,
+                           map, active, it_base, k);
+}
+
+void launch_batched_update3(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,
+                            const int* active, const int* it_base, int k)
+{
+    if (fp32)
+        hipLaunchKernelGGL(kb_update3<float>, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<float>*)kargs,
+                           map, active, it_base, k);
+    else
+        hipLaunchKernelGGL(kb_update3<double>, dim3(nblocks), dim3(kBlock), 0, s,
+                           (const KrylovArgs<double>*)kargs, map, active, it_base, k);
+}
+
+void launch_batched_spmv2(hipStream_t s, const void* kargs, const BlkMap* map, int nblocks, const int* active,
+                          const int* it_base, int k)
+{
+    hipLaunchKernelGGL(kb_spmv2, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<double>*)kargs, map, active,
+                       it_base, k);
+}
+
+void launch_batched_update2(hipStream_t s, const void* kargs, const BlkMap* map, int nblocks, const int* active,
+                            const int* it_base, int k)
+{
+    hipLaunchKernelGGL(kb_update2, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<double>*)kargs, map,
+                       active, it_base, k);
+}
+
+void launch_advance(hipStream_t s, int* it_base, int by)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, it_base, by);
+}
+
+void launch_reduce_models(hipStream_t s, int nmodels, const double* const* part, const int* nparts, int nv,
+                          bool take_max, double* out)
+{
+    hipLaunchKernelGGL(k_reduce_models, dim3(nmodels), dim3(kBlock), 0, s, part, nparts, nv, take_max ? 1 : 0,
+                       out);
 }
 
 }  // namespace dh

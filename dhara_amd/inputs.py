@@ -151,9 +151,15 @@ def c3_coupled_mixed(**kw):
 
 def c4_adjoint(nx=64, ny=64, nz=30, perturbed=False):
     """Adjoint workload: 64x64x30, dz = 0.05, lnKs GRF (sigma 1, 10 m), per-cell
-    alpha ~ U[2,5] 1/m and n ~ U[1.3,1.8], 10 mm/h rain, dt = 60 s, 50 steps."""
+    alpha ~ U[2,5] 1/m and n ~ U[1.3,1.8], 10 mm/h rain, dt = 60 s, 50 steps.
+
+    The initial state is hydrostatic with the water table 2.0 m down (the 1.5 m column starts
+    unsaturated) and the Picard iterate is relaxed (am = 0.7): with cell-to-cell random n < 2 a
+    saturated zone inside the column, or am = 0, keeps the modified Picard iteration from
+    converging, and the adjoint is that of the converged step."""
     g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
-    p = _abi.default_params(dt=60.0, overland=1, top_bc=_abi.TOP_SURFACE, **LOAM)
+    p = _abi.default_params(dt=60.0, overland=1, top_bc=_abi.TOP_SURFACE, am=0.7, picard_maxiter=300,
+                            **LOAM)
     n3 = nx * ny * nz
     lnk = np.log(KS_LOAM) + grf((ny, nx), (1.0, 1.0), 1.0, 10.0, 3)
     lnks = np.tile(lnk.ravel(), nz)
@@ -166,7 +172,7 @@ def c4_adjoint(nx=64, ny=64, nz=30, perturbed=False):
         nvg = 1.0 + (nvg - 1.0) * (1.0 + 0.05 * rng.standard_normal(n3))
     fields = {
         "KSAT": np.exp(lnks), "ALPHA": alpha, "NVG": nvg,
-        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 1.0),
+        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 2.0),
         "ZTOPO": _surface(nx, ny, 1.0, 1),
         "MANNING": np.full(nx * ny, 0.03),
         "PONDING": np.zeros(nx * ny),
@@ -189,10 +195,13 @@ def c5_instances(count=256, seed=100):
 
 
 def c5_member(spec, dt=30.0, precision=_abi.MIXED):
+    """One ensemble member.  As in the adjoint workload (per-cell random n < 2), the column
+    starts unsaturated (hydrostatic, water table 2.5 m down) and the Picard iterate is relaxed
+    (am = 0.7) so that the modified Picard iteration converges."""
     nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
     g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
     p = _abi.default_params(dt=dt, overland=1, top_bc=_abi.TOP_SURFACE, precision=precision,
-                            **LOAM)
+                            am=0.7, picard_maxiter=100, olf_precond=_abi.OLF_PRECOND_JACOBI, **LOAM)
     seed = spec["seed"]
     n3 = nx * ny * nz
     lnk = np.log(KS_LOAM) + grf((ny, nx), (1.0, 1.0), spec["sigma_lnk"], 10.0, seed)
@@ -201,7 +210,7 @@ def c5_member(spec, dt=30.0, precision=_abi.MIXED):
         "KSAT": np.tile(np.exp(lnk).ravel(), nz),
         "ALPHA": rng.uniform(2.0, 5.0, n3),
         "NVG": rng.uniform(1.3, 1.8, n3),
-        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 1.0),
+        "PSI": hydrostatic_psi(nx, ny, nz, 0.05, 2.5),
         "ZTOPO": _surface(nx, ny, 1.0, seed + 13),
         "MANNING": np.full(nx * ny, 0.03),
         "PONDING": np.zeros(nx * ny),

@@ -236,6 +236,16 @@ def load_hip_library(path=HIP_LIB_PATH):
         lib.dhara_overland_conductance.restype = C.c_int
         lib.dhara_overland_precond.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         lib.dhara_overland_precond.restype = C.c_int
+        lib.dhara_ensemble_create.argtypes = [C.c_int32, P(C.c_void_p), P(C.c_void_p)]
+        lib.dhara_ensemble_create.restype = C.c_int
+        lib.dhara_ensemble_destroy.argtypes = [C.c_void_p]
+        lib.dhara_ensemble_destroy.restype = C.c_int
+        lib.dhara_ensemble_last_error.argtypes = [C.c_void_p]
+        lib.dhara_ensemble_last_error.restype = C.c_char_p
+        lib.dhara_ensemble_set_graph.argtypes = [C.c_void_p, C.c_int32]
+        lib.dhara_ensemble_set_graph.restype = C.c_int
+        lib.dhara_ensemble_step.argtypes = [C.c_void_p, C.c_int32, P(_abi.StepStats)]
+        lib.dhara_ensemble_step.restype = C.c_int
         lib.dhara_profile_enable.argtypes = [C.c_void_p, C.c_int32]
         lib.dhara_profile_enable.restype = C.c_int
         lib.dhara_profile_read.argtypes = [C.c_void_p, P(_abi.KernelProfile)]
@@ -329,3 +339,45 @@ class FlowModel(BaseModel):
     def set_precision(self, prec):
         self._check(self._lib.dhara_set_precision(self._h, prec), "set_precision")
         self._params.precision = prec
+
+
+class Ensemble:
+    """Many FlowModel members stepped together by batched launches (dhara_ensemble_*).
+    The members must be initialised, share one precision, and outlive the ensemble."""
+
+    def __init__(self, members):
+        self._lib = load_hip_library()
+        self.members = list(members)
+        n = len(self.members)
+        arr = (C.c_void_p * n)(*[m._h for m in self.members])
+        h = C.c_void_p()
+        rc = self._lib.dhara_ensemble_create(n, arr, C.byref(h))
+        if rc != 0:
+            raise DharaError(rc, "dhara_ensemble_create failed")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.dhara_ensemble_last_error(self._h)
+            raise DharaError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_graph(self, on=True):
+        self._check(self._lib.dhara_ensemble_set_graph(self._h, 1 if on else 0), "ensemble_set_graph")
+
+    def step(self, nsteps=1):
+        """Returns stats[step][member] as dicts."""
+        n = len(self.members)
+        arr = (_abi.StepStats * max(nsteps * n, 1))()
+        self._check(self._lib.dhara_ensemble_step(self._h, nsteps, arr), "ensemble_step")
+        return [[arr[s * n + m].as_dict() for m in range(n)] for s in range(nsteps)]
+
+    def close(self):
+        if self._h:
+            self._lib.dhara_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

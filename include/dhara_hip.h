@@ -306,7 +306,8 @@ int dhara_overland_precond(dhara_ctx* ctx, const double* r, double* z);
  * solves J^T lambda = dL/dpsi^{n+1} with J = dF/dpsi^{n+1} (BiCGStab on the transposed
  * 7-point Jacobian stencil) and A2^T mu = dL/dH^{n+1} for the overland system, and recomputes
  * the forward states of every checkpoint interval from its checkpoint.
- * Requirements: FP64 precision, am = 0, DHARA_THETA_CARRY_CONSISTENT, no Dirichlet faces.
+ * Requirements: FP64 precision, DHARA_THETA_CARRY_CONSISTENT, no Dirichlet faces, and a
+ * forward whose Picard iterations converge (the adjoint is that of the converged step).
  * The forcing fields in place at the call stay fixed for the whole run. */
 typedef struct dhara_loss_spec {
     int32_t nsteps;           /* steps run from the current state                          */
@@ -341,6 +342,25 @@ int dhara_adjoint_residual(dhara_ctx* ctx, const double* psi_new, double* F);
 /* out = J v (transpose = 0) or J^T v (transpose = 1), J = dF/dpsi_new at psi_new. */
 int dhara_adjoint_jvp(dhara_ctx* ctx, const double* psi_new, int32_t transpose, const double* v,
                       double* out);
+
+/* ---- batched ensemble -------------------------------------------------------------------
+ * No reference counterpart (the reference runs one model per process).  Many independent
+ * models of uneven size, each an initialised context, stepped together: every kernel is
+ * launched once for all members, members run the step in lockstep and drop out of each solve
+ * when their own stopping rule is met, and batches of Krylov iterations are replayed from a
+ * captured hipGraph.  All members must have the same precision and flux faces only.  Mixed
+ * precision is FP32 Jacobi-PCG with FP64 iterative refinement (the FP32 recurrence restarts
+ * after each refinement); the overland solve is Jacobi-PCG.  The members stay usable through
+ * the other entry points between ensemble steps and must outlive the ensemble. */
+typedef struct dhara_ensemble dhara_ensemble;
+int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble** out);
+int dhara_ensemble_destroy(dhara_ensemble* ens);
+const char* dhara_ensemble_last_error(const dhara_ensemble* ens);
+/* Krylov batches from a captured hipGraph (1, default) or as individual launches (0). */
+int dhara_ensemble_set_graph(dhara_ensemble* ens, int32_t on);
+/* nsteps coupled steps of every member; stats (may be NULL) holds nsteps*n entries,
+ * stats[step*n + member]. */
+int dhara_ensemble_step(dhara_ensemble* ens, int32_t nsteps, dhara_step_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -5,8 +5,8 @@
  * computes no sensitivities.  The forward it differentiates is the oracle's restatement of
  * src/subsurface.cu:686-847 (with src/global.cu:172-257) and src/overland.cu:387-419.
  *
- * Step n -> n+1 as differentiated here (SI units, flux boundaries only, am = 0, theta^n =
- * theta(psi^n)):
+ * Step n -> n+1 as differentiated here (SI units, flux boundaries only, theta^n = theta(psi^n);
+ * the Picard relaxation am only changes how the fixed point is reached, not the fixed point):
  *   surface   hp = h^n + (rain - evap) dt,  q = IdentifyTopBoundary(hp, qcapa(K(psi^n_top)),
  *             theta^n_top, Ks_top)  (one branch of a min/max chain, src/global.cu:172-257)
  *   Richards  F(psi^{n+1}) = 0, the fixed point of the modified Picard iteration:
@@ -453,7 +453,6 @@ void restore(dor_ctx& c, const std::vector<double>& psi, const std::vector<doubl
 int check_supported(dor_ctx* c)
 {
     if (!c->initialized) return fail(c, DHARA_E_STATE, "dor_init_state not called");
-    if (c->p.am != 0.0) return fail(c, DHARA_E_UNSUPPORTED, "adjoint needs am = 0");
     if (c->p.theta_carry != DHARA_THETA_CARRY_CONSISTENT)
         return fail(c, DHARA_E_UNSUPPORTED, "adjoint needs the consistent theta carry");
     for (int f = 0; f < 6; ++f) {

@@ -1,0 +1,151 @@
+"""Central finite-difference check of the discrete adjoint on the 64x64x30 adjoint workload.
+
+Used by tests/test_gpu_adjoint.py (a few cells) and runnable on its own for the full check:
+
+    python tests/c4_fd.py --cells 32 --out gpurun_out/c4_fd.md
+
+Forward tolerances are tightened for the check (Picard 1e-12 m, linear solves 1e-14 relative).
+The loss of a forward run carries ~1e-16 of FP64 solver rounding noise, which is not
+negligible against 2 h |g| for cells with small gradients; where the h = 1e-6 difference misses
+1e-4, a second difference at h = 1e-5 measures that noise.  Gradients below what a central
+difference can resolve at all (4 eps L / h) count as agreeing when the adjoint is below it too.  The perturbed forward runs go
+through independent contexts on several host threads (each context has its own stream), since
+one 64x64x30 run leaves most of the GPU idle.
+"""
+import argparse
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import dhara_amd  # noqa: E402
+from dhara_amd import _abi, inputs  # noqa: E402
+
+
+def c4_tight(perturbed=False):
+    w = inputs.c4_adjoint(perturbed=perturbed)
+    p = _abi.Params.from_buffer_copy(w.params)
+    p.picard_tol = 1e-12
+    p.lin_tol, p.olf_tol, p.lin_maxiter, p.olf_maxiter = 1e-14, 1e-14, 5000, 5000
+    w.params = p
+    return w
+
+
+def make(w, **override):
+    m = dhara_amd.FlowModel(w.grid, _abi.Params.from_buffer_copy(w.params))
+    f = dict(w.fields)
+    f.update(override)
+    m.upload_many(f)
+    m.upload("RAIN", w.rain_at(0))
+    m.init_state(_abi.INIT_FROM_PSI)
+    return m
+
+
+FIELDS = (("KSAT", "ln Ks", True), ("ALPHA", "alpha", False), ("NVG", "n", False))
+
+
+class C4Check:
+    def __init__(self, workers=4):
+        self.w = c4_tight()
+        self.T = self.w.nsteps
+        t0 = time.time()
+        tm = make(c4_tight(perturbed=True))
+        self.q, self.wt = tm.observe(self.T)
+        tm.close()
+        self.kw = dict(checkpoint_every=10, q_target=self.q, w_target=self.wt, w_q=1.0, w_w=1.0)
+        m = make(self.w)
+        self.loss, gk, ga, gn, self.stats = m.adjoint_run(self.T, **self.kw)
+        m.close()
+        self.grad = {"KSAT": gk, "ALPHA": ga, "NVG": gn}
+        self.adjoint_s = time.time() - t0
+        self.pool = ThreadPoolExecutor(max_workers=workers)
+
+    def _loss(self, key, c, log, h):
+        f = self.w.fields[key].copy()
+        f[c] = np.exp(np.log(f[c]) + h) if log else f[c] + h
+        m = make(self.w, **{key: f})
+        loss = m.adjoint_run(self.T, gradients=False, **self.kw)[0]
+        m.close()
+        return loss
+
+    def fd(self, jobs, h):
+        """Central differences for jobs = [(key, cell, log)], forward runs spread over the pool."""
+        futs = [(self.pool.submit(self._loss, k, c, lg, +h), self.pool.submit(self._loss, k, c, lg, -h))
+                for k, c, lg in jobs]
+        return [(a.result() - b.result()) / (2 * h) for a, b in futs]
+
+    def check(self, cells):
+        jobs = [(k, int(c), lg) for k, _, lg in FIELDS for c in cells]
+        fd1 = self.fd(jobs, 1e-6)
+        rows = []
+        redo = []
+        for (k, c, lg), f1 in zip(jobs, fd1):
+            g = float(self.grad[k][c])
+            rel = abs(g - f1) / abs(f1) if f1 != 0 else float("inf")
+            rows.append(dict(field=k, cell=c, adjoint=g, fd1=f1, rel1=rel, fd2=None, noise=None))
+            if not rel <= 1e-4:
+                redo.append(len(rows) - 1)
+        if redo:
+            fd2 = self.fd([jobs[i] for i in redo], 1e-5)
+            for i, f2 in zip(redo, fd2):
+                rows[i]["fd2"] = f2
+                rows[i]["noise"] = abs(rows[i]["fd1"] - f2)
+        # a central difference cannot resolve less than the rounding of the two losses, eps L / h
+        floor = 4 * np.finfo(float).eps * self.loss
+        for r in rows:
+            if r["rel1"] <= 1e-4 or abs(r["adjoint"] - r["fd1"]) <= floor / 1e-6:
+                r["ok"] = True
+            else:
+                r["ok"] = abs(r["adjoint"] - r["fd2"]) <= 1e-4 * abs(r["fd2"]) + 4 * r["noise"] + floor / 1e-5
+        return rows
+
+
+def seeded_cells(w, count, seed=7):
+    n = w.grid.nx * w.grid.ny * w.grid.nz
+    return np.random.default_rng(seed).choice(n, count, replace=False)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cells", type=int, default=32)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    chk = C4Check(workers=a.workers)
+    st = chk.stats
+    print(f"loss {chk.loss:.6e}; forward + adjoint {chk.adjoint_s:.1f} s; BiCGStab {st['lin_iters']}, "
+          f"overland CG {st['olf_iters']} iterations; converged {st['lin_converged']} {st['olf_converged']}",
+          flush=True)
+    cells = seeded_cells(chk.w, a.cells)
+    t0 = time.time()
+    rows = chk.check(cells)
+    S = chk.w.grid.nx * chk.w.grid.ny
+    lines = ["| parameter | cell (i, j, k) | adjoint | FD h=1e-6 | rel. diff | FD h=1e-5 | pass |",
+             "|---|---|---|---|---|---|---|"]
+    for r in rows:
+        c = r["cell"]
+        k, rem = divmod(c, S)
+        j, i = divmod(rem, chk.w.grid.nx)
+        f2 = "" if r["fd2"] is None else f"{r['fd2']:.6e}"
+        lines.append(f"| {dict((x, y) for x, y, _ in FIELDS)[r['field']]} | ({i}, {j}, {k}) | {r['adjoint']:.6e} | "
+                     f"{r['fd1']:.6e} | {r['rel1']:.1e} | {f2} | {'yes' if r['ok'] else 'NO'} |")
+    strict = sum(r["rel1"] <= 1e-4 for r in rows)
+    summary = (f"{len(rows)} checks ({a.cells} seeded cells x 3 parameters): {strict} within 1e-4 at h = 1e-6, "
+               f"{sum(r['ok'] for r in rows)} within 1e-4 + the measured rounding noise; "
+               f"max rel. diff at h = 1e-6 {max(r['rel1'] for r in rows):.1e}; FD time {time.time() - t0:.0f} s")
+    text = "\n".join(lines) + "\n\n" + summary + "\n"
+    print(text, flush=True)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(f"loss {chk.loss:.6e}, BiCGStab {st['lin_iters']} / overland CG {st['olf_iters']} reverse "
+                     f"iterations, adjoint wall {st['wall_ms']:.0f} ms\n\n" + text)
+
+
+if __name__ == "__main__":
+    main()

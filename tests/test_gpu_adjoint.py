@@ -86,50 +86,21 @@ def test_zero_misfit_and_checkpoint_interval():
     np.testing.assert_array_equal(m.download("PSI"), ref.download("PSI"))
 
 
-def c4_tight(perturbed=False):
-    """The 64x64x30 adjoint workload with the forward tolerances tightened for the
-    finite-difference check (Picard 1e-12 m, linear solves 1e-14 relative)."""
-    w = inputs.c4_adjoint(perturbed=perturbed)
-    p = _abi.Params.from_buffer_copy(w.params)
-    p.picard_tol, p.picard_maxiter = 1e-12, 200
-    p.lin_tol, p.olf_tol, p.lin_maxiter, p.olf_maxiter = 1e-14, 1e-14, 5000, 5000
-    w.params = p
-    return w
-
-
 def test_c4_gradient_finite_differences():
-    """Central differences (h = 1e-6) on seeded cells of the 64x64x30, 50-step workload.
-    Each check allows the loss's FP64 rounding noise, measured by the spread of two step sizes."""
-    w = c4_tight()
-    T = w.nsteps
-    q, wt = make_gpu(c4_tight(perturbed=True)).observe(T)
-    kw = dict(checkpoint_every=10, q_target=q, w_target=wt, w_q=1.0, w_w=1.0)
-    loss, gk, ga, gn, st = make_gpu(w).adjoint_run(T, **kw)
-    print(f"C4 loss {loss:.6e} bicg {st['lin_iters']} olf {st['olf_iters']} {st['wall_ms']:.0f} ms", flush=True)
-    assert st["lin_converged"] and st["olf_converged"] and loss > 0
-    S = w.grid.nx * w.grid.ny
-    rng = np.random.default_rng(7)
-    cells = np.concatenate([rng.choice(S, 3, replace=False), S + rng.choice(S, 1, replace=False)])
+    """Central differences (h = 1e-6) on seeded cells of the 64x64x30, 50-step adjoint workload
+    with tightened forward tolerances; where h = 1e-6 misses 1e-4 the FP64 rounding noise of the
+    loss, measured with a second step size, is allowed (tests/c4_fd.py)."""
+    from c4_fd import C4Check, seeded_cells
 
-    def fd(key, c, log, h):
-        vals = []
-        for sgn in (1, -1):
-            f = w.fields[key].copy()
-            f[c] = np.exp(np.log(f[c]) + sgn * h) if log else f[c] + sgn * h
-            vals.append(make_gpu(w, **{key: f}).adjoint_run(T, gradients=False, **kw)[0])
-        return (vals[0] - vals[1]) / (2 * h)
-
-    strict = 0
-    for key, g, log in (("KSAT", gk, True), ("ALPHA", ga, False), ("NVG", gn, False)):
-        for c in cells:
-            fd1, fd2 = fd(key, c, log, 1e-6), fd(key, c, log, 1e-5)
-            noise = abs(fd1 - fd2)
-            rel = abs(g[c] - fd1) / abs(fd1)
-            print(f"  {key} cell {c}: adjoint {g[c]: .6e} fd(1e-6) {fd1: .6e} fd(1e-5) {fd2: .6e} rel {rel:.1e}",
-                  flush=True)
-            assert abs(g[c] - fd2) <= 1e-4 * abs(fd2) + 4 * noise
-            strict += rel <= 1e-4
-    assert strict >= len(cells)
+    chk = C4Check(workers=4)
+    st = chk.stats
+    print(f"C4 loss {chk.loss:.6e} bicg {st['lin_iters']} olf {st['olf_iters']} {st['wall_ms']:.0f} ms", flush=True)
+    assert st["lin_converged"] and st["olf_converged"] and chk.loss > 0
+    rows = chk.check(seeded_cells(chk.w, 2))
+    for r in rows:
+        print(r, flush=True)
+    assert all(r["ok"] for r in rows)
+    assert sum(r["rel1"] <= 1e-4 for r in rows) >= len(rows) // 2
 
 
 def test_torch_autograd_wrapper():
@@ -162,7 +133,10 @@ def test_torch_autograd_wrapper():
     nvg = torch.tensor(w.fields["NVG"], device=dev, requires_grad=True)
     L = DharaLoss.apply(lnks, alpha, nvg)
     (2.0 * L).backward()
-    _, gk, ga, gn, _ = make_gpu(w).adjoint_run(T_STEPS, checkpoint_every=4, q_target=q, w_target=wt, w_q=1e6)
+    # same inputs as the wrapper (Ks went through log and exp)
+    ks_rt = np.exp(np.log(w.fields["KSAT"]))
+    _, gk, ga, gn, _ = make_gpu(w, KSAT=ks_rt).adjoint_run(T_STEPS, checkpoint_every=4, q_target=q, w_target=wt,
+                                                           w_q=1e6)
     np.testing.assert_array_equal(lnks.grad.cpu().numpy(), 2.0 * gk)
     np.testing.assert_array_equal(nvg.grad.cpu().numpy(), 2.0 * gn)
     # a small step against the gradient lowers the loss
