@@ -48,9 +48,14 @@ def parse():
     ap.add_argument("--refine-max", type=int, default=None, help="mixed: FP64 refinement sweeps per Picard iteration")
     ap.add_argument("--inner-tol", type=float, default=None, help="mixed: FP32 inner relative tolerance")
     ap.add_argument("--mixed-lin-tol", type=float, default=None, help="mixed: FP64 residual target")
+    ap.add_argument("--krylov", choices=["one", "two"], default="one",
+                    help="subsurface PCG: one launch per iteration (single-reduction recurrences) or the "
+                         "two-kernel Hestenes-Stiefel form")
     ap.add_argument("--workload", choices=["coupled", "ensemble"], default="coupled",
                     help="ensemble: the batched 256-member ensemble (200 steps, mixed precision)")
     ap.add_argument("--members", type=int, default=256, help="ensemble: number of members")
+    ap.add_argument("--prof-steps", type=int, default=3,
+                    help="steps after the timed region run with kernel timestamps (roofline)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run three plain 1 GiB device-to-device copies (for scaling HBM counters)")
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV", default=None,
@@ -82,18 +87,31 @@ def trace_summary(path):
 
 
 def calibrate_copy():
-    """Three plain device-to-device copies of 1 GiB (torch elementwise copy kernel)."""
-    import torch
-    n = (1 << 30) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
+    """Three device-to-device copies of 1 GiB (the HIP runtime's copy kernel: reads 1 GiB,
+    writes 1 GiB), the yardstick for the FETCH_SIZE / WRITE_SIZE counters.  Plain HIP calls
+    through ctypes, so the profiled process loads no other GPU library."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 0, ctypes.c_size_t(n)) != 0:
+        raise RuntimeError("hipMemset failed")
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"# calibration: 3 copies of {n * 8} bytes", file=sys.stderr, flush=True)
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"# calibration: 3 device-to-device copies of {n} bytes", file=sys.stderr, flush=True)
 
 
-def counted_bytes(stats, n3, n2, mixed, olf_mg=True):
+def counted_bytes(stats, n3, n2, mixed, olf_mg=True, one_kernel=False):
     """Algorithmic HBM bytes of the steps (per-kernel byte model of the solver)."""
     w = 4 if mixed else 8
     # overland CG iteration per surface cell: SpMV 6 and update 7 arrays; with the multigrid,
@@ -105,7 +123,13 @@ def counted_bytes(stats, n3, n2, mixed, olf_mg=True):
         P, I, R, Io = s["picard_iters"], s["lin_iters"], s["refine_sweeps"], s["olf_iters"]
         vg = 56 + (4 if mixed else 0)                  # read x, psi_m, Ks; write theta, K, C, psi_m
         asm = 80 if not mixed else 84                  # read 7 fields; write D, b, r (or D, b, D32, r32, d32)
-        per_cell = P * (vg + asm) + I * 13 * w + R * 64 + 48
+        if one_kernel:
+            # one launch per iteration: read r, D, p, K, x; write r, p, x (8 arrays), plus a
+            # start-up pass per segment (read r, D, K, p: 4 arrays)
+            krylov = I * 8 * w + (P + R) * 4 * w
+        else:
+            krylov = I * 13 * w
+        per_cell = P * (vg + asm) + krylov + R * 64 + 48
         per_surf = olf_iter * Io + 200                 # overland CG + surface/overland passes
         total += n3 * per_cell + n2 * per_surf
     return total
@@ -155,6 +179,9 @@ def main():
 
     w64 = inputs.c2_coupled(nx=nx, ny=ny, nz=nz)
     w32 = inputs.c3_coupled_mixed(nx=nx, ny=ny, nz=nz)
+    one_kernel = args.krylov == "one"
+    for w_ in (w64, w32):
+        w_.params.krylov = _abi.KRYLOV_ONE_KERNEL if one_kernel else _abi.KRYLOV_TWO_KERNEL
     if args.refine_max is not None:
         w32.params.refine_maxiter = args.refine_max
     if args.inner_tol is not None:
@@ -177,12 +204,20 @@ def main():
         tune_us = m.autotune()
         m.close()
         m = dhara_amd.FlowModel(w.grid, cp(w.params))
-        tiles[name] = {"fp64": m.get_tile(8), "fp32": m.get_tile(4), "tune_us": tune_us}
+        if one_kernel:
+            tiles[name] = {"fp64": m.get_krylov_tile(8), "fp32": m.get_krylov_tile(4), "tune_us": tune_us,
+                           "kernel": "k_cg1 (tx, ty, planes per tile, grid cap)"}
+        else:
+            tiles[name] = {"fp64": m.get_tile(8), "fp32": m.get_tile(4), "tune_us": tune_us,
+                           "kernel": "k_pcg3_spmv_tiled (tx, ty, planes per block)"}
         w.apply(m)
-        m.profile(True)
         print(f"# {name}: {K} steps", file=sys.stderr, flush=True)
         stats, secs, snaps = run_timed(m, w, K, marks=cpu_marks if name == "fp64" else ())
+        # kernel timestamps (an event pair per Krylov launch) only outside the timed region
+        m.profile(True)
+        w.run(m, args.prof_steps, start=K)
         prof = m.profile_read()
+        m.profile(False)
         psi = m.download("PSI")
         h = m.download("PONDING")
         m.close()
@@ -202,7 +237,7 @@ def main():
         "data": "synthetic (seeded GRF topography and log-normal Ks; no external data)",
         "config": {"workload": f"coupled {nx}x{ny}x{nz} (surface {nx}x{ny}), 20 mm/h rain 3600 s + "
                                f"3600 s recession, dt 30 s; {head} run timed over steps 0..{K - 1}",
-                   "precision": head, "device": dev, "spmv_tiles": tiles,
+                   "precision": head, "device": dev, "krylov": args.krylov, "tiles": tiles,
                    "mixed_solver": {"refine_maxiter": w32.params.refine_maxiter, "inner_tol": w32.params.inner_tol,
                                     "lin_tol": w32.params.lin_tol},
                    "fp64_solver": {"lin_tol": w64.params.lin_tol}},
@@ -218,7 +253,7 @@ def main():
             "overland_iters": sum(s["olf_iters"] for s in st),
             "picard_unconverged_steps": sum(1 for s in st if not s["picard_converged"]),
             "max_mb_rel_converged_steps": max((s["mb_rel"] for s in conv), default=None),
-            "counted_hbm_bytes": counted_bytes(st, n3, n2, name == "mixed"),
+            "counted_hbm_bytes": counted_bytes(st, n3, n2, name == "mixed", one_kernel=one_kernel),
         }
         out[name]["step_hbm_frac"] = out[name]["counted_hbm_bytes"] / (rr["secs"] * HBM_PEAK)
     if "fp64" in runs and "mixed" in runs:
@@ -230,7 +265,7 @@ def main():
     # roofline of the dominant kernel, from the library's HIP-event timing
     pr = r["prof"]
     kern = []
-    for kname, key in (("pcg3_spmv", "spmv"), ("pcg_update", "update")):
+    for kname, key in ((("cg1" if one_kernel else "pcg3_spmv"), "spmv"), ("pcg_update", "update")):
         nl = pr[f"{key}_launches"]
         if nl:
             avg = pr[f"{key}_ms"] / nl * 1e-3
@@ -243,7 +278,7 @@ def main():
         tpath = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tpath):
             try:
-                traffic = json.load(open(tpath)).get(f"k_pcg3_spmv<{prec}>", None) if kname == "pcg3_spmv" else None
+                traffic = json.load(open(tpath)).get(f"k_{kname}<{prec}>", None)
             except Exception:
                 traffic = None
         out["roofline"] = {"bound": "hbm", "kernel": f"k_{kname}<{prec}>", "achieved": byts / avg / 1e9,

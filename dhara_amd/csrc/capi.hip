@@ -32,13 +32,23 @@ namespace {
 
 struct TuneKey {
     int nx, ny, nz, elem;
-    bool operator==(const TuneKey& o) const { return nx == o.nx && ny == o.ny && nz == o.nz && elem == o.elem; }
+    int kind;   // 0: SpMV kernel of the two-kernel PCG, 1: one-kernel PCG
+    bool operator==(const TuneKey& o) const
+    {
+        return nx == o.nx && ny == o.ny && nz == o.nz && elem == o.elem && kind == o.kind;
+    }
 };
 struct TuneEntry {
     TuneKey key;
     dh::TileCfg cfg;
 };
 std::vector<TuneEntry> g_tune_table;   // process-wide autotuner results
+void remember_tile(const TuneKey& key, const dh::TileCfg& cfg)
+{
+    for (TuneEntry& te : g_tune_table)
+        if (te.key == key) { te.cfg = cfg; return; }
+    g_tune_table.push_back(TuneEntry{key, cfg});
+}
 
 // Default tile: 32 x 8 columns (128 B rows in FP32), planes per block chosen for ~2048 blocks.
 dh::TileCfg default_tile(const dh::Geo& g)
@@ -94,6 +104,8 @@ int validate_params(dhara_ctx* c, const dhara_params* p)
     if (p->picard_rule < 0 || p->picard_rule > 2) return set_err(c, DHARA_E_ARG, "bad picard_rule");
     if (p->top_bc != DHARA_TOP_SURFACE && p->top_bc != DHARA_TOP_PRESCRIBED)
         return set_err(c, DHARA_E_ARG, "bad top_bc");
+    if (p->krylov != DHARA_KRYLOV_TWO_KERNEL && p->krylov != DHARA_KRYLOV_ONE_KERNEL)
+        return set_err(c, DHARA_E_ARG, "bad krylov");
     return DHARA_OK;
 }
 
@@ -354,8 +366,126 @@ int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
             k.it = it_end;
             k.maxit = it_end;
             k.segment_start = 1;
+            k.first_sweep = 0;   // ||b||^2 is in the state since the first segment
             if (tc.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, tc, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
             else dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+            LAUNCHCHK(c);
+            if ((rc = read_state(c, c->st_sub))) return rc;
+            outer_conv = c->h_st->outer_conv != 0;
+            break;
+        }
+        it = it_end;
+        sc.first_sweep = 0;
+    }
+    st->lin_iters += c->h_st->iter;
+    st->refine_sweeps += updates;
+    st->lin_converged &= outer_conv ? 1 : 0;
+    return DHARA_OK;
+}
+
+// One-kernel PCG (no Dirichlet rows): a start-up kernel on the assembled residual, then one
+// launch per iteration.  Residual and search direction alternate between two buffers each
+// (q is free in this form): iteration it reads r[it&1], p[(it+1)&1] and writes r[(it+1)&1],
+// p[it&1], x in place.
+int solve_fp64_cg1(dhara_ctx* c, dhara_step_stats* st)
+{
+    dh::launch_assemble(c->s, asm_args(c, false), false);
+    LAUNCHCHK(c);
+    double* const rb[2] = {c->r, c->q};
+    double* const pb[2] = {c->pv[0], c->pv[1]};
+    const dh::TileCfg tc = c->cg1_tile[0];
+    const int nb = dh::cg1_blocks(c->g, tc);
+    dh::SolveCtl sc = solve_ctl(c, c->st_sub, c->p.lin_maxiter, dh::grid_for(c->N), false, true);
+    {
+        dh::SolveCtl k = sc;   // reads the assembly partials in part[0]
+        k.part_out = c->part[1];
+        dh::launch_cg1_init_f64(c->s, c->g, c->tile[0], c->K, c->D, rb[0], pb[1], k);
+    }
+    const int init_parts = dh::cg1_init_blocks(c->g, c->tile[0]);
+    int rc = run_krylov1(c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, c->prof, 8.0,
+                         [&](int it, const dh::LaunchEv& ev) {
+                             dh::SolveCtl k = sc;
+                             k.it = it;
+                             k.after_init = it == 0;
+                             k.part_in = c->part[(it + 1) & 1];
+                             k.nparts = it == 0 ? init_parts : nb;
+                             k.part_out = c->part[it & 1];
+                             dh::launch_cg1_f64(c->s, c->g, tc, c->K, c->D, rb[it & 1], rb[(it + 1) & 1],
+                                                pb[(it + 1) & 1], pb[it & 1], c->x, k, ev);
+                         });
+    if (rc) return rc;
+    st->lin_iters += c->h_st->iter;
+    st->lin_converged &= c->h_st->conv;
+    if (c->h_st->nan) st->nan_seen = 1;
+    return DHARA_OK;
+}
+
+// Mixed precision with the one-kernel PCG: the reliable-update scheme of solve_mixed; every
+// segment (first sweep, and after each FP64 update) starts with the start-up kernel, which
+// also makes the outer convergence test.
+int solve_mixed_cg1(dhara_ctx* c, dhara_step_stats* st)
+{
+    dh::launch_assemble(c->s, asm_args(c, false), true);
+    LAUNCHCHK(c);
+    float* const rb[2] = {c->r32, c->q32};
+    float* const pb[2] = {c->p32[0], c->p32[1]};
+    dh::RefineArgs ra{};
+    ra.g = c->g;
+    ra.K = c->K;
+    ra.D = c->D;
+    ra.b = c->b;
+    ra.x = c->x;
+    ra.flags = nullptr;
+    ra.D32 = c->D32;
+    ra.d32 = c->d32;
+    ra.part_out = c->part[0];
+    ra.st = c->st_sub;
+    const int maxit = c->p.inner_maxiter;
+    const dh::TileCfg tc = c->cg1_tile[1];
+    const int nb = dh::cg1_blocks(c->g, tc);
+    const int init_parts = dh::cg1_init_blocks(c->g, c->tile[1]);
+    dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, true);
+    int it = 0, updates = 0;
+    bool outer_conv = false;
+    for (;;) {
+        {
+            dh::SolveCtl k = sc;   // fresh residual partials in part[0]
+            k.it = it;
+            k.segment_start = it > 0 ? 1 : 0;
+            k.part_out = c->part[1];
+            dh::launch_cg1_init_f32(c->s, c->g, c->tile[1], c->K32, c->D32, rb[it & 1], pb[(it + 1) & 1], k);
+        }
+        const int seg = it;
+        int rc = run_krylov1(c, c->st_sub, maxit, c->pcg_hint, c->prof, 4.0,
+                             [&](int j, const dh::LaunchEv& ev) {
+                                 dh::SolveCtl k = sc;
+                                 k.it = j;
+                                 k.after_init = j == seg;
+                                 k.part_in = c->part[(j - seg + 1) & 1];
+                                 k.nparts = j == seg ? init_parts : nb;
+                                 k.part_out = c->part[(j - seg) & 1];
+                                 dh::launch_cg1_f32(c->s, c->g, tc, c->K32, c->D32, rb[j & 1], rb[(j + 1) & 1],
+                                                    pb[(j + 1) & 1], pb[j & 1], c->d32, k, ev);
+                             },
+                             it);
+        if (rc) return rc;
+        if (c->h_st->nan) { st->nan_seen = 1; break; }
+        if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
+        const int it_end = c->h_st->iter;
+        ra.r32 = rb[it_end & 1];
+        dh::launch_refine(c->s, ra);
+        LAUNCHCHK(c);
+        ++updates;
+        if (it_end >= maxit || updates > c->p.refine_maxiter) {
+            // final convergence check only: a start-up at the limit ends the solve
+            dh::SolveCtl k = sc;
+            k.it = it_end;
+            k.maxit = it_end;
+            k.segment_start = 1;
+            k.first_sweep = 0;
+            k.part_out = c->part[1];
+            dh::launch_cg1_init_f32(c->s, c->g, c->tile[1], c->K32, c->D32, rb[it_end & 1], pb[(it_end + 1) & 1],
+                                    k);
             LAUNCHCHK(c);
             if ((rc = read_state(c, c->st_sub))) return rc;
             outer_conv = c->h_st->outer_conv != 0;
@@ -397,7 +527,9 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
     int niter = 0, conv = 0;
     double delta = 0.0;
     while (!conv && niter < c->p.picard_maxiter) {
-        int rc = mixed ? solve_mixed(c, st) : solve_fp64(c, st);
+        const bool one = c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !dir;
+        int rc = mixed ? (one ? solve_mixed_cg1(c, st) : solve_mixed(c, st))
+                       : (one ? solve_fp64_cg1(c, st) : solve_fp64(c, st));
         if (rc) return rc;
         if (st->nan_seen) { c->theta_m_current = false; break; }
         dh::launch_vg_picard(c->s, vg_args(c, c->x, c->theta_new));
@@ -659,8 +791,12 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
     g2.cy = params->dt / (grid->dy * grid->dy);
     for (int e = 0; e < 2; ++e) {
         c->tile[e] = default_tile(g);
-        for (const TuneEntry& te : g_tune_table)
-            if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4}) c->tile[e] = te.cfg;
+        c->cg1_tile[e] = dh::TileCfg{32, 8, g.nz, 0};
+        while (dh::cg1_blocks(g, c->cg1_tile[e]) > dh::kMaxBlocks) c->cg1_tile[e].kc *= 2;
+        for (const TuneEntry& te : g_tune_table) {
+            if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4, 0}) c->tile[e] = te.cfg;
+            if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4, 1}) c->cg1_tile[e] = te.cfg;
+        }
     }
 
     const size_t N = c->N, S = c->S;
@@ -952,6 +1088,28 @@ int dhara_set_tile(dhara_ctx* c, int32_t elem_bytes, int32_t tx, int32_t ty, int
     return DHARA_OK;
 }
 
+int dhara_set_krylov_tile(dhara_ctx* c, int32_t elem_bytes, int32_t tx, int32_t ty, int32_t kc, int32_t grid)
+{
+    if (!c || (elem_bytes != 8 && elem_bytes != 4)) return DHARA_E_ARG;
+    dh::TileCfg t{tx, ty, kc, grid};
+    if (!dh::cg1_tile_supported(t)) return set_err(c, DHARA_E_ARG, "unsupported one-kernel PCG tile");
+    if (dh::cg1_blocks(c->g, t) > dh::kMaxBlocks) return set_err(c, DHARA_E_ARG, "tile grid too large; raise kc or cap the grid");
+    c->cg1_tile[elem_bytes == 8 ? 0 : 1] = t;
+    return DHARA_OK;
+}
+
+int dhara_get_krylov_tile(const dhara_ctx* c, int32_t elem_bytes, int32_t* tx, int32_t* ty, int32_t* kc,
+                          int32_t* grid)
+{
+    if (!c || (elem_bytes != 8 && elem_bytes != 4) || !tx || !ty || !kc || !grid) return DHARA_E_ARG;
+    const dh::TileCfg& t = c->cg1_tile[elem_bytes == 8 ? 0 : 1];
+    *tx = t.tx;
+    *ty = t.ty;
+    *kc = t.kc;
+    *grid = t.grid;
+    return DHARA_OK;
+}
+
 int dhara_get_tile(const dhara_ctx* c, int32_t elem_bytes, int32_t* tx, int32_t* ty, int32_t* kc)
 {
     if (!c || (elem_bytes != 8 && elem_bytes != 4) || !tx || !ty || !kc) return DHARA_E_ARG;
@@ -1057,11 +1215,65 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
             }
         c->tile[e] = best_cfg;
         best[e] = best_t;
-        const TuneKey key{c->g.nx, c->g.ny, c->g.nz, e == 0 ? 8 : 4};
-        bool found = false;
-        for (TuneEntry& te : g_tune_table)
-            if (te.key == key) { te.cfg = best_cfg; found = true; }
-        if (!found) g_tune_table.push_back(TuneEntry{key, best_cfg});
+        remember_tile(TuneKey{c->g.nx, c->g.ny, c->g.nz, e == 0 ? 8 : 4, 0}, best_cfg);
+    }
+    // one-kernel PCG: tile shape x planes per chunk x grid cap, on scratch output buffers
+    if (c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !fl) {
+        HIPCHK(c, hipMemcpyAsync(st, &tune, sizeof(tune), hipMemcpyHostToDevice, c->s));
+        const size_t N = c->N;
+        void *xs = nullptr, *rs = nullptr, *ps = nullptr;
+        HIPCHK(c, hipMalloc(&xs, N * 8));
+        HIPCHK(c, hipMalloc(&rs, N * 8));
+        HIPCHK(c, hipMalloc(&ps, N * 8));
+        HIPCHK(c, hipMemsetAsync(xs, 0, N * 8, c->s));
+        const int cshapes[4][2] = {{32, 8}, {16, 16}, {32, 16}, {64, 8}};
+        const int ckcs[3] = {c->g.nz, (c->g.nz + 1) / 2, (c->g.nz + 2) / 3};
+        const int grids[4] = {0, 512, 1024, 2048};
+        int rc2 = DHARA_OK;
+        for (int e = 0; e < 2 && !rc2; ++e) {
+            if (e == 1 && !mixed) continue;
+            float best_t = 1e30f;
+            dh::TileCfg best_cfg = c->cg1_tile[e];
+            for (const auto& sh : cshapes)
+                for (int kc : ckcs)
+                    for (int gcap : grids) {
+                        dh::TileCfg t{sh[0], sh[1], std::max(1, kc), gcap};
+                        const int nt = dh::tile_blocks(c->g, t);
+                        if (gcap != 0 && gcap >= nt) continue;   // same launch as gcap = 0
+                        if (dh::cg1_blocks(c->g, t) > dh::kMaxBlocks) continue;
+                        dh::SolveCtl sc{};
+                        sc.maxit = 1 << 30;
+                        sc.part_in = c->part[0];
+                        sc.nparts = dh::grid_for(c->N);
+                        sc.part_out = c->part[1];
+                        sc.st = st;
+                        float us = 0.f;
+                        if (e == 0)
+                            rc2 = time_spmv<double>(c, [&](int it) {
+                                dh::SolveCtl k = sc;
+                                k.it = it;
+                                dh::launch_cg1_f64(c->s, c->g, t, c->K, c->D, c->r, (double*)rs, c->pv[0],
+                                                   (double*)ps, (double*)xs, k);
+                            }, &us);
+                        else
+                            rc2 = time_spmv<float>(c, [&](int it) {
+                                dh::SolveCtl k = sc;
+                                k.it = it;
+                                dh::launch_cg1_f32(c->s, c->g, t, c->K32, c->D32, c->r32, (float*)rs, c->p32[0],
+                                                   (float*)ps, (float*)xs, k);
+                            }, &us);
+                        if (rc2) break;
+                        if (us < best_t) { best_t = us; best_cfg = t; }
+                    }
+            c->cg1_tile[e] = best_cfg;
+            best[e] = best_t;
+            remember_tile(TuneKey{c->g.nx, c->g.ny, c->g.nz, e == 0 ? 8 : 4, 1}, best_cfg);
+        }
+        (void)hipStreamSynchronize(c->s);
+        (void)hipFree(xs);
+        (void)hipFree(rs);
+        (void)hipFree(ps);
+        if (rc2) return rc2;
     }
     // the scratch state stays in c->allocs until dhara_destroy
     if (best_us) { best_us[0] = best[0]; best_us[1] = best[1]; }

@@ -63,6 +63,8 @@ struct dhara_ctx {
     std::vector<void*> allocs;
     // SpMV tile shapes: [0] FP64 solves, [1] FP32 inner solves (tx = 0: untiled kernel)
     dh::TileCfg tile[2] = {{32, 8, 0}, {32, 8, 0}};
+    // one-kernel PCG tiles: [0] FP64, [1] FP32 (kc = 0: whole columns)
+    dh::TileCfg cg1_tile[2] = {{32, 8, 0, 0}, {32, 8, 0, 0}};
     // kernel profiling (dhara_profile_enable)
     bool prof = false;
     std::vector<hipEvent_t> ev;
@@ -131,10 +133,10 @@ int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, 
                SpmvFn spmv, UpdFn upd, int it_start = 0)
 {
     int it = it_start;
-    // first batch: the previous solve's count (the next one is usually within +-2 of it);
-    // then small batches, since an over-queued iteration costs two empty launches and an
+    // first batch: the previous solve's count + the launch that detects the end (the next
+    // solve is usually within +-2 of it); then small batches, since an over-queued iteration costs two empty launches and an
     // under-queued one a host round trip
-    int batch = std::max(2, std::min(kMaxBatch, hint));
+    int batch = std::max(2, std::min(kMaxBatch, hint + 1));
     for (;;) {
         const int it0 = it;
         for (int b = 0; b < batch; ++b, ++it) {
@@ -163,6 +165,46 @@ int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, 
             c->prof_acc.update_launches += real;
             c->prof_acc.spmv_bytes = 6.0 * elem * c->N;
             c->prof_acc.update_bytes = 7.0 * elem * c->N;
+            c->prof_acc.elem_bytes = (int)elem;
+        }
+        if (c->h_st->done) break;
+        if (it > maxit + 1) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
+        batch = 3;
+    }
+    hint = std::max(1, c->h_st->iter - it_start);
+    return DHARA_OK;
+}
+
+// One launch per iteration (one-kernel PCG): the same batching as run_krylov; iteration `it`
+// of the batch is queued by kern(it, events).  Profiling counts every launch that did work
+// as an "spmv" launch of 8 algorithmic arrays.
+template <class KernFn>
+int run_krylov1(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem, KernFn kern,
+                int it_start = 0)
+{
+    int it = it_start;
+    int batch = std::max(2, std::min(kMaxBatch, hint + 1));
+    for (;;) {
+        const int it0 = it;
+        for (int b = 0; b < batch; ++b, ++it) {
+            dh::LaunchEv e;
+            if (prof) e = dh::LaunchEv{c->ev[2 * b], c->ev[2 * b + 1]};
+            kern(it, e);
+        }
+        LAUNCHCHK(c);
+        int rc = read_state(c, st);
+        if (rc) return rc;
+        const int done_iters = c->h_st->done ? c->h_st->iter : it;
+        if (prof) {
+            const int real = std::max(0, std::min(batch, done_iters - it0));
+            for (int b = 0; b < real; ++b) {
+                float m0 = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&m0, c->ev[2 * b], c->ev[2 * b + 1]));
+                c->prof_acc.spmv_ms += m0;
+            }
+            c->prof_acc.spmv_launches += real;
+            c->prof_acc.spmv_bytes = 8.0 * elem * c->N;
+            c->prof_acc.update_bytes = 0.0;
             c->prof_acc.elem_bytes = (int)elem;
         }
         if (c->h_st->done) break;

@@ -48,6 +48,7 @@ struct SolveCtl {
     int segment_start;    // part_in holds a fresh residual (assembly or FP64 reliable update):
                           // read all four partials, re-check the outer residual, restart the
                           // inner target, keep the search direction
+    int after_init;       // one-kernel PCG: part_in comes from the segment start-up kernel
     double lin_tol, inner_tol;
     const double* part_in;
     int nparts;
@@ -183,6 +184,7 @@ void launch_pcg_update_f32(hipStream_t s, int n, float* x, float* r, const float
 // neighbours stay in registers.  Returns the number of blocks (= partials for the update).
 struct TileCfg {
     int tx, ty, kc;
+    int grid = 0;   // one-kernel PCG: at most this many blocks (multiple of 8), 0 = one per tile
 };
 bool tile_supported(const TileCfg& t);
 int tile_blocks(const Geo& g, const TileCfg& t);
@@ -192,6 +194,24 @@ void launch_pcg3_spmv_tiled_f64(hipStream_t s, const Geo& g, const TileCfg& t, c
 void launch_pcg3_spmv_tiled_f32(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,
                                 const float* K, const float* D, const float* r, float* const p[2],
                                 float* q, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
+// One kernel per Jacobi-PCG iteration (single-reduction recurrences, no Dirichlet rows):
+// iteration sc.it reads r_in and p_in (= p_{it-1}), writes r_out, p_out and x in place, and
+// the partials [gamma, rho, delta, eps, pi]; k_cg1_init starts a segment from a fresh residual
+// (partials [rz, rr, bb, rr64] of the assembly or of a reliable update).
+bool cg1_tile_supported(const TileCfg& t);
+int cg1_blocks(const Geo& g, const TileCfg& t);
+void launch_cg1_f64(hipStream_t s, const Geo& g, const TileCfg& t, const double* K, const double* D,
+                    const double* r_in, double* r_out, const double* p_in, double* p_out, double* x,
+                    const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
+void launch_cg1_f32(hipStream_t s, const Geo& g, const TileCfg& t, const float* K, const float* D,
+                    const float* r_in, float* r_out, const float* p_in, float* p_out, float* x,
+                    const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
+// start-up kernel on the SpMV tile t (tx = 0: untiled); cg1_init_blocks = its partial count
+int cg1_init_blocks(const Geo& g, const TileCfg& t);
+void launch_cg1_init_f64(hipStream_t s, const Geo& g, const TileCfg& t, const double* K, const double* D,
+                         const double* r, const double* p_old, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
+void launch_cg1_init_f32(hipStream_t s, const Geo& g, const TileCfg& t, const float* K, const float* D,
+                         const float* r, const float* p_old, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 // y = A x (tests)
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                        const double* D, const double* x, double* y);

@@ -540,7 +540,9 @@ __device__ __forceinline__ int xcd_tile(int w, int nb)
     return x * q + (x < rem ? x : rem) + slot;
 }
 
-template <typename T, bool DIR, int TX, int TY>
+// INIT: start-up of a one-kernel PCG segment instead -- u = M^-1 r, A u, no stores, partials
+// [(r, u), (r, r), (u, A u), (p_old, A u), 0] (p_old: the kept direction, it > 0).
+template <typename T, bool DIR, int TX, int TY, bool INIT = false>
 __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_t* __restrict__ flags,
                                                              const T* __restrict__ K, const T* __restrict__ D,
                                                              const T* __restrict__ r, const T* __restrict__ pold,
@@ -549,7 +551,9 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
 {
     double beta_d;
     if (!pcg_begin(sc, beta_d, lin_bid())) return;
-    const bool first = (sc.it == 0);
+    const bool first = INIT || (sc.it == 0);
+    const bool kept = INIT && sc.it > 0;
+    double acc5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const T beta = (T)beta_d;
     __shared__ T sp[2][TY + 2][TX + 2];
     __shared__ T sk[2][TY + 2][TX + 2];
@@ -637,9 +641,17 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
                     qv -= wy * (half * (kcur + sk[buf][ty + 2][tx + 1])) * sp[buf][ty + 2][tx + 1];
                 if (k < g.nz - 1 && !(DIR && (fl & RF_CUT_U))) qv -= wz * (half * (kcur + kn)) * pn;
             }
-            pnew[c] = pc;
-            q[c] = qv;
-            acc += (double)pc * (double)qv;
+            if constexpr (INIT) {
+                const T rc = r[c];
+                acc5[0] += (double)rc * (double)pc;
+                acc5[1] += (double)rc * (double)rc;
+                acc5[2] += (double)pc * (double)qv;
+                if (kept) acc5[3] += (double)pold[c] * (double)qv;
+            } else {
+                pnew[c] = pc;
+                q[c] = qv;
+                acc += (double)pc * (double)qv;
+            }
         }
         pm = pc;
         km = kcur;
@@ -647,8 +659,12 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
         kcur = kn;
         dc = dn;
     }
-    double v[1] = {acc};
-    block_partials<1, false>(v, sc.part_out);
+    if constexpr (INIT) {
+        block_partials<5, false>(acc5, sc.part_out);
+    } else {
+        double v[1] = {acc};
+        block_partials<1, false>(v, sc.part_out);
+    }
 }
 
 template <typename T, bool DIR>
@@ -999,6 +1015,315 @@ __global__ __launch_bounds__(kBlock) void k_reduce_models(const double* const* _
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// One kernel per Jacobi-PCG iteration: the single-reduction recurrences of Chronopoulos &
+// Gear (1989).  With u = M^-1 r (M = diag(A)):
+//   p_i = u_i + beta_i p_{i-1},  s_i = A p_i,  x_{i+1} = x_i + alpha_i p_i,  r_{i+1} = r_i - alpha_i s_i
+//   gamma_{i+1} = (r_{i+1}, u_{i+1}),  delta_{i+1} = (u_{i+1}, A u_{i+1}),
+//   eps_{i+1} = (p_i, A u_{i+1}),  pi_i = (p_i, s_i),  rho_{i+1} = (r_{i+1}, r_{i+1})
+//   beta_{i+1} = gamma_{i+1} / gamma_i,  alpha_{i+1} = gamma_{i+1} / pi_{i+1},
+//   pi_{i+1} = (p_{i+1}, A p_{i+1}) = delta_{i+1} + 2 beta_{i+1} eps_{i+1} + beta_{i+1}^2 pi_i.
+// All five sums come out of the same kernel, so one launch is one iteration.  In exact
+// arithmetic the iterates are those of the two-kernel (Hestenes-Stiefel) PCG above, so the
+// stopping rule ||r|| <= tol ||b|| is met at the same iteration.  (p, Ap) is expanded from
+// inner products computed directly rather than from the classic recurrence
+// delta - beta gamma / alpha, which stays valid when the residual is replaced (mixed
+// precision) and the search direction kept.
+//
+// A u_{i+1} needs u_{i+1} on the neighbours, which needs s_i there, which needs p_i on the
+// neighbours' neighbours: a block owning TX x TY columns computes p_i on the tile plus a
+// two-cell halo (LDS), s_i, r_{i+1}, u_{i+1} on the tile plus a one-cell halo, and A u_{i+1} on
+// the tile.  Vertically the block marches through its planes k0..k1-1 as a pipeline: in the
+// step of plane m it computes p_i of plane m, u_{i+1} of plane m-1 and A u_{i+1} of plane m-2;
+// the own column's values stay in registers, the halo's come from LDS rings (p: 3 planes,
+// K: 4, u: 2), two barriers per plane.  Cells outside the domain hold p = u = K = 0, which
+// removes exactly the couplings the operator does not have.  Without Dirichlet rows only.
+// ---------------------------------------------------------------------------------------
+// Start of iteration sc.it: sum the five partials of the previous kernel, stop or return
+// alpha_i and beta_i.  After a start-up kernel (sc.after_init) the stopping test has been
+// made there and pi_{i-1} comes from the state (the partial is not available).
+__device__ __forceinline__ bool cg1_begin(const SolveCtl& sc, double& alpha, double& beta, int bid)
+{
+    PcgState* st = sc.st;
+    if (st->done) return false;
+    double in[5];
+    sum_partials<5>(sc.part_in, sc.nparts, 5, in);
+    const double gam = in[0], rr = in[1], del = in[2], eps = in[3], pi_direct = in[4];
+    const bool lead = bid == 0 && lin_tid() == 0;
+    if (!sc.after_init) {
+        const int conv = (rr <= st->tol2) ? 1 : 0;
+        const int nan = !isfinite(rr);
+        if (conv || nan || sc.it >= sc.maxit) {
+            if (lead) {
+                st->iter = sc.it;
+                st->conv = conv;
+                st->rr = rr;
+                st->nan = st->nan | nan;
+                if (!sc.mixed_inner) st->outer_conv = conv;
+                st->done = 1;
+            }
+            return false;
+        }
+    }
+    double b = 0.0, pi = del;
+    if (sc.it > 0) {
+        b = gam / st->rz[(sc.it - 1) & 1];
+        const double pi_prev = sc.after_init ? st->pi[(sc.it - 1) & 1] : pi_direct;
+        pi = del + 2.0 * b * eps + b * b * pi_prev;
+    }
+    const double a = gam / pi;
+    if (!(pi > 0.0) || !isfinite(a) || !isfinite(b)) {
+        if (lead) {
+            st->iter = sc.it;
+            st->conv = 0;
+            st->nan = 1;
+            st->done = 1;
+        }
+        return false;
+    }
+    if (lead) {
+        st->rz[sc.it & 1] = gam;
+        st->pi[sc.it & 1] = pi;
+    }
+    alpha = a;
+    beta = b;
+    return true;
+}
+
+// Start-up of a solve segment (fresh residual from the assembly or an FP64 reliable update):
+// the stopping test of pcg_begin on the four partials [rz, rr, bb, rr64], then
+// [gamma, rho, delta, eps, 0] of u = M^-1 r (eps against the kept direction p_{i-1}).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_init(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                                     const T* __restrict__ r, const T* __restrict__ p_old,
+                                                     SolveCtl sc)
+{
+    double unused;
+    if (!pcg_begin(sc, unused, lin_bid())) return;
+    const bool kept = sc.it > 0;
+    auto uf = [&](int c) -> T { return r[c] / D[c]; };
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        decompose(g, c, i, j, k);
+        const T rc = r[c];
+        const T uc = uf(c);
+        const T au = op3_row<T, false>(g, c, i, j, k, D[c], uc, uf, K, 0u);
+        acc[0] += (double)rc * (double)uc;
+        acc[1] += (double)rc * (double)rc;
+        acc[2] += (double)uc * (double)au;
+        if (kept) acc[3] += (double)p_old[c] * (double)au;
+    }
+    block_partials<5, false>(acc, sc.part_out, lin_bid());
+}
+
+// Local position (li, lj) of halo cell h of a TX x TY tile: first the one-cell ring (top row,
+// bottom row, left column, right column), then the second ring in the same order.
+template <int TX, int TY>
+__device__ __forceinline__ void cg1_halo_pos(int h, int& li, int& lj)
+{
+    constexpr int W1 = TX + 2, W2 = TX + 4, H1 = TY + 2;
+    constexpr int NH1 = 2 * W1 + 2 * TY;
+    if (h < NH1) {
+        if (h < W1) { lj = -1; li = h - 1; }
+        else if (h < 2 * W1) { lj = TY; li = h - W1 - 1; }
+        else if (h < 2 * W1 + TY) { li = -1; lj = h - 2 * W1; }
+        else { li = TX; lj = h - 2 * W1 - TY; }
+    } else {
+        const int q = h - NH1;
+        if (q < W2) { lj = -2; li = q - 2; }
+        else if (q < 2 * W2) { lj = TY + 1; li = q - W2 - 2; }
+        else if (q < 2 * W2 + H1) { li = -2; lj = q - 2 * W2 - 1; }
+        else { li = TX + 1; lj = q - 2 * W2 - H1 - 1; }
+    }
+}
+
+
+// The iteration kernel.  Blocks loop over tiles t = blockIdx.x, blockIdx.x + gridDim.x, ...
+// (gridDim.x a multiple of 8, so every tile of a block runs on the block's XCD and xcd_tile
+// keeps an XCD's tiles contiguous).
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                                 const T* __restrict__ r_in, T* __restrict__ r_out,
+                                                 const T* __restrict__ p_in, T* __restrict__ p_out,
+                                                 T* __restrict__ x, SolveCtl sc, int kc, int ntx, int nty, int nkz)
+{
+    constexpr int NT = TX * TY;
+    constexpr int W2 = TX + 4, A2 = W2 * (TY + 4);
+    constexpr int W1 = TX + 2, A1 = W1 * (TY + 2);
+    constexpr int NH1 = A1 - NT, NH2 = A2 - NT;
+    static_assert(NH1 <= NT && NH2 <= 2 * NT, "tile too small for its halo");
+    double alpha_d, beta_d;
+    if (!cg1_begin(sc, alpha_d, beta_d, blockIdx.x)) return;
+    const bool first = sc.it == 0;
+    const T alpha = (T)alpha_d, beta = (T)beta_d;
+    const T wx = (T)g.wx, wy = (T)g.wy, wz = (T)g.wz, half = T(0.5);
+
+    __shared__ T sp[3][A2];
+    __shared__ T sk[4][A2];
+    __shared__ T su[2][A1];
+
+    const int lid = threadIdx.x;
+    const int tx = lid % TX, ty = lid / TX;
+    const int oq2 = (ty + 2) * W2 + tx + 2;
+    const int oq1 = (ty + 1) * W1 + tx + 1;
+    // halo slots of this thread: h = lid (if < NH2) and h = lid + NT (if < NH2); for lid < NH1
+    // slot 0 is a cell of the one-cell ring, whose s, r, u this thread also computes
+    int hli[2], hlj[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        hli[s] = hlj[s] = 0;
+        if (lid + s * NT < NH2) cg1_halo_pos<TX, TY>(lid + s * NT, hli[s], hlj[s]);
+    }
+    const bool has_h0 = lid < NH2, has_h1 = lid + NT < NH2, ring = lid < NH1;
+    const int hq2_0 = (hlj[0] + 2) * W2 + hli[0] + 2, hq2_1 = (hlj[1] + 2) * W2 + hli[1] + 2;
+    const int rq1 = (hlj[0] + 1) * W1 + hli[0] + 1;
+
+    const int nb = ntx * nty * nkz;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // gamma, rho, delta, eps, pi
+    for (int tw = blockIdx.x; tw < nb; tw += gridDim.x) {
+        const int t = xcd_tile(tw, nb);
+        const int bz = t % nkz, bxy = t / nkz;
+        const int i0 = (bxy % ntx) * TX, j0 = (bxy / ntx) * TY;
+        const int k0 = bz * kc, k1 = min(k0 + kc, g.nz);
+        const int oi = i0 + tx, oj = j0 + ty;
+        const bool own_in = oi < g.nx && oj < g.ny;
+        const int ocol = oj * g.nx + oi;
+        const int h0i = i0 + hli[0], h0j = j0 + hlj[0], h1i = i0 + hli[1], h1j = j0 + hlj[1];
+        const bool h0_in = has_h0 && h0i >= 0 && h0j >= 0 && h0i < g.nx && h0j < g.ny;
+        const bool h1_in = has_h1 && h1i >= 0 && h1j >= 0 && h1i < g.nx && h1j < g.ny;
+        const int h0col = h0j * g.nx + h0i, h1col = h1j * g.nx + h1i;
+
+        // loads of plane m into the "next" registers (zeros outside the domain)
+        T nr = 0, nd = 1, np = 0, nk = 0, nx_ = 0;
+        T hr0 = 0, hd0 = 1, hp0 = 0, hk0 = 0, hr1 = 0, hd1 = 1, hp1 = 0, hk1 = 0;
+        auto load_plane = [&](int m) {
+            const bool pv = m >= 0 && m < g.nz;
+            const size_t off = (size_t)m * g.nxy;
+            if (pv && own_in) {
+                const size_t c = off + ocol;
+                nr = r_in[c]; nd = D[c]; nk = K[c]; nx_ = x[c];
+                np = first ? T(0) : p_in[c];
+            } else { nr = 0; nd = 1; nk = 0; nx_ = 0; np = 0; }
+            if (pv && h0_in) {
+                const size_t c = off + h0col;
+                hr0 = r_in[c]; hd0 = D[c]; hk0 = K[c];
+                hp0 = first ? T(0) : p_in[c];
+            } else { hr0 = 0; hd0 = 1; hk0 = 0; hp0 = 0; }
+            if (pv && h1_in) {
+                const size_t c = off + h1col;
+                hr1 = r_in[c]; hd1 = D[c]; hk1 = K[c];
+                hp1 = first ? T(0) : p_in[c];
+            } else { hr1 = 0; hd1 = 1; hk1 = 0; hp1 = 0; }
+        };
+
+        // own column pipeline (plane of each register relative to the step m)
+        T P1 = 0, P2 = 0;            // p(m-1), p(m-2)
+        T K1 = 0, K2 = 0, K3 = 0;    // K(m-1), K(m-2), K(m-3)
+        T D1 = 1, D2 = 1;            // D(m-1), D(m-2)
+        T R1 = 0, X1 = 0;            // r(m-1), x(m-1)
+        T U2 = 0, U3 = 0;            // u(m-2), u(m-3)
+        T RR1 = 0, RD1 = 1;          // ring cell: r(m-1), D(m-1)
+
+        load_plane(k0 - 2);
+        for (int m = k0 - 2; m <= k1 + 1; ++m) {
+            const int s3 = (m + 12) % 3, s4 = (m + 12) % 4;
+            // phase 1: p_i and K of plane m on the tile + two-cell halo
+            const T P0 = nr / nd + (first ? T(0) : beta * np);
+            const T K0 = nk, D0 = nd, R0 = nr, X0 = nx_;
+            sp[s3][oq2] = P0;
+            sk[s4][oq2] = K0;
+            const T RR0 = hr0, RD0 = hd0;
+            if (has_h0) {
+                sp[s3][hq2_0] = hr0 / hd0 + (first ? T(0) : beta * hp0);
+                sk[s4][hq2_0] = hk0;
+            }
+            if (has_h1) {
+                sp[s3][hq2_1] = hr1 / hd1 + (first ? T(0) : beta * hp1);
+                sk[s4][hq2_1] = hk1;
+            }
+            if (m + 1 <= k1 + 1) load_plane(m + 1);
+            __syncthreads();
+            // phase 3: s_i, r_{i+1}, u_{i+1} of plane m-1 on the tile + one-cell halo
+            const int pm = m - 1;
+            T U1 = 0;
+            if (pm >= k0 - 1) {
+                const T* spc = sp[(pm + 12) % 3];
+                const T* spu = sp[(pm - 1 + 12) % 3];   // plane m-2 (k-1)
+                const T* spd = sp[s3];                  // plane m   (k+1)
+                const T* skc = sk[(pm + 12) % 4];
+                const T* sku = sk[(pm - 1 + 12) % 4];
+                const T* skd = sk[s4];
+                const bool pv = pm >= 0 && pm < g.nz;
+                if (pv && own_in) {
+                    const int q = oq2;
+                    T sv = D1 * P1;
+                    sv -= wz * (half * (K1 + K2)) * P2;
+                    sv -= wy * (half * (K1 + skc[q - W2])) * spc[q - W2];
+                    sv -= wx * (half * (K1 + skc[q - 1])) * spc[q - 1];
+                    sv -= wx * (half * (K1 + skc[q + 1])) * spc[q + 1];
+                    sv -= wy * (half * (K1 + skc[q + W2])) * spc[q + W2];
+                    sv -= wz * (half * (K1 + K0)) * P0;
+                    const T rn = R1 - alpha * sv;
+                    U1 = rn / D1;
+                    if (pm >= k0 && pm < k1) {
+                        const size_t c = (size_t)pm * g.nxy + ocol;
+                        r_out[c] = rn;
+                        p_out[c] = P1;
+                        x[c] = X1 + alpha * P1;
+                        acc[0] += (double)rn * (double)U1;
+                        acc[1] += (double)rn * (double)rn;
+                        acc[4] += (double)P1 * (double)sv;
+                    }
+                }
+                su[(pm + 12) % 2][oq1] = U1;
+                if (ring) {
+                    T ur = 0;
+                    if (pv && h0_in) {
+                        const int q = hq2_0;
+                        const T kc_ = skc[q];
+                        T sv = RD1 * spc[q];
+                        sv -= wz * (half * (kc_ + sku[q])) * spu[q];
+                        sv -= wy * (half * (kc_ + skc[q - W2])) * spc[q - W2];
+                        sv -= wx * (half * (kc_ + skc[q - 1])) * spc[q - 1];
+                        sv -= wx * (half * (kc_ + skc[q + 1])) * spc[q + 1];
+                        sv -= wy * (half * (kc_ + skc[q + W2])) * spc[q + W2];
+                        sv -= wz * (half * (kc_ + skd[q])) * spd[q];
+                        ur = (RR1 - alpha * sv) / RD1;
+                    }
+                    su[(pm + 12) % 2][rq1] = ur;
+                }
+            }
+            __syncthreads();
+            // phase 5: A u_{i+1} of plane m-2 on the tile
+            const int pm2 = m - 2;
+            if (pm2 >= k0 && pm2 < k1 && own_in) {
+                const T* suc = su[(pm2 + 12) % 2];
+                const T* skc = sk[(pm2 + 12) % 4];
+                const int q = oq2, q1 = oq1;
+                T au = D2 * U2;
+                au -= wz * (half * (K2 + K3)) * U3;
+                au -= wy * (half * (K2 + skc[q - W2])) * suc[q1 - W1];
+                au -= wx * (half * (K2 + skc[q - 1])) * suc[q1 - 1];
+                au -= wx * (half * (K2 + skc[q + 1])) * suc[q1 + 1];
+                au -= wy * (half * (K2 + skc[q + W2])) * suc[q1 + W1];
+                au -= wz * (half * (K2 + K1)) * U1;
+                acc[2] += (double)U2 * (double)au;
+                acc[3] += (double)P2 * (double)au;
+            }
+            // shift the pipeline
+            P2 = P1; P1 = P0;
+            K3 = K2; K2 = K1; K1 = K0;
+            D2 = D1; D1 = D0;
+            R1 = R0; X1 = X0;
+            U3 = U2; U2 = U1;
+            RR1 = RR0; RD1 = RD0;
+        }
+        __syncthreads();   // the LDS rings are reused by the next tile
+    }
+    block_partials<5, false>(acc, sc.part_out, blockIdx.x);
+}
+
 }  // namespace
 
 // =========================================================================================
@@ -1319,6 +1644,91 @@ void launch_reduce_models(hipStream_t s, int nmodels, const double* const* part,
 {
     hipLaunchKernelGGL(k_reduce_models, dim3(nmodels), dim3(kBlock), 0, s, part, nparts, nv, take_max ? 1 : 0,
                        out);
+}
+
+// ---- one-kernel PCG iteration ----------------------------------------------------------
+bool cg1_tile_supported(const TileCfg& t)
+{
+    return t.kc >= 1 && t.grid >= 0 && (t.grid % 8) == 0 &&
+           ((t.tx == 32 && t.ty == 8) || (t.tx == 16 && t.ty == 16) || (t.tx == 32 && t.ty == 16) ||
+            (t.tx == 64 && t.ty == 8));
+}
+
+int cg1_blocks(const Geo& g, const TileCfg& t)
+{
+    const int nb = tile_blocks(g, t);
+    return (t.grid > 0 && t.grid < nb) ? t.grid : nb;
+}
+
+template <typename T, int TX, int TY>
+static void cg1_launch(hipStream_t s, const Geo& g, const TileCfg& t, const T* K, const T* D, const T* r_in,
+                       T* r_out, const T* p_in, T* p_out, T* x, const SolveCtl& sc, const LaunchEv& ev)
+{
+    const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, nkz = (g.nz + t.kc - 1) / t.kc;
+    launch_ev(k_cg1<T, TX, TY>, dim3(cg1_blocks(g, t)), dim3(TX * TY), s, ev, g, K, D, r_in, r_out, p_in, p_out,
+              x, sc, t.kc, ntx, nty, nkz);
+}
+
+template <typename T>
+static void cg1_dispatch(hipStream_t s, const Geo& g, const TileCfg& t, const T* K, const T* D, const T* r_in,
+                         T* r_out, const T* p_in, T* p_out, T* x, const SolveCtl& sc, const LaunchEv& ev)
+{
+    if (t.tx == 32 && t.ty == 8) cg1_launch<T, 32, 8>(s, g, t, K, D, r_in, r_out, p_in, p_out, x, sc, ev);
+    else if (t.tx == 16 && t.ty == 16) cg1_launch<T, 16, 16>(s, g, t, K, D, r_in, r_out, p_in, p_out, x, sc, ev);
+    else if (t.tx == 32 && t.ty == 16) cg1_launch<T, 32, 16>(s, g, t, K, D, r_in, r_out, p_in, p_out, x, sc, ev);
+    else cg1_launch<T, 64, 8>(s, g, t, K, D, r_in, r_out, p_in, p_out, x, sc, ev);
+}
+
+void launch_cg1_f64(hipStream_t s, const Geo& g, const TileCfg& t, const double* K, const double* D,
+                    const double* r_in, double* r_out, const double* p_in, double* p_out, double* x,
+                    const SolveCtl& sc, const LaunchEv& ev)
+{
+    cg1_dispatch<double>(s, g, t, K, D, r_in, r_out, p_in, p_out, x, sc, ev);
+}
+
+void launch_cg1_f32(hipStream_t s, const Geo& g, const TileCfg& t, const float* K, const float* D,
+                    const float* r_in, float* r_out, const float* p_in, float* p_out, float* x,
+                    const SolveCtl& sc, const LaunchEv& ev)
+{
+    cg1_dispatch<float>(s, g, t, K, D, r_in, r_out, p_in, p_out, x, sc, ev);
+}
+
+template <typename T, int TX, int TY>
+static void cg1_init_tiled(hipStream_t s, const Geo& g, int kc, const T* K, const T* D, const T* r,
+                           const T* p_old, const SolveCtl& sc, const LaunchEv& ev)
+{
+    const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, nkz = (g.nz + kc - 1) / kc;
+    launch_ev(k_pcg3_spmv_tiled<T, false, TX, TY, true>, dim3(ntx * nty * nkz), dim3(TX, TY), s, ev, g,
+              (const uint8_t*)nullptr, K, D, r, p_old, (T*)nullptr, (T*)nullptr, sc, kc, ntx, nty, nkz);
+}
+
+template <typename T>
+static void cg1_init(hipStream_t s, const Geo& g, const TileCfg& t, const T* K, const T* D, const T* r,
+                     const T* p_old, const SolveCtl& sc, const LaunchEv& ev)
+{
+    if (t.tx == 32 && t.ty == 8) cg1_init_tiled<T, 32, 8>(s, g, t.kc, K, D, r, p_old, sc, ev);
+    else if (t.tx == 64 && t.ty == 4) cg1_init_tiled<T, 64, 4>(s, g, t.kc, K, D, r, p_old, sc, ev);
+    else if (t.tx == 16 && t.ty == 16) cg1_init_tiled<T, 16, 16>(s, g, t.kc, K, D, r, p_old, sc, ev);
+    else if (t.tx == 32 && t.ty == 4) cg1_init_tiled<T, 32, 4>(s, g, t.kc, K, D, r, p_old, sc, ev);
+    else if (t.tx == 64 && t.ty == 8) cg1_init_tiled<T, 64, 8>(s, g, t.kc, K, D, r, p_old, sc, ev);
+    else launch_ev(k_cg1_init<T>, dim3(grid_for(g.n)), dim3(kBlock), s, ev, g, K, D, r, p_old, sc);
+}
+
+int cg1_init_blocks(const Geo& g, const TileCfg& t)
+{
+    return t.tx ? tile_blocks(g, t) : grid_for(g.n);
+}
+
+void launch_cg1_init_f64(hipStream_t s, const Geo& g, const TileCfg& t, const double* K, const double* D,
+                         const double* r, const double* p_old, const SolveCtl& sc, const LaunchEv& ev)
+{
+    cg1_init<double>(s, g, t, K, D, r, p_old, sc, ev);
+}
+
+void launch_cg1_init_f32(hipStream_t s, const Geo& g, const TileCfg& t, const float* K, const float* D,
+                         const float* r, const float* p_old, const SolveCtl& sc, const LaunchEv& ev)
+{
+    cg1_init<float>(s, g, t, K, D, r, p_old, sc, ev);
 }
 
 }  // namespace dh

@@ -254,6 +254,10 @@ def load_hip_library(path=HIP_LIB_PATH):
         lib.dhara_set_tile.restype = C.c_int
         lib.dhara_get_tile.argtypes = [C.c_void_p, C.c_int32] + [P(C.c_int32)] * 3
         lib.dhara_get_tile.restype = C.c_int
+        lib.dhara_set_krylov_tile.argtypes = [C.c_void_p] + [C.c_int32] * 5
+        lib.dhara_set_krylov_tile.restype = C.c_int
+        lib.dhara_get_krylov_tile.argtypes = [C.c_void_p, C.c_int32] + [P(C.c_int32)] * 4
+        lib.dhara_get_krylov_tile.restype = C.c_int
         lib.dhara_autotune.argtypes = [C.c_void_p, P(C.c_double)]
         lib.dhara_autotune.restype = C.c_int
         lib.dhara_set_precision.argtypes = [C.c_void_p, C.c_int32]
@@ -319,6 +323,16 @@ class FlowModel(BaseModel):
     def get_tile(self, elem_bytes):
         v = [C.c_int32() for _ in range(3)]
         self._check(self._lib.dhara_get_tile(self._h, elem_bytes, *[C.byref(x) for x in v]), "get_tile")
+        return tuple(x.value for x in v)
+
+    def set_krylov_tile(self, elem_bytes, tx, ty, kc, grid=0):
+        """Tile of the one-kernel PCG for FP64 (8) or FP32 (4) solves; grid caps the blocks."""
+        self._check(self._lib.dhara_set_krylov_tile(self._h, elem_bytes, tx, ty, kc, grid), "set_krylov_tile")
+
+    def get_krylov_tile(self, elem_bytes):
+        v = [C.c_int32() for _ in range(4)]
+        self._check(self._lib.dhara_get_krylov_tile(self._h, elem_bytes, *[C.byref(x) for x in v]),
+                    "get_krylov_tile")
         return tuple(x.value for x in v)
 
     def autotune(self):

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define DHARA_ABI_VERSION 1
+#define DHARA_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------------- */
 #define DHARA_OK            0
@@ -71,6 +71,13 @@ extern "C" {
 #define DHARA_OLF_PRECOND_JACOBI 0
 #define DHARA_OLF_PRECOND_MG     1   /* aggregation-multigrid V-cycle (same solution, far fewer
                                         iterations on ponded surfaces)                         */
+
+/* Kernel structure of the subsurface Jacobi-PCG (same method, same stopping rule)          */
+#define DHARA_KRYLOV_TWO_KERNEL 0   /* Hestenes-Stiefel recurrences, the reference's CUSP CG:
+                                       two launches per iteration                           */
+#define DHARA_KRYLOV_ONE_KERNEL 1   /* single-reduction recurrences (Chronopoulos-Gear): one
+                                       launch per iteration; without Dirichlet rows only, the
+                                       two-kernel form is used otherwise                    */
 
 /* State initialisation. */
 #define DHARA_INIT_FROM_THETA 0   /* psi <- vG^-1(theta), then theta,K,C <- vG(psi)
@@ -149,6 +156,7 @@ typedef struct dhara_params {
                                                      (src/subsurface.cu:223,259,294,328) */
     int32_t top_bc;                               /* DHARA_TOP_*                          */
     int32_t olf_precond;                          /* DHARA_OLF_PRECOND_* (overland CG)    */
+    int32_t krylov;                               /* DHARA_KRYLOV_* (subsurface PCG)      */
 } dhara_params;
 
 typedef struct dhara_step_stats {
@@ -201,6 +209,7 @@ static inline void dhara_params_defaults(dhara_params* p)
     p->dirichlet_face_quirk = 1;
     p->top_bc = DHARA_TOP_SURFACE;
     p->olf_precond = DHARA_OLF_PRECOND_MG;
+    p->krylov = DHARA_KRYLOV_TWO_KERNEL;
 }
 
 /* ---- lifecycle ------------------------------------------------------------------------ */
@@ -264,8 +273,15 @@ int dhara_profile_read(dhara_ctx* ctx, dhara_kernel_profile* out);
  * Supported (tx, ty): (32,8) (64,4) (16,16) (32,4) (64,8); kc >= 1 planes per block. */
 int dhara_set_tile(dhara_ctx* ctx, int32_t elem_bytes, int32_t tx, int32_t ty, int32_t kc);
 int dhara_get_tile(const dhara_ctx* ctx, int32_t elem_bytes, int32_t* tx, int32_t* ty, int32_t* kc);
+/* Tile of the one-kernel PCG (DHARA_KRYLOV_ONE_KERNEL): (tx, ty) in (32,8) (16,16) (32,16)
+ * (64,8), kc >= 1 planes per tile, grid = 0 (one block per tile) or a multiple of 8 capping
+ * the blocks, which then loop over tiles.  No reference counterpart (tuning hook). */
+int dhara_set_krylov_tile(dhara_ctx* ctx, int32_t elem_bytes, int32_t tx, int32_t ty, int32_t kc, int32_t grid);
+int dhara_get_krylov_tile(const dhara_ctx* ctx, int32_t elem_bytes, int32_t* tx, int32_t* ty, int32_t* kc,
+                          int32_t* grid);
 /* Times the candidate tile shapes on the context's current operator (HIP events, a few
- * launches each) for the precisions the context uses, keeps the fastest, and records it in a
+ * launches each) for the precisions the context uses -- of the SpMV kernel and, with
+ * DHARA_KRYLOV_ONE_KERNEL, of the one-kernel PCG -- keeps the fastest, and records it in a
  * process-wide table keyed by (nx, ny, nz, elem_bytes) that later contexts of the same shape
  * reuse without timing.  best_us (may be NULL) receives the winner's time per launch for
  * [FP64, FP32]. */
