@@ -527,7 +527,8 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
     int niter = 0, conv = 0;
     double delta = 0.0;
     while (!conv && niter < c->p.picard_maxiter) {
-        const bool one = c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !dir;
+        // (the one-kernel form addresses planes with 32-bit byte offsets)
+        const bool one = c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !dir && (size_t)c->N * 8 < (size_t(1) << 32);
         int rc = mixed ? (one ? solve_mixed_cg1(c, st) : solve_mixed(c, st))
                        : (one ? solve_fp64_cg1(c, st) : solve_fp64(c, st));
         if (rc) return rc;

@@ -9,6 +9,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace dh {
@@ -530,6 +532,31 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
 // so one barrier per plane suffices.  Blocks are numbered so that each XCD (workgroups are
 // dealt to the 8 XCDs round-robin) gets a contiguous band of tiles: neighbouring tiles, whose
 // halos overlap, then share that XCD's L2.
+// Jacobi scaling of the one-kernel PCG: u = r * jac_inv(D), the hardware reciprocal (FP32) or
+// the reciprocal refined by two Newton steps (FP64).  It is a fixed diagonal preconditioner like
+// r / D, cheaper than an IEEE division; every kernel of the one-kernel form uses this function,
+// so the preconditioner is the same in all of them.
+__device__ __forceinline__ float jac_inv(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ double jac_inv(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    y = fma(y, fma(-d, y, 1.0), y);
+    return fma(y, fma(-d, y, 1.0), y);
+}
+
+// Load / store at a 32-bit byte offset from a kernel-argument base (the scalar-base plus vector
+// offset addressing form; the caller guarantees that every offset fits in 32 bits).
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned byte_off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(T* base, unsigned byte_off, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 constexpr int kXcd = 8;
 
 __device__ __forceinline__ int xcd_tile(int w, int nb)
@@ -581,8 +608,9 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
     const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
     const int hcol = hj * g.nx + hi;
 
+    auto jac = [&](T rv, T dv) -> T { return INIT ? rv * jac_inv(dv) : rv / dv; };
     auto pval = [&](int c) -> T {
-        const T z = r[c] / D[c];
+        const T z = jac(r[c], D[c]);
         return first ? z : z + beta * pold[c];
     };
     const T wx = (T)g.wx, wy = (T)g.wy, wz = (T)g.wz, half = T(0.5);
@@ -597,7 +625,7 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
         if (k0 < k1) {
             const int c = k0 * g.nxy + col;
             dc = D[c];
-            const T z = r[c] / dc;
+            const T z = jac(r[c], dc);
             pc = first ? z : z + beta * pold[c];
             kcur = K[c];
         }
@@ -609,7 +637,7 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
         if (in && k + 1 < g.nz) {
             const int c = (k + 1) * g.nxy + col;
             dn = D[c];
-            const T z = r[c] / dn;
+            const T z = jac(r[c], dn);
             pn = first ? z : z + beta * pold[c];
             kn = K[c];
         }
@@ -1101,7 +1129,7 @@ __global__ __launch_bounds__(kBlock) void k_cg1_init(Geo g, const T* __restrict_
     double unused;
     if (!pcg_begin(sc, unused, lin_bid())) return;
     const bool kept = sc.it > 0;
-    auto uf = [&](int c) -> T { return r[c] / D[c]; };
+    auto uf = [&](int c) -> T { return r[c] * jac_inv(D[c]); };
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
         int i, j, k;
@@ -1120,8 +1148,9 @@ __global__ __launch_bounds__(kBlock) void k_cg1_init(Geo g, const T* __restrict_
 // Local position (li, lj) of halo cell h of a TX x TY tile: first the one-cell ring (top row,
 // bottom row, left column, right column), then the second ring in the same order.
 template <int TX, int TY>
-__device__ __forceinline__ void cg1_halo_pos(int h, int& li, int& lj)
+__device__ __forceinline__ int2 cg1_halo_pos(int h)
 {
+    int li, lj;
     constexpr int W1 = TX + 2, W2 = TX + 4, H1 = TY + 2;
     constexpr int NH1 = 2 * W1 + 2 * TY;
     if (h < NH1) {
@@ -1136,12 +1165,26 @@ __device__ __forceinline__ void cg1_halo_pos(int h, int& li, int& lj)
         else if (q < 2 * W2 + H1) { li = -2; lj = q - 2 * W2 - 1; }
         else { li = TX + 1; lj = q - 2 * W2 - H1 - 1; }
     }
+    return make_int2(li, lj);
 }
 
 
+// Global values of one plane for one thread: its own cell and its (up to two) halo cells.
+template <typename T>
+struct Cg1Plane {
+    T r, d, p, k, x;
+    T hr0, hd0, hp0, hk0;
+    T hr1, hd1, hp1, hk1;
+};
+
 // The iteration kernel.  Blocks loop over tiles t = blockIdx.x, blockIdx.x + gridDim.x, ...
 // (gridDim.x a multiple of 8, so every tile of a block runs on the block's XCD and xcd_tile
-// keeps an XCD's tiles contiguous).
+// keeps an XCD's tiles contiguous).  The stencil is evaluated in factored form: with
+// h = w / 2 per direction and q = K p,
+//   (A p)_c = D_c p_c - sum over directions h (K_c (p_nb1 + p_nb2) + (q_nb1 + q_nb2)),
+// which equals the arithmetic-mean face form; p and q (and u, K u for A u) are staged in LDS, so
+// a direction costs four operations.  The plane loop is unrolled six times, so the LDS ring
+// slots are compile-time constants and the register pipeline shifts are renames.
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K, const T* __restrict__ D,
                                                  const T* __restrict__ r_in, T* __restrict__ r_out,
@@ -1157,11 +1200,10 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
     if (!cg1_begin(sc, alpha_d, beta_d, blockIdx.x)) return;
     const bool first = sc.it == 0;
     const T alpha = (T)alpha_d, beta = (T)beta_d;
-    const T wx = (T)g.wx, wy = (T)g.wy, wz = (T)g.wz, half = T(0.5);
+    const T hx = (T)(0.5 * g.wx), hy = (T)(0.5 * g.wy), hz = (T)(0.5 * g.wz);
 
-    __shared__ T sp[3][A2];
-    __shared__ T sk[4][A2];
-    __shared__ T su[2][A1];
+    __shared__ T sp[3][A2], sq[3][A2];     // p, K p of planes m, m-1, m-2
+    __shared__ T su[2][A1], squ[2][A1];    // u, K u of planes m-1, m-2
 
     const int lid = threadIdx.x;
     const int tx = lid % TX, ty = lid / TX;
@@ -1169,15 +1211,13 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
     const int oq1 = (ty + 1) * W1 + tx + 1;
     // halo slots of this thread: h = lid (if < NH2) and h = lid + NT (if < NH2); for lid < NH1
     // slot 0 is a cell of the one-cell ring, whose s, r, u this thread also computes
-    int hli[2], hlj[2];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        hli[s] = hlj[s] = 0;
-        if (lid + s * NT < NH2) cg1_halo_pos<TX, TY>(lid + s * NT, hli[s], hlj[s]);
-    }
+    const int2 hp0 = lid < NH2 ? cg1_halo_pos<TX, TY>(lid) : make_int2(0, 0);
+    const int2 hp1 = lid + NT < NH2 ? cg1_halo_pos<TX, TY>(lid + NT) : make_int2(0, 0);
     const bool has_h0 = lid < NH2, has_h1 = lid + NT < NH2, ring = lid < NH1;
-    const int hq2_0 = (hlj[0] + 2) * W2 + hli[0] + 2, hq2_1 = (hlj[1] + 2) * W2 + hli[1] + 2;
-    const int rq1 = (hlj[0] + 1) * W1 + hli[0] + 1;
+    const int hq2_0 = (hp0.y + 2) * W2 + hp0.x + 2, hq2_1 = (hp1.y + 2) * W2 + hp1.x + 2;
+    const int rq1 = (hp0.y + 1) * W1 + hp0.x + 1;
+    constexpr unsigned ES = sizeof(T);
+    const unsigned plane_bytes = (unsigned)g.nxy * ES;
 
     const int nb = ntx * nty * nkz;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // gamma, rho, delta, eps, pi
@@ -1188,136 +1228,158 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
         const int k0 = bz * kc, k1 = min(k0 + kc, g.nz);
         const int oi = i0 + tx, oj = j0 + ty;
         const bool own_in = oi < g.nx && oj < g.ny;
-        const int ocol = oj * g.nx + oi;
-        const int h0i = i0 + hli[0], h0j = j0 + hlj[0], h1i = i0 + hli[1], h1j = j0 + hlj[1];
+        const int h0i = i0 + hp0.x, h0j = j0 + hp0.y, h1i = i0 + hp1.x, h1j = j0 + hp1.y;
         const bool h0_in = has_h0 && h0i >= 0 && h0j >= 0 && h0i < g.nx && h0j < g.ny;
         const bool h1_in = has_h1 && h1i >= 0 && h1j >= 0 && h1i < g.nx && h1j < g.ny;
-        const int h0col = h0j * g.nx + h0i, h1col = h1j * g.nx + h1i;
+        const unsigned obyte = (unsigned)(oj * g.nx + oi) * ES;
+        const unsigned h0byte = (unsigned)(h0j * g.nx + h0i) * ES, h1byte = (unsigned)(h1j * g.nx + h1i) * ES;
 
-        // loads of plane m into the "next" registers (zeros outside the domain)
-        T nr = 0, nd = 1, np = 0, nk = 0, nx_ = 0;
-        T hr0 = 0, hd0 = 1, hp0 = 0, hk0 = 0, hr1 = 0, hd1 = 1, hp1 = 0, hk1 = 0;
+        // loads of plane m (zeros outside the domain), one plane ahead of their use
+        Cg1Plane<T> L;
         auto load_plane = [&](int m) {
             const bool pv = m >= 0 && m < g.nz;
-            const size_t off = (size_t)m * g.nxy;
+            const unsigned off = (unsigned)m * plane_bytes;
             if (pv && own_in) {
-                const size_t c = off + ocol;
-                nr = r_in[c]; nd = D[c]; nk = K[c]; nx_ = x[c];
-                np = first ? T(0) : p_in[c];
-            } else { nr = 0; nd = 1; nk = 0; nx_ = 0; np = 0; }
+                const unsigned c = off + obyte;
+                L.r = ld_off(r_in, c); L.d = ld_off(D, c); L.k = ld_off(K, c); L.x = ld_off((const T*)x, c);
+                L.p = first ? T(0) : ld_off(p_in, c);
+            } else { L.r = 0; L.d = 1; L.k = 0; L.x = 0; L.p = 0; }
             if (pv && h0_in) {
-                const size_t c = off + h0col;
-                hr0 = r_in[c]; hd0 = D[c]; hk0 = K[c];
-                hp0 = first ? T(0) : p_in[c];
-            } else { hr0 = 0; hd0 = 1; hk0 = 0; hp0 = 0; }
+                const unsigned c = off + h0byte;
+                L.hr0 = ld_off(r_in, c); L.hd0 = ld_off(D, c); L.hk0 = ld_off(K, c);
+                L.hp0 = first ? T(0) : ld_off(p_in, c);
+            } else { L.hr0 = 0; L.hd0 = 1; L.hk0 = 0; L.hp0 = 0; }
             if (pv && h1_in) {
-                const size_t c = off + h1col;
-                hr1 = r_in[c]; hd1 = D[c]; hk1 = K[c];
-                hp1 = first ? T(0) : p_in[c];
-            } else { hr1 = 0; hd1 = 1; hk1 = 0; hp1 = 0; }
+                const unsigned c = off + h1byte;
+                L.hr1 = ld_off(r_in, c); L.hd1 = ld_off(D, c); L.hk1 = ld_off(K, c);
+                L.hp1 = first ? T(0) : ld_off(p_in, c);
+            } else { L.hr1 = 0; L.hd1 = 1; L.hk1 = 0; L.hp1 = 0; }
         };
 
         // own column pipeline (plane of each register relative to the step m)
-        T P1 = 0, P2 = 0;            // p(m-1), p(m-2)
-        T K1 = 0, K2 = 0, K3 = 0;    // K(m-1), K(m-2), K(m-3)
-        T D1 = 1, D2 = 1;            // D(m-1), D(m-2)
-        T R1 = 0, X1 = 0;            // r(m-1), x(m-1)
-        T U2 = 0, U3 = 0;            // u(m-2), u(m-3)
-        T RR1 = 0, RD1 = 1;          // ring cell: r(m-1), D(m-1)
-
-        load_plane(k0 - 2);
-        for (int m = k0 - 2; m <= k1 + 1; ++m) {
-            const int s3 = (m + 12) % 3, s4 = (m + 12) % 4;
-            // phase 1: p_i and K of plane m on the tile + two-cell halo
-            const T P0 = nr / nd + (first ? T(0) : beta * np);
-            const T K0 = nk, D0 = nd, R0 = nr, X0 = nx_;
-            sp[s3][oq2] = P0;
-            sk[s4][oq2] = K0;
-            const T RR0 = hr0, RD0 = hd0;
+        T P1 = 0, P2 = 0, Q1 = 0, Q2 = 0;   // p, K p of planes m-1, m-2
+        T K1 = 0, K2 = 0;                   // K(m-1), K(m-2)
+        T D1 = 1, D2 = 1, DI1 = 1;          // D(m-1), D(m-2), jac_inv(D(m-1))
+        T R1 = 0, X1 = 0;                   // r(m-1), x(m-1)
+        T U2 = 0, U3 = 0, QU2 = 0, QU3 = 0; // u, K u of planes m-2, m-3
+        T RR1 = 0, RD1 = 1, RDI1 = 1, RK1 = 0;   // ring cell: r, D, jac_inv(D), K of plane m-1
+        // step J (mod 6) of plane m; the LDS slots of the rings are compile-time constants:
+        // sp/sq hold planes m, m-1, m-2 in slots a0, a1, a2, su/squ planes m-1, m-2 in b0, b1
+        auto step = [&](auto J, int m) {
+            constexpr int j = decltype(J)::value;
+            constexpr int a0 = (3 - j % 3) % 3, a1 = (4 - j % 3) % 3, a2 = (5 - j % 3) % 3;
+            constexpr int b0 = j % 2, b1 = 1 - j % 2;
+            // phase 1: p_i, K p_i of plane m on the tile + two-cell halo
+            const T DI0 = jac_inv(L.d);
+            const T P0 = L.r * DI0 + (first ? T(0) : beta * L.p);
+            const T K0 = L.k, Q0 = L.k * P0, D0 = L.d, R0 = L.r, X0 = L.x;
+            sp[a0][oq2] = P0;
+            sq[a0][oq2] = Q0;
+            const T RR0 = L.hr0, RD0 = L.hd0, RK0 = L.hk0;
+            T RDI0 = 1;
             if (has_h0) {
-                sp[s3][hq2_0] = hr0 / hd0 + (first ? T(0) : beta * hp0);
-                sk[s4][hq2_0] = hk0;
+                RDI0 = jac_inv(L.hd0);
+                const T ph = L.hr0 * RDI0 + (first ? T(0) : beta * L.hp0);
+                sp[a0][hq2_0] = ph;
+                sq[a0][hq2_0] = L.hk0 * ph;
             }
             if (has_h1) {
-                sp[s3][hq2_1] = hr1 / hd1 + (first ? T(0) : beta * hp1);
-                sk[s4][hq2_1] = hk1;
+                const T ph = L.hr1 * jac_inv(L.hd1) + (first ? T(0) : beta * L.hp1);
+                sp[a0][hq2_1] = ph;
+                sq[a0][hq2_1] = L.hk1 * ph;
             }
             if (m + 1 <= k1 + 1) load_plane(m + 1);
             __syncthreads();
             // phase 3: s_i, r_{i+1}, u_{i+1} of plane m-1 on the tile + one-cell halo
             const int pm = m - 1;
-            T U1 = 0;
+            T U1 = 0, QU1 = 0;
             if (pm >= k0 - 1) {
-                const T* spc = sp[(pm + 12) % 3];
-                const T* spu = sp[(pm - 1 + 12) % 3];   // plane m-2 (k-1)
-                const T* spd = sp[s3];                  // plane m   (k+1)
-                const T* skc = sk[(pm + 12) % 4];
-                const T* sku = sk[(pm - 1 + 12) % 4];
-                const T* skd = sk[s4];
+                const T* pc = sp[a1];
+                const T* qc = sq[a1];
                 const bool pv = pm >= 0 && pm < g.nz;
                 if (pv && own_in) {
                     const int q = oq2;
+                    const T tx_ = fma(K1, pc[q - 1] + pc[q + 1], qc[q - 1] + qc[q + 1]);
+                    const T ty_ = fma(K1, pc[q - W2] + pc[q + W2], qc[q - W2] + qc[q + W2]);
+                    const T tz_ = fma(K1, P2 + P0, Q2 + Q0);
                     T sv = D1 * P1;
-                    sv -= wz * (half * (K1 + K2)) * P2;
-                    sv -= wy * (half * (K1 + skc[q - W2])) * spc[q - W2];
-                    sv -= wx * (half * (K1 + skc[q - 1])) * spc[q - 1];
-                    sv -= wx * (half * (K1 + skc[q + 1])) * spc[q + 1];
-                    sv -= wy * (half * (K1 + skc[q + W2])) * spc[q + W2];
-                    sv -= wz * (half * (K1 + K0)) * P0;
-                    const T rn = R1 - alpha * sv;
-                    U1 = rn / D1;
+                    sv = fma(-hx, tx_, sv);
+                    sv = fma(-hy, ty_, sv);
+                    sv = fma(-hz, tz_, sv);
+                    const T rn = fma(-alpha, sv, R1);
+                    U1 = rn * DI1;
+                    QU1 = K1 * U1;
                     if (pm >= k0 && pm < k1) {
-                        const size_t c = (size_t)pm * g.nxy + ocol;
-                        r_out[c] = rn;
-                        p_out[c] = P1;
-                        x[c] = X1 + alpha * P1;
+                        const unsigned c = (unsigned)pm * plane_bytes + obyte;
+                        st_off<T>(r_out, c, rn);
+                        st_off<T>(p_out, c, P1);
+                        st_off<T>(x, c, fma(alpha, P1, X1));
                         acc[0] += (double)rn * (double)U1;
                         acc[1] += (double)rn * (double)rn;
                         acc[4] += (double)P1 * (double)sv;
                     }
                 }
-                su[(pm + 12) % 2][oq1] = U1;
+                su[b0][oq1] = U1;
+                squ[b0][oq1] = QU1;
                 if (ring) {
-                    T ur = 0;
+                    T ur = 0, qur = 0;
                     if (pv && h0_in) {
                         const int q = hq2_0;
-                        const T kc_ = skc[q];
-                        T sv = RD1 * spc[q];
-                        sv -= wz * (half * (kc_ + sku[q])) * spu[q];
-                        sv -= wy * (half * (kc_ + skc[q - W2])) * spc[q - W2];
-                        sv -= wx * (half * (kc_ + skc[q - 1])) * spc[q - 1];
-                        sv -= wx * (half * (kc_ + skc[q + 1])) * spc[q + 1];
-                        sv -= wy * (half * (kc_ + skc[q + W2])) * spc[q + W2];
-                        sv -= wz * (half * (kc_ + skd[q])) * spd[q];
-                        ur = (RR1 - alpha * sv) / RD1;
+                        const T tx_ = fma(RK1, pc[q - 1] + pc[q + 1], qc[q - 1] + qc[q + 1]);
+                        const T ty_ = fma(RK1, pc[q - W2] + pc[q + W2], qc[q - W2] + qc[q + W2]);
+                        const T tz_ = fma(RK1, sp[a2][q] + sp[a0][q], sq[a2][q] + sq[a0][q]);
+                        T sv = RD1 * pc[q];
+                        sv = fma(-hx, tx_, sv);
+                        sv = fma(-hy, ty_, sv);
+                        sv = fma(-hz, tz_, sv);
+                        ur = fma(-alpha, sv, RR1) * RDI1;
+                        qur = RK1 * ur;
                     }
-                    su[(pm + 12) % 2][rq1] = ur;
+                    su[b0][rq1] = ur;
+                    squ[b0][rq1] = qur;
                 }
             }
             __syncthreads();
             // phase 5: A u_{i+1} of plane m-2 on the tile
             const int pm2 = m - 2;
             if (pm2 >= k0 && pm2 < k1 && own_in) {
-                const T* suc = su[(pm2 + 12) % 2];
-                const T* skc = sk[(pm2 + 12) % 4];
-                const int q = oq2, q1 = oq1;
+                const T* uc = su[b1];
+                const T* quc = squ[b1];
+                const int q1 = oq1;
+                const T tx_ = fma(K2, uc[q1 - 1] + uc[q1 + 1], quc[q1 - 1] + quc[q1 + 1]);
+                const T ty_ = fma(K2, uc[q1 - W1] + uc[q1 + W1], quc[q1 - W1] + quc[q1 + W1]);
+                const T tz_ = fma(K2, U3 + U1, QU3 + QU1);
                 T au = D2 * U2;
-                au -= wz * (half * (K2 + K3)) * U3;
-                au -= wy * (half * (K2 + skc[q - W2])) * suc[q1 - W1];
-                au -= wx * (half * (K2 + skc[q - 1])) * suc[q1 - 1];
-                au -= wx * (half * (K2 + skc[q + 1])) * suc[q1 + 1];
-                au -= wy * (half * (K2 + skc[q + W2])) * suc[q1 + W1];
-                au -= wz * (half * (K2 + K1)) * U1;
+                au = fma(-hx, tx_, au);
+                au = fma(-hy, ty_, au);
+                au = fma(-hz, tz_, au);
                 acc[2] += (double)U2 * (double)au;
                 acc[3] += (double)P2 * (double)au;
             }
             // shift the pipeline
-            P2 = P1; P1 = P0;
-            K3 = K2; K2 = K1; K1 = K0;
-            D2 = D1; D1 = D0;
+            P2 = P1; P1 = P0; Q2 = Q1; Q1 = Q0;
+            K2 = K1; K1 = K0;
+            D2 = D1; D1 = D0; DI1 = DI0;
             R1 = R0; X1 = X0;
-            U3 = U2; U2 = U1;
-            RR1 = RR0; RD1 = RD0;
+            U3 = U2; U2 = U1; QU3 = QU2; QU2 = QU1;
+            RR1 = RR0; RD1 = RD0; RDI1 = RDI0; RK1 = RK0;
+        };
+
+        load_plane(k0 - 2);
+        const int mend = k1 + 1;
+        for (int m = k0 - 2;;) {
+            step(std::integral_constant<int, 0>{}, m);
+            if (++m > mend) break;
+            step(std::integral_constant<int, 1>{}, m);
+            if (++m > mend) break;
+            step(std::integral_constant<int, 2>{}, m);
+            if (++m > mend) break;
+            step(std::integral_constant<int, 3>{}, m);
+            if (++m > mend) break;
+            step(std::integral_constant<int, 4>{}, m);
+            if (++m > mend) break;
+            step(std::integral_constant<int, 5>{}, m);
+            if (++m > mend) break;
         }
         __syncthreads();   // the LDS rings are reused by the next tile
     }
