@@ -278,7 +278,7 @@ def main():
         tpath = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tpath):
             try:
-                traffic = json.load(open(tpath)).get(f"k_{kname}<{prec}>", None)
+                traffic = json.load(open(tpath)).get(f"k_{kname}<{prec}>", None)   # HBM bytes per launch
             except Exception:
                 traffic = None
         out["roofline"] = {"bound": "hbm", "kernel": f"k_{kname}<{prec}>", "achieved": byts / avg / 1e9,

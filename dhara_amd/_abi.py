@@ -112,7 +112,7 @@ def default_params(**overrides):
         overland=1, olf_maxiter=2000, olf_tol=1e-12,
         hmin=1e-5, delta=1e-7, K0=1e-5,
         theta_carry=THETA_CARRY_CONSISTENT, dirichlet_face_quirk=1,
-        top_bc=TOP_SURFACE, olf_precond=OLF_PRECOND_MG, krylov=KRYLOV_TWO_KERNEL)
+        top_bc=TOP_SURFACE, olf_precond=OLF_PRECOND_MG, krylov=KRYLOV_ONE_KERNEL)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(f"unknown parameter {k!r}")

@@ -1216,6 +1216,10 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
     const bool has_h0 = lid < NH2, has_h1 = lid + NT < NH2, ring = lid < NH1;
     const int hq2_0 = (hp0.y + 2) * W2 + hp0.x + 2, hq2_1 = (hp1.y + 2) * W2 + hp1.x + 2;
     const int rq1 = (hp0.y + 1) * W1 + hp0.x + 1;
+    // lowest neighbour index of a stencil read (the j-1 neighbour), so that every LDS read of
+    // a stencil is a non-negative constant offset from one register
+    const unsigned obase = (unsigned)(oq2 - W2), ubase = (unsigned)(oq1 - W1);
+    const unsigned rbase = has_h0 && lid < NH1 ? (unsigned)(hq2_0 - W2) : 0u;
     constexpr unsigned ES = sizeof(T);
     const unsigned plane_bytes = (unsigned)g.nxy * ES;
 
@@ -1234,26 +1238,33 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
         const unsigned obyte = (unsigned)(oj * g.nx + oi) * ES;
         const unsigned h0byte = (unsigned)(h0j * g.nx + h0i) * ES, h1byte = (unsigned)(h1j * g.nx + h1i) * ES;
 
-        // loads of plane m (zeros outside the domain), one plane ahead of their use
+        // loads of plane m, one plane ahead of their use: only lanes whose cell is in the domain
+        // load, the others keep the zeros (D = 1) of zero_plane(); planes outside the domain are
+        // all zeros.  Lanes without a cell compute garbage where a neighbour is not zero, which
+        // is masked before it reaches LDS, memory or a sum.
         Cg1Plane<T> L;
-        auto load_plane = [&](int m) {
-            const bool pv = m >= 0 && m < g.nz;
+        auto zero_plane = [&]() {
+            L.r = 0; L.d = 1; L.k = 0; L.x = 0; L.p = 0;
+            L.hr0 = 0; L.hd0 = 1; L.hk0 = 0; L.hp0 = 0;
+            L.hr1 = 0; L.hd1 = 1; L.hk1 = 0; L.hp1 = 0;
+        };
+        auto load_plane = [&](int m) {   // 0 <= m < nz
             const unsigned off = (unsigned)m * plane_bytes;
-            if (pv && own_in) {
+            if (own_in) {
                 const unsigned c = off + obyte;
                 L.r = ld_off(r_in, c); L.d = ld_off(D, c); L.k = ld_off(K, c); L.x = ld_off((const T*)x, c);
-                L.p = first ? T(0) : ld_off(p_in, c);
-            } else { L.r = 0; L.d = 1; L.k = 0; L.x = 0; L.p = 0; }
-            if (pv && h0_in) {
+                if (!first) L.p = ld_off(p_in, c);
+            }
+            if (h0_in) {
                 const unsigned c = off + h0byte;
                 L.hr0 = ld_off(r_in, c); L.hd0 = ld_off(D, c); L.hk0 = ld_off(K, c);
-                L.hp0 = first ? T(0) : ld_off(p_in, c);
-            } else { L.hr0 = 0; L.hd0 = 1; L.hk0 = 0; L.hp0 = 0; }
-            if (pv && h1_in) {
+                if (!first) L.hp0 = ld_off(p_in, c);
+            }
+            if (h1_in) {
                 const unsigned c = off + h1byte;
                 L.hr1 = ld_off(r_in, c); L.hd1 = ld_off(D, c); L.hk1 = ld_off(K, c);
-                L.hp1 = first ? T(0) : ld_off(p_in, c);
-            } else { L.hr1 = 0; L.hd1 = 1; L.hk1 = 0; L.hp1 = 0; }
+                if (!first) L.hp1 = ld_off(p_in, c);
+            }
         };
 
         // own column pipeline (plane of each register relative to the step m)
@@ -1263,10 +1274,13 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
         T R1 = 0, X1 = 0;                   // r(m-1), x(m-1)
         T U2 = 0, U3 = 0, QU2 = 0, QU3 = 0; // u, K u of planes m-2, m-3
         T RR1 = 0, RD1 = 1, RDI1 = 1, RK1 = 0;   // ring cell: r, D, jac_inv(D), K of plane m-1
-        // step J (mod 6) of plane m; the LDS slots of the rings are compile-time constants:
-        // sp/sq hold planes m, m-1, m-2 in slots a0, a1, a2, su/squ planes m-1, m-2 in b0, b1
-        auto step = [&](auto J, int m) {
+
+        // step J (mod 6) of plane m.  The LDS ring slots are compile-time constants: sp/sq hold
+        // planes m, m-1, m-2 in slots a0, a1, a2, su/squ planes m-1, m-2 in b0, b1.  CHK =
+        // false: a step inside the steady range, where every plane test below is known true.
+        auto step = [&](auto J, auto CHK, int m) {
             constexpr int j = decltype(J)::value;
+            constexpr bool chk = decltype(CHK)::value;
             constexpr int a0 = (3 - j % 3) % 3, a1 = (4 - j % 3) % 3, a2 = (5 - j % 3) % 3;
             constexpr int b0 = j % 2, b1 = 1 - j % 2;
             // phase 1: p_i, K p_i of plane m on the tile + two-cell halo
@@ -1276,9 +1290,8 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
             sp[a0][oq2] = P0;
             sq[a0][oq2] = Q0;
             const T RR0 = L.hr0, RD0 = L.hd0, RK0 = L.hk0;
-            T RDI0 = 1;
+            const T RDI0 = jac_inv(L.hd0);
             if (has_h0) {
-                RDI0 = jac_inv(L.hd0);
                 const T ph = L.hr0 * RDI0 + (first ? T(0) : beta * L.hp0);
                 sp[a0][hq2_0] = ph;
                 sq[a0][hq2_0] = L.hk0 * ph;
@@ -1288,32 +1301,38 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
                 sp[a0][hq2_1] = ph;
                 sq[a0][hq2_1] = L.hk1 * ph;
             }
-            if (m + 1 <= k1 + 1) load_plane(m + 1);
+            if (!chk) {
+                load_plane(m + 1);
+            } else if (m + 1 <= k1 + 1) {
+                if (m + 1 >= 0 && m + 1 < g.nz) load_plane(m + 1);
+                else zero_plane();
+            }
             __syncthreads();
             // phase 3: s_i, r_{i+1}, u_{i+1} of plane m-1 on the tile + one-cell halo
             const int pm = m - 1;
             T U1 = 0, QU1 = 0;
-            if (pm >= k0 - 1) {
-                const T* pc = sp[a1];
-                const T* qc = sq[a1];
-                const bool pv = pm >= 0 && pm < g.nz;
-                if (pv && own_in) {
-                    const int q = oq2;
-                    const T tx_ = fma(K1, pc[q - 1] + pc[q + 1], qc[q - 1] + qc[q + 1]);
-                    const T ty_ = fma(K1, pc[q - W2] + pc[q + W2], qc[q - W2] + qc[q + W2]);
+            if (!chk || pm >= k0 - 1) {
+                const bool pv = !chk || (pm >= 0 && pm < g.nz);
+                if (pv) {
+                    const T* pc = &sp[a1][obase];   // obase = oq2 - W2: neighbours at 0, W2-1, W2+1, 2 W2
+                    const T* qc = &sq[a1][obase];
+                    const T tx_ = fma(K1, pc[W2 - 1] + pc[W2 + 1], qc[W2 - 1] + qc[W2 + 1]);
+                    const T ty_ = fma(K1, pc[0] + pc[2 * W2], qc[0] + qc[2 * W2]);
                     const T tz_ = fma(K1, P2 + P0, Q2 + Q0);
                     T sv = D1 * P1;
                     sv = fma(-hx, tx_, sv);
                     sv = fma(-hy, ty_, sv);
                     sv = fma(-hz, tz_, sv);
-                    const T rn = fma(-alpha, sv, R1);
+                    const T rn = own_in ? fma(-alpha, sv, R1) : T(0);
                     U1 = rn * DI1;
                     QU1 = K1 * U1;
-                    if (pm >= k0 && pm < k1) {
-                        const unsigned c = (unsigned)pm * plane_bytes + obyte;
-                        st_off<T>(r_out, c, rn);
-                        st_off<T>(p_out, c, P1);
-                        st_off<T>(x, c, fma(alpha, P1, X1));
+                    if (!chk || (pm >= k0 && pm < k1)) {
+                        if (own_in) {
+                            const unsigned c = (unsigned)pm * plane_bytes + obyte;
+                            st_off<T>(r_out, c, rn);
+                            st_off<T>(p_out, c, P1);
+                            st_off<T>(x, c, fma(alpha, P1, X1));
+                        }
                         acc[0] += (double)rn * (double)U1;
                         acc[1] += (double)rn * (double)rn;
                         acc[4] += (double)P1 * (double)sv;
@@ -1323,16 +1342,17 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
                 squ[b0][oq1] = QU1;
                 if (ring) {
                     T ur = 0, qur = 0;
-                    if (pv && h0_in) {
-                        const int q = hq2_0;
-                        const T tx_ = fma(RK1, pc[q - 1] + pc[q + 1], qc[q - 1] + qc[q + 1]);
-                        const T ty_ = fma(RK1, pc[q - W2] + pc[q + W2], qc[q - W2] + qc[q + W2]);
-                        const T tz_ = fma(RK1, sp[a2][q] + sp[a0][q], sq[a2][q] + sq[a0][q]);
-                        T sv = RD1 * pc[q];
+                    if (pv) {
+                        const T* pc = &sp[a1][rbase];   // rbase = hq2_0 - W2
+                        const T* qc = &sq[a1][rbase];
+                        const T tx_ = fma(RK1, pc[W2 - 1] + pc[W2 + 1], qc[W2 - 1] + qc[W2 + 1]);
+                        const T ty_ = fma(RK1, pc[0] + pc[2 * W2], qc[0] + qc[2 * W2]);
+                        const T tz_ = fma(RK1, sp[a2][hq2_0] + sp[a0][hq2_0], sq[a2][hq2_0] + sq[a0][hq2_0]);
+                        T sv = RD1 * pc[W2];
                         sv = fma(-hx, tx_, sv);
                         sv = fma(-hy, ty_, sv);
                         sv = fma(-hz, tz_, sv);
-                        ur = fma(-alpha, sv, RR1) * RDI1;
+                        ur = h0_in ? fma(-alpha, sv, RR1) * RDI1 : T(0);
                         qur = RK1 * ur;
                     }
                     su[b0][rq1] = ur;
@@ -1340,14 +1360,13 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
                 }
             }
             __syncthreads();
-            // phase 5: A u_{i+1} of plane m-2 on the tile
+            // phase 5: A u_{i+1} of plane m-2 on the tile (lanes without a cell have U2 = P2 = 0)
             const int pm2 = m - 2;
-            if (pm2 >= k0 && pm2 < k1 && own_in) {
-                const T* uc = su[b1];
-                const T* quc = squ[b1];
-                const int q1 = oq1;
-                const T tx_ = fma(K2, uc[q1 - 1] + uc[q1 + 1], quc[q1 - 1] + quc[q1 + 1]);
-                const T ty_ = fma(K2, uc[q1 - W1] + uc[q1 + W1], quc[q1 - W1] + quc[q1 + W1]);
+            if (!chk || (pm2 >= k0 && pm2 < k1)) {
+                const T* uc = &su[b1][ubase];   // ubase = oq1 - W1
+                const T* quc = &squ[b1][ubase];
+                const T tx_ = fma(K2, uc[W1 - 1] + uc[W1 + 1], quc[W1 - 1] + quc[W1 + 1]);
+                const T ty_ = fma(K2, uc[0] + uc[2 * W1], quc[0] + quc[2 * W1]);
                 const T tz_ = fma(K2, U3 + U1, QU3 + QU1);
                 T au = D2 * U2;
                 au = fma(-hx, tx_, au);
@@ -1365,21 +1384,26 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
             RR1 = RR0; RD1 = RD0; RDI1 = RDI0; RK1 = RK0;
         };
 
-        load_plane(k0 - 2);
+        zero_plane();
+        if (k0 - 2 >= 0) load_plane(k0 - 2);
+        // (a second, check-free copy of the steps for the steady range measured slower: it
+        // raises the register count and lowers the occupancy more than it saves)
         const int mend = k1 + 1;
-        for (int m = k0 - 2;;) {
-            step(std::integral_constant<int, 0>{}, m);
-            if (++m > mend) break;
-            step(std::integral_constant<int, 1>{}, m);
-            if (++m > mend) break;
-            step(std::integral_constant<int, 2>{}, m);
-            if (++m > mend) break;
-            step(std::integral_constant<int, 3>{}, m);
-            if (++m > mend) break;
-            step(std::integral_constant<int, 4>{}, m);
-            if (++m > mend) break;
-            step(std::integral_constant<int, 5>{}, m);
-            if (++m > mend) break;
+        using Chk = std::integral_constant<bool, true>;
+        for (int m = k0 - 2; m <= mend; m += 6) {
+            {
+                step(std::integral_constant<int, 0>{}, Chk{}, m);
+                if (m + 1 > mend) break;
+                step(std::integral_constant<int, 1>{}, Chk{}, m + 1);
+                if (m + 2 > mend) break;
+                step(std::integral_constant<int, 2>{}, Chk{}, m + 2);
+                if (m + 3 > mend) break;
+                step(std::integral_constant<int, 3>{}, Chk{}, m + 3);
+                if (m + 4 > mend) break;
+                step(std::integral_constant<int, 4>{}, Chk{}, m + 4);
+                if (m + 5 > mend) break;
+                step(std::integral_constant<int, 5>{}, Chk{}, m + 5);
+            }
         }
         __syncthreads();   // the LDS rings are reused by the next tile
     }

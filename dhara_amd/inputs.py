@@ -197,11 +197,14 @@ def c5_instances(count=256, seed=100):
 def c5_member(spec, dt=30.0, precision=_abi.MIXED):
     """One ensemble member.  As in the adjoint workload (per-cell random n < 2), the column
     starts unsaturated (hydrostatic, water table 2.5 m down) and the Picard iterate is relaxed
-    (am = 0.7) so that the modified Picard iteration converges."""
+    (am = 0.7) so that the modified Picard iteration converges.  The batched ensemble runs the
+    two-kernel PCG and the Jacobi-preconditioned overland CG, so a member stepped on its own
+    uses the same forms."""
     nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
     g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
     p = _abi.default_params(dt=dt, overland=1, top_bc=_abi.TOP_SURFACE, precision=precision,
-                            am=0.7, picard_maxiter=100, olf_precond=_abi.OLF_PRECOND_JACOBI, **LOAM)
+                            am=0.7, picard_maxiter=100, olf_precond=_abi.OLF_PRECOND_JACOBI,
+                            krylov=_abi.KRYLOV_TWO_KERNEL, **LOAM)
     seed = spec["seed"]
     n3 = nx * ny * nz
     lnk = np.log(KS_LOAM) + grf((ny, nx), (1.0, 1.0), spec["sigma_lnk"], 10.0, seed)

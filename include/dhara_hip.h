@@ -76,8 +76,8 @@ extern "C" {
 #define DHARA_KRYLOV_TWO_KERNEL 0   /* Hestenes-Stiefel recurrences, the reference's CUSP CG:
                                        two launches per iteration                           */
 #define DHARA_KRYLOV_ONE_KERNEL 1   /* single-reduction recurrences (Chronopoulos-Gear): one
-                                       launch per iteration; without Dirichlet rows only, the
-                                       two-kernel form is used otherwise                    */
+                                       launch per iteration (default); without Dirichlet rows
+                                       only, the two-kernel form is used otherwise          */
 
 /* State initialisation. */
 #define DHARA_INIT_FROM_THETA 0   /* psi <- vG^-1(theta), then theta,K,C <- vG(psi)
@@ -209,7 +209,7 @@ static inline void dhara_params_defaults(dhara_params* p)
     p->dirichlet_face_quirk = 1;
     p->top_bc = DHARA_TOP_SURFACE;
     p->olf_precond = DHARA_OLF_PRECOND_MG;
-    p->krylov = DHARA_KRYLOV_TWO_KERNEL;
+    p->krylov = DHARA_KRYLOV_ONE_KERNEL;
 }
 
 /* ---- lifecycle ------------------------------------------------------------------------ */

@@ -5,12 +5,15 @@ Used by tests/test_gpu_adjoint.py (a few cells) and runnable on its own for the 
     python tests/c4_fd.py --cells 32 --out gpurun_out/c4_fd.md
 
 Forward tolerances are tightened for the check (Picard 1e-12 m, linear solves 1e-14 relative).
-The loss of a forward run carries ~1e-16 of FP64 solver rounding noise, which is not
-negligible against 2 h |g| for cells with small gradients; where the h = 1e-6 difference misses
-1e-4, a second difference at h = 1e-5 measures that noise.  Gradients below what a central
-difference can resolve at all (4 eps L / h) count as agreeing when the adjoint is below it too.  The perturbed forward runs go
-through independent contexts on several host threads (each context has its own stream), since
-one 64x64x30 run leaves most of the GPU idle.
+The loss of a forward run still carries solver noise (a perturbed run takes another path through
+the Picard and Krylov iterations), up to ~1e-13 of L here, which is not negligible against
+2 h |g| for cells with small gradients.  Where the h = 1e-6 difference misses 1e-4, the check is
+repeated at h = 1e-5, and the noise of the loss is measured by the symmetric second differences
+L(+h) + L(-h) - 2 L(0) at both step sizes (for a smooth loss they are O(h^2 L''), far below the
+first differences of any resolvable gradient).  A cell passes when, at either step size,
+|adjoint - FD| <= 1e-4 |FD| + 2 dL / (2 h), dL = the largest second difference seen for it (at
+least 4 eps L).  The perturbed forward runs go through independent contexts on several host
+threads (each context has its own stream), since one 64x64x30 run leaves most of the GPU idle.
 """
 import argparse
 import os
@@ -75,34 +78,39 @@ class C4Check:
         return loss
 
     def fd(self, jobs, h):
-        """Central differences for jobs = [(key, cell, log)], forward runs spread over the pool."""
+        """Central differences and second differences (loss units) for jobs = [(key, cell, log)],
+        forward runs spread over the pool."""
         futs = [(self.pool.submit(self._loss, k, c, lg, +h), self.pool.submit(self._loss, k, c, lg, -h))
                 for k, c, lg in jobs]
-        return [(a.result() - b.result()) / (2 * h) for a, b in futs]
+        out = []
+        for a, b in futs:
+            lp, lm = a.result(), b.result()
+            out.append(((lp - lm) / (2 * h), abs(lp + lm - 2 * self.loss)))
+        return out
 
-    def check(self, cells):
+    def check(self, cells, h1=1e-6, h2=1e-5):
         jobs = [(k, int(c), lg) for k, _, lg in FIELDS for c in cells]
-        fd1 = self.fd(jobs, 1e-6)
         rows = []
         redo = []
-        for (k, c, lg), f1 in zip(jobs, fd1):
+        for (k, c, lg), (f1, s1) in zip(jobs, self.fd(jobs, h1)):
             g = float(self.grad[k][c])
             rel = abs(g - f1) / abs(f1) if f1 != 0 else float("inf")
-            rows.append(dict(field=k, cell=c, adjoint=g, fd1=f1, rel1=rel, fd2=None, noise=None))
+            rows.append(dict(field=k, cell=c, adjoint=g, fd1=f1, rel1=rel, sd1=s1, fd2=None, sd2=None))
             if not rel <= 1e-4:
                 redo.append(len(rows) - 1)
         if redo:
-            fd2 = self.fd([jobs[i] for i in redo], 1e-5)
-            for i, f2 in zip(redo, fd2):
+            for i, (f2, s2) in zip(redo, self.fd([jobs[i] for i in redo], h2)):
                 rows[i]["fd2"] = f2
-                rows[i]["noise"] = abs(rows[i]["fd1"] - f2)
-        # a central difference cannot resolve less than the rounding of the two losses, eps L / h
-        floor = 4 * np.finfo(float).eps * self.loss
+                rows[i]["sd2"] = s2
+        floor = 4 * np.finfo(float).eps * self.loss   # rounding of the loss itself
         for r in rows:
-            if r["rel1"] <= 1e-4 or abs(r["adjoint"] - r["fd1"]) <= floor / 1e-6:
+            if r["rel1"] <= 1e-4:
                 r["ok"] = True
-            else:
-                r["ok"] = abs(r["adjoint"] - r["fd2"]) <= 1e-4 * abs(r["fd2"]) + 4 * r["noise"] + floor / 1e-5
+                continue
+            dl = max(r["sd1"], r["sd2"] or 0.0, floor)
+            r["noise"] = dl
+            r["ok"] = any(abs(r["adjoint"] - f) <= 1e-4 * abs(f) + 2 * dl / (2 * h)
+                          for f, h in ((r["fd1"], h1), (r["fd2"], h2)) if f is not None)
         return rows
 
 
@@ -136,8 +144,11 @@ def main():
         lines.append(f"| {dict((x, y) for x, y, _ in FIELDS)[r['field']]} | ({i}, {j}, {k}) | {r['adjoint']:.6e} | "
                      f"{r['fd1']:.6e} | {r['rel1']:.1e} | {f2} | {'yes' if r['ok'] else 'NO'} |")
     strict = sum(r["rel1"] <= 1e-4 for r in rows)
+    strict2 = sum(min(r["rel1"], abs(r["adjoint"] - r["fd2"]) / abs(r["fd2"]) if r["fd2"] else float("inf")) <= 1e-4
+                  for r in rows)
     summary = (f"{len(rows)} checks ({a.cells} seeded cells x 3 parameters): {strict} within 1e-4 at h = 1e-6, "
-               f"{sum(r['ok'] for r in rows)} within 1e-4 + the measured rounding noise; "
+               f"{strict2} within 1e-4 at h = 1e-6 or 1e-5, "
+               f"{sum(r['ok'] for r in rows)} within 1e-4 + the measured solver noise of the loss; "
                f"max rel. diff at h = 1e-6 {max(r['rel1'] for r in rows):.1e}; FD time {time.time() - t0:.0f} s")
     text = "\n".join(lines) + "\n\n" + summary + "\n"
     print(text, flush=True)

@@ -173,6 +173,7 @@ def test_dirichlet_faces_and_reference_switches(hip, orc):
 def test_tiled_spmv_matches_untiled(hip, orc, tile):
     # odd sizes: nx not a multiple of any tile width, ny not of any height, kc not dividing nz
     w = inputs.c2_coupled(nx=37, ny=21, nz=30)
+    w.params.krylov = _abi.KRYLOV_TWO_KERNEL   # the SpMV kernel of the two-kernel PCG
     ref = hip.FlowModel(w.grid, cp(w.params))
     til = hip.FlowModel(w.grid, cp(w.params))
     w.apply(ref)
