@@ -445,13 +445,14 @@ int solve_mixed_cg1(dhara_ctx* c, dhara_step_stats* st)
     const int nb = dh::cg1_blocks(c->g, tc);
     const int init_parts = dh::cg1_init_blocks(c->g, c->tile[1]);
     dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, true);
-    int it = 0, updates = 0;
+    int it = 0, updates = 0, refine_parts = dh::grid_for(c->N);
     bool outer_conv = false;
     for (;;) {
         {
             dh::SolveCtl k = sc;   // fresh residual partials in part[0]
             k.it = it;
             k.segment_start = it > 0 ? 1 : 0;
+            k.nparts = it > 0 ? refine_parts : dh::grid_for(c->N);
             k.part_out = c->part[1];
             dh::launch_cg1_init_f32(c->s, c->g, c->tile[1], c->K32, c->D32, rb[it & 1], pb[(it + 1) & 1], k);
         }
@@ -473,7 +474,7 @@ int solve_mixed_cg1(dhara_ctx* c, dhara_step_stats* st)
         if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
         const int it_end = c->h_st->iter;
         ra.r32 = rb[it_end & 1];
-        dh::launch_refine(c->s, ra);
+        refine_parts = dh::launch_refine_tiled(c->s, ra, c->tile[0]);
         LAUNCHCHK(c);
         ++updates;
         if (it_end >= maxit || updates > c->p.refine_maxiter) {
@@ -483,6 +484,7 @@ int solve_mixed_cg1(dhara_ctx* c, dhara_step_stats* st)
             k.maxit = it_end;
             k.segment_start = 1;
             k.first_sweep = 0;
+            k.nparts = refine_parts;
             k.part_out = c->part[1];
             dh::launch_cg1_init_f32(c->s, c->g, c->tile[1], c->K32, c->D32, rb[it_end & 1], pb[(it_end + 1) & 1],
                                     k);

@@ -164,6 +164,10 @@ void launch_surface_pre(hipStream_t s, const SurfArgs& a);
 void launch_surface_post(hipStream_t s, const SurfArgs& a);
 void launch_assemble(hipStream_t s, const AsmArgs& a, bool mixed);
 void launch_refine(hipStream_t s, const RefineArgs& a);
+// the same on an SpMV tile (LDS-tiled FP64 residual; untiled with Dirichlet rows or tx = 0);
+// returns the number of partials written
+struct TileCfg;
+int launch_refine_tiled(hipStream_t s, const RefineArgs& a, const TileCfg& t);
 void launch_end_step(hipStream_t s, const EndArgs& a);
 
 // Jacobi-PCG iteration `sc.it` on the 3D operator, split in its two kernels:

@@ -695,6 +695,80 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
     }
 }
 
+// FP64 residual of a reliable update (after k_refine_add), LDS-tiled like the SpMV: r = b - A x,
+// r32 = (float) r, d32 = 0, partials [rz32, rr32, 0, rr64].  No Dirichlet rows.  The untiled
+// grid-stride form re-reads the k +- 1 and j +- 1 neighbours from HBM (2.4x the algorithmic
+// bytes); here x and K of a plane are staged once per tile and the vertical neighbours stay in
+// registers.  Same operations in the same order as op3_row.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_refine_residual_tiled(RefineArgs a, int kc, int ntx, int nty, int nkz)
+{
+    const Geo g = a.g;
+    if (lin_bid() == 0 && lin_tid() == 0) a.st->done = 0;   // the next inner solve starts
+    __shared__ double sx[2][TY + 2][TX + 2];
+    __shared__ double sk[2][TY + 2][TX + 2];
+    const int nb = ntx * nty * nkz;
+    const int t = xcd_tile(blockIdx.x, nb);
+    const int bz = t % nkz, bxy = t / nkz;
+    const int bx = bxy % ntx, by = bxy / ntx;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * TX + tx, j = by * TY + ty;
+    const int k0 = bz * kc, k1 = min(k0 + kc, g.nz);
+    const bool in = (i < g.nx) && (j < g.ny);
+    const int col = j * g.nx + i;
+    const int lid = ty * TX + tx;
+    int hx = -2, hy = -2;
+    if (lid < TX) { hx = lid; hy = -1; }
+    else if (lid < 2 * TX) { hx = lid - TX; hy = TY; }
+    else if (lid < 2 * TX + TY) { hx = -1; hy = lid - 2 * TX; }
+    else if (lid < 2 * TX + 2 * TY) { hx = TX; hy = lid - 2 * TX - TY; }
+    const int hi = bx * TX + hx, hj = by * TY + hy;
+    const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
+    const int hcol = hj * g.nx + hi;
+    const double* __restrict__ x = a.x;
+    const double* __restrict__ K = a.K;
+    const double wx = g.wx, wy = g.wy, wz = g.wz, half = 0.5;
+
+    double xm = 0.0, km = 0.0, xc = 0.0, kcur = 0.0;
+    if (in) {
+        if (k0 > 0) { xm = x[(k0 - 1) * g.nxy + col]; km = K[(k0 - 1) * g.nxy + col]; }
+        if (k0 < k1) { xc = x[k0 * g.nxy + col]; kcur = K[k0 * g.nxy + col]; }
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = k0; k < k1; ++k) {
+        double xn = 0.0, kn = 0.0;
+        if (in && k + 1 < g.nz) { xn = x[(k + 1) * g.nxy + col]; kn = K[(k + 1) * g.nxy + col]; }
+        const int buf = k & 1;
+        sx[buf][ty + 1][tx + 1] = xc;
+        sk[buf][ty + 1][tx + 1] = kcur;
+        if (has_halo) {
+            sx[buf][hy + 1][hx + 1] = x[k * g.nxy + hcol];
+            sk[buf][hy + 1][hx + 1] = K[k * g.nxy + hcol];
+        }
+        __syncthreads();
+        if (in) {
+            const int c = k * g.nxy + col;
+            double ax = a.D[c] * xc;
+            if (k > 0) ax -= wz * (half * (kcur + km)) * xm;
+            if (j > 0) ax -= wy * (half * (kcur + sk[buf][ty][tx + 1])) * sx[buf][ty][tx + 1];
+            if (i > 0) ax -= wx * (half * (kcur + sk[buf][ty + 1][tx])) * sx[buf][ty + 1][tx];
+            if (i < g.nx - 1) ax -= wx * (half * (kcur + sk[buf][ty + 1][tx + 2])) * sx[buf][ty + 1][tx + 2];
+            if (j < g.ny - 1) ax -= wy * (half * (kcur + sk[buf][ty + 2][tx + 1])) * sx[buf][ty + 2][tx + 1];
+            if (k < g.nz - 1) ax -= wz * (half * (kcur + kn)) * xn;
+            const double r = a.b[c] - ax;
+            const float r32 = (float)r;
+            a.r32[c] = r32;
+            a.d32[c] = 0.0f;
+            const float z32 = r32 / a.D32[c];
+            acc[3] += r * r;
+            acc[0] += (double)r32 * (double)z32;
+            acc[1] += (double)r32 * (double)r32;
+        }
+        xm = xc; km = kcur; xc = xn; kcur = kn;
+    }
+    block_partials<4, false>(acc, a.part_out);
+}
+
 template <typename T, bool DIR>
 __global__ __launch_bounds__(kBlock) void k_apply3(Geo g, const uint8_t* flags, const T* K,
                                                    const T* D, const T* x, T* y)
@@ -1471,6 +1545,29 @@ void launch_refine(hipStream_t s, const RefineArgs& a)
     hipLaunchKernelGGL(k_refine_add, gr, bl, 0, s, a.g.n, a.x, (const float*)a.d32);
     if (a.flags) hipLaunchKernelGGL((k_refine_residual<true>), gr, bl, 0, s, a);
     else hipLaunchKernelGGL((k_refine_residual<false>), gr, bl, 0, s, a);
+}
+
+template <int TX, int TY>
+static void refine_tiled(hipStream_t s, const RefineArgs& a, int kc)
+{
+    const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY, nkz = (a.g.nz + kc - 1) / kc;
+    hipLaunchKernelGGL((k_refine_residual_tiled<TX, TY>), dim3(ntx * nty * nkz), dim3(TX, TY), 0, s, a, kc, ntx,
+                       nty, nkz);
+}
+
+int launch_refine_tiled(hipStream_t s, const RefineArgs& a, const TileCfg& t)
+{
+    if (a.flags || !t.tx) {
+        launch_refine(s, a);
+        return grid_for(a.g.n);
+    }
+    hipLaunchKernelGGL(k_refine_add, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a.g.n, a.x, (const float*)a.d32);
+    if (t.tx == 32 && t.ty == 8) refine_tiled<32, 8>(s, a, t.kc);
+    else if (t.tx == 64 && t.ty == 4) refine_tiled<64, 4>(s, a, t.kc);
+    else if (t.tx == 16 && t.ty == 16) refine_tiled<16, 16>(s, a, t.kc);
+    else if (t.tx == 32 && t.ty == 4) refine_tiled<32, 4>(s, a, t.kc);
+    else refine_tiled<64, 8>(s, a, t.kc);
+    return tile_blocks(a.g, t);
 }
 
 void launch_end_step(hipStream_t s, const EndArgs& a)
