@@ -403,7 +403,10 @@ __device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta, int 
         const double bb = sc.first_sweep ? in[2] : st->bb;
         if (sc.mixed_inner) {
             outer = (in[3] <= sc.lin_tol * sc.lin_tol * bb) ? 1 : 0;
-            tol2 = sc.inner_tol * sc.inner_tol * rr;
+            // reduce by inner_tol, but not below half the outer target: the FP64 residual
+            // after the next reliable update then meets the outer test without a further
+            // FP32 segment that over-solves
+            tol2 = fmax(sc.inner_tol * sc.inner_tol * rr, 0.25 * sc.lin_tol * sc.lin_tol * bb);
         } else {
             tol2 = sc.lin_tol * sc.lin_tol * bb;
             outer = (rr <= tol2) ? 1 : 0;
