@@ -108,7 +108,7 @@ def default_params(**overrides):
         dt=60.0, theta_s=0.43, theta_r=0.078, alpha=3.6, n_vg=1.56, Ss=1e-5, porosity=0.43,
         picard_maxiter=20, picard_rule=PICARD_ABS_PSI, picard_tol=1e-6, am=0.0,
         lin_maxiter=2000, precision=FP64, lin_tol=1e-12,
-        refine_maxiter=16, inner_maxiter=2000, inner_tol=1e-3,
+        refine_maxiter=16, inner_maxiter=2000, inner_tol=1e-5,
         overland=1, olf_maxiter=2000, olf_tol=1e-12,
         hmin=1e-5, delta=1e-7, K0=1e-5,
         theta_carry=THETA_CARRY_CONSISTENT, dirichlet_face_quirk=1,

@@ -26,7 +26,7 @@ constexpr double kMgOver = 1.5;   // coarse-correction scaling
 constexpr int kMgT = 28;          // tile edge of the large-level kernels (even)
 constexpr int kMgHd = 5;          // halo: 4 half-sweeps + the residual
 constexpr int kMgHu = 4;          // halo: 4 half-sweeps
-constexpr int kMgThreads = 256;
+constexpr int kMgThreads = 1024;   // one pass of a 38 x 38 tile's half-sweep per block
 constexpr int kMgCoarseCells = 1366;   // cells of all levels from 32 x 32 down to 1 x 1
 
 namespace {
@@ -77,13 +77,18 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel F, MgLevel C)
 
 // One red-black Gauss-Seidel half-sweep over an LDS grid of w x w cells (global origin ox, oy),
 // updating the cells of `color` ((gi + gj) & 1) that are not on the LDS border.
+// Threads are dealt only the cells of the colour: row lj holds the interior cells li = p, p+2, ...
+// with p fixed by the colour, so a half-sweep of a 38 x 38 tile is one pass of 648 threads.
 __device__ __forceinline__ void lds_half_sweep(int w, int ox, int oy, int color, const double* sb, const double* sd,
                                                const double* swe, const double* swn, double* sx)
 {
-    for (int t = threadIdx.x; t < w * w; t += blockDim.x) {
-        const int lj = t / w, li = t - lj * w;
-        if (li == 0 || lj == 0 || li == w - 1 || lj == w - 1) continue;
-        if (((ox + li + oy + lj) & 1) != color) continue;
+    const int hw = (w - 1) / 2;   // colour cells per interior row, at most
+    for (int u = threadIdx.x; u < (w - 2) * hw; u += blockDim.x) {
+        const int lj = 1 + u / hw;
+        const int p = ((color - ox - oy - lj) & 1) ? 1 : 2;   // first interior li of the colour
+        const int li = p + 2 * (u - (lj - 1) * hw);
+        if (li > w - 2) continue;
+        const int t = lj * w + li;
         const double s = sb[t] + swe[t] * sx[t + 1] + swe[t - 1] * sx[t - 1] + swn[t] * sx[t + w] + swn[t - w] * sx[t - w];
         sx[t] = s / sd[t];
     }
@@ -208,11 +213,14 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
         }
     }
     __syncthreads();
-    auto half = [&](int l, int color) {
+    auto half = [&](int l, int color) {   // cells of the colour only: row j holds i = (color - j) & 1, +2, ...
         const int nx = h.lv[l].nx, ny = h.lv[l].ny, o = off[l];
-        for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
-            const int j = t / nx, i = t - j * nx;
-            if (((i + j) & 1) != color) continue;
+        const int hw = (nx + 1) / 2;
+        for (int u = threadIdx.x; u < ny * hw; u += blockDim.x) {
+            const int j = u / hw;
+            const int i = ((color - j) & 1) + 2 * (u - j * hw);
+            if (i >= nx) continue;
+            const int t = j * nx + i;
             double s = sb[o + t];
             if (i + 1 < nx) s += swe[o + t] * sx[o + t + 1];
             if (i > 0) s += swe[o + t - 1] * sx[o + t - 1];

@@ -202,7 +202,7 @@ static inline void dhara_params_defaults(dhara_params* p)
     p->picard_maxiter = 20; p->picard_rule = DHARA_PICARD_ABS_PSI;
     p->picard_tol = 1e-6; p->am = 0.0;
     p->lin_maxiter = 2000; p->precision = DHARA_FP64; p->lin_tol = 1e-12;
-    p->refine_maxiter = 16; p->inner_maxiter = 2000; p->inner_tol = 1e-3;
+    p->refine_maxiter = 16; p->inner_maxiter = 2000; p->inner_tol = 1e-5;
     p->overland = 1; p->olf_maxiter = 2000; p->olf_tol = 1e-12;
     p->hmin = 1e-5; p->delta = 1e-7; p->K0 = 1e-5;
     p->theta_carry = DHARA_THETA_CARRY_CONSISTENT;
