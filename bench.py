@@ -204,6 +204,8 @@ def main():
         tune_us = m.autotune()
         m.close()
         m = dhara_amd.FlowModel(w.grid, cp(w.params))
+        m_tile_fp64 = m.get_krylov_tile(8) if one_kernel else m.get_tile(8)
+        m_tile_fp32 = m.get_krylov_tile(4) if one_kernel else m.get_tile(4)
         if one_kernel:
             tiles[name] = {"fp64": m.get_krylov_tile(8), "fp32": m.get_krylov_tile(4), "tune_us": tune_us,
                            "kernel": "k_cg1 (tx, ty, planes per tile, grid cap)"}
@@ -221,7 +223,8 @@ def main():
         psi = m.download("PSI")
         h = m.download("PONDING")
         m.close()
-        runs[name] = dict(stats=stats, secs=secs, snaps=snaps, prof=prof, psi=psi, h=h)
+        runs[name] = dict(stats=stats, secs=secs, snaps=snaps, prof=prof, psi=psi, h=h,
+                          tile=(m_tile_fp32 if name == "mixed" else m_tile_fp64))
         print(f"# {name}: {secs:.3f} s, {n3 * K / secs:.4e} cell-steps/s, picard "
               f"{sum(s['picard_iters'] for s in stats)}, krylov {sum(s['lin_iters'] for s in stats)}",
               file=sys.stderr, flush=True)
@@ -274,14 +277,20 @@ def main():
         kern.sort(reverse=True)
         tot, kname, avg, byts, nl = kern[0]
         prec = "f32" if pr["elem_bytes"] == 4 else "f64"
+        # measured HBM bytes per launch (rocprof PMC, copy-calibrated) for the tile in use, if the
+        # profile holds that tile
         traffic = None
         tpath = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tpath):
             try:
-                traffic = json.load(open(tpath)).get(f"k_{kname}<{prec}>", None)   # HBM bytes per launch
+                entry = json.load(open(tpath)).get(f"k_{kname}<{prec}>")
+                tile_now = list(r["tile"]) if r.get("tile") is not None else None
+                if isinstance(entry, dict) and entry.get("tile") == tile_now:
+                    traffic = entry["bytes_per_launch"]
             except Exception:
                 traffic = None
-        out["roofline"] = {"bound": "hbm", "kernel": f"k_{kname}<{prec}>", "achieved": byts / avg / 1e9,
+        out["roofline"] = {"bound": "hbm", "kernel": f"k_{kname}<{prec}>", "tile": r.get("tile"),
+                           "achieved": byts / avg / 1e9,
                            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": byts / avg / HBM_PEAK,
                            "traffic": traffic, "avg_us": avg * 1e6, "launches": nl,
                            "bytes_per_launch": byts}

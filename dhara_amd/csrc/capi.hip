@@ -1266,7 +1266,9 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
                                                    (float*)ps, (float*)xs, k);
                             }, &us);
                         if (rc2) break;
-                        if (us < best_t) { best_t = us; best_cfg = t; }
+                        // a later candidate must be 3 % faster: ties keep the first (whole columns, one
+                        // block per tile), so the choice is stable from run to run
+                        if (us < 0.97f * best_t || best_t >= 1e29f) { best_t = us; best_cfg = t; }
                     }
             c->cg1_tile[e] = best_cfg;
             best[e] = best_t;
