@@ -68,6 +68,13 @@ struct dhara_ensemble {
     std::vector<dh::KrylovArgs<float>> h_k3f;
     hipGraphExec_t graph3 = nullptr, graph2 = nullptr;
     int batch = 8;
+    // one-kernel PCG of the subsurface solves (every member with DHARA_KRYLOV_ONE_KERNEL)
+    bool one = false;
+    dh::BlkMap* map_c1 = nullptr;   // one block per 32 x 8 tile of every member
+    int nb_c1 = 0;
+    void* a_c1 = nullptr;
+    std::vector<dh::Cg1Args<double>> h_c1d;
+    std::vector<dh::Cg1Args<float>> h_c1f;
 };
 
 namespace {
@@ -133,6 +140,8 @@ void krylov_batch(dhara_ensemble* e, bool two_d)
         if (two_d) {
             dh::launch_batched_spmv2(e->s, e->a_k2, e->map2, e->nb2, e->active, e->it_base, k);
             dh::launch_batched_update2(e->s, e->a_k2, e->map2, e->nb2, e->active, e->it_base, k);
+        } else if (e->one) {
+            dh::launch_batched_cg1(e->s, e->mixed, e->a_c1, e->map_c1, e->nb_c1, e->active, e->it_base, k);
         } else {
             dh::launch_batched_spmv3(e->s, e->mixed, e->a_k3, e->map3, e->nb3, e->active, e->it_base, k);
             dh::launch_batched_update3(e->s, e->mixed, e->a_k3, e->map3, e->nb3, e->active, e->it_base, k);
@@ -159,6 +168,10 @@ int ensure_graph(dhara_ensemble* e, bool two_d)
 int krylov_solve(dhara_ensemble* e, bool two_d, const std::vector<int>& on, int maxit)
 {
     EHIP(e, hipMemsetAsync(e->it_base, 0, sizeof(int), e->s));   // outside any captured graph
+    if (!two_d && e->one) {
+        dh::launch_batched_cg1_init(e->s, e->mixed, e->a_c1, e->map3, e->nb3, e->active);
+        EHIP(e, hipGetLastError());
+    }
     if (e->use_graph) {
         int rc = ensure_graph(e, two_d);
         if (rc) return rc;
@@ -226,6 +239,64 @@ int upload_k3(dhara_ensemble* e)
                     : upload(e, (dh::KrylovArgs<double>*)e->a_k3, e->h_k3d);
 }
 
+// Tiles of a member in the batched one-kernel PCG: 32 x 8 columns; the member's own (tuned)
+// planes per tile when its tile is 32 x 8, whole columns otherwise.
+void c1_tiles(const dhara_ensemble* e, const dhara_ctx* c, int& kc, int& ntx, int& nty, int& nkz)
+{
+    const dh::TileCfg& t = c->cg1_tile[e->mixed ? 1 : 0];
+    kc = (t.tx == dh::kCg1BatchTx && t.ty == dh::kCg1BatchTy && t.kc > 0) ? std::min(t.kc, c->g.nz) : c->g.nz;
+    ntx = (c->g.nx + dh::kCg1BatchTx - 1) / dh::kCg1BatchTx;
+    nty = (c->g.ny + dh::kCg1BatchTy - 1) / dh::kCg1BatchTy;
+    nkz = (c->g.nz + kc - 1) / kc;
+}
+
+template <typename T>
+void fill_c1(dhara_ensemble* e, dhara_ctx* c, int m, bool first_sweep, const T* K, const T* D, T* r0, T* r1, T* p0,
+             T* p1, T* x, dh::Cg1Args<T>& a)
+{
+    a = dh::Cg1Args<T>{};
+    a.g = c->g;
+    a.K = K; a.D = D;
+    a.r[0] = r0; a.r[1] = r1; a.p[0] = p0; a.p[1] = p1; a.x = x;
+    a.part_asm = c->part[0];
+    a.part[0] = c->part[0]; a.part[1] = c->part[1];
+    a.nparts_asm = dh::grid_for(c->N);
+    c1_tiles(e, c, a.kc, a.ntx, a.nty, a.nkz);
+    a.ntiles = a.ntx * a.nty * a.nkz;
+    const int maxit = e->mixed ? c->p.inner_maxiter : c->p.lin_maxiter;
+    a.ctl = ctl(c, &e->st[m], maxit, c->p.lin_tol, e->mixed, first_sweep, nullptr, 0, nullptr);
+}
+
+// Arguments of the one-kernel PCG of every member (first_sweep: ||b||^2 from the assembly).
+void build_c1(dhara_ensemble* e, bool first_sweep)
+{
+    for (int m = 0; m < e->M; ++m) {
+        dhara_ctx* c = e->mem[m];
+        if (e->mixed)
+            fill_c1<float>(e, c, m, first_sweep, c->K32, c->D32, c->r32, c->q32, c->p32[0], c->p32[1], c->d32,
+                           e->h_c1f[m]);
+        else
+            fill_c1<double>(e, c, m, first_sweep, c->K, c->D, c->r, c->q, c->pv[0], c->pv[1], c->x, e->h_c1d[m]);
+    }
+}
+
+int upload_c1(dhara_ensemble* e)
+{
+    return e->mixed ? upload(e, (dh::Cg1Args<float>*)e->a_c1, e->h_c1f)
+                    : upload(e, (dh::Cg1Args<double>*)e->a_c1, e->h_c1d);
+}
+
+// Kernel arguments of the subsurface Krylov solve of every member.
+int build_krylov(dhara_ensemble* e, bool first_sweep)
+{
+    if (e->one) {
+        build_c1(e, first_sweep);
+        return upload_c1(e);
+    }
+    build_k3(e, first_sweep);
+    return upload_k3(e);
+}
+
 // Subsurface step of every member (do_subsurface_step in lockstep).
 int subsurface_step(dhara_ensemble* e, std::vector<dhara_step_stats>& st)
 {
@@ -261,8 +332,7 @@ int subsurface_step(dhara_ensemble* e, std::vector<dhara_step_stats>& st)
         }
         if ((rc = upload(e, e->a_asm, e->h_asm)) || (rc = set_active(e, pic))) return rc;
         dh::launch_batched(e->s, e->mixed ? dh::BK_ASM_MIXED : dh::BK_ASM_F64, e->a_asm, e->map3, e->nb3, e->active);
-        build_k3(e, true);
-        if ((rc = upload_k3(e))) return rc;
+        if ((rc = build_krylov(e, true))) return rc;
         std::vector<int> solving = pic;
         int round = 0;
         for (;;) {
@@ -303,10 +373,7 @@ int subsurface_step(dhara_ensemble* e, std::vector<dhara_step_stats>& st)
                     if (refine[m]) st[m].lin_converged = 0;
                 break;
             }
-            if (round == 1) {
-                build_k3(e, false);   // later rounds keep the first round's ||b||
-                if ((rc = upload_k3(e))) return rc;
-            }
+            if (round == 1 && (rc = build_krylov(e, false))) return rc;   // later rounds keep the first round's ||b||
             solving = refine;
         }
         // theta, K, C of the new iterate, Picard increment (k_vg_picard) for every member
@@ -473,6 +540,26 @@ int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble**
     }
     e->nb3 = (int)m3.size();
     e->nb2 = (int)m2.size();
+    // one-kernel PCG when every member asks for it (32-bit plane offsets: N * 8 < 4 GiB)
+    e->one = true;
+    for (dhara_ctx* c : e->mem)
+        if (c->p.krylov != DHARA_KRYLOV_ONE_KERNEL || (size_t)c->N * 8 >= (size_t(1) << 32)) e->one = false;
+    std::vector<dh::BlkMap> mc1;
+    if (e->one) {
+        e->h_c1d.resize(n);
+        e->h_c1f.resize(n);
+        for (int m = 0; m < n; ++m) {
+            int kc, ntx, nty, nkz;
+            c1_tiles(e, e->mem[m], kc, ntx, nty, nkz);
+            const int nt = ntx * nty * nkz;
+            if (nt > dh::kMaxBlocks) {
+                dhara_ensemble_destroy(e);
+                return DHARA_E_UNSUPPORTED;
+            }
+            for (int b = 0; b < nt; ++b) mc1.push_back(dh::BlkMap{m, b, nt});
+        }
+        e->nb_c1 = (int)mc1.size();
+    }
     int rc = 0;
     if ((rc = ealloc(e, &e->map3, m3.size())) || (rc = ealloc(e, &e->map2, m2.size())) ||
         (rc = ealloc(e, &e->active, n)) || (rc = ealloc(e, &e->it_base, 1)) || (rc = ealloc(e, &e->st, 2 * n)) ||
@@ -485,6 +572,12 @@ int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble**
     }
     if (e->mixed) rc = ealloc(e, (dh::KrylovArgs<float>**)&e->a_k3, n);
     else rc = ealloc(e, (dh::KrylovArgs<double>**)&e->a_k3, n);
+    if (!rc && e->one) {
+        if (e->mixed) rc = ealloc(e, (dh::Cg1Args<float>**)&e->a_c1, n);
+        else rc = ealloc(e, (dh::Cg1Args<double>**)&e->a_c1, n);
+        if (!rc) rc = ealloc(e, &e->map_c1, mc1.size());
+        if (!rc) rc = upload(e, e->map_c1, mc1);
+    }
     if (rc) { dhara_ensemble_destroy(e); return rc; }
     if (hipHostMalloc((void**)&e->h_active, n * sizeof(int)) != hipSuccess ||
         hipHostMalloc((void**)&e->h_st, 2 * n * sizeof(dh::PcgState)) != hipSuccess ||

@@ -251,6 +251,28 @@ struct KrylovArgs {
     const double *ke, *kn;       // overland operator (2D solves)
     SolveCtl spmv, upd;          // part_in/part_out/st of the two kernels; `it` is ignored
 };
+// One-kernel PCG of one model in a batched launch: tiles of kCg1BatchTx x kCg1BatchTy columns
+// and kc planes, one block per tile.  Iteration it reads r[it & 1], p[(it + 1) & 1] and the
+// partials part[(it + 1) & 1], writes r[(it + 1) & 1], p[it & 1], x and part[it & 1]; the
+// start-up reads the assembly / reliable-update partials part_asm and writes part[1].  Every
+// solve (and every refinement round) restarts at it = 0.
+template <typename T>
+struct Cg1Args {
+    Geo g;
+    const T *K, *D;
+    T *r[2], *p[2], *x;
+    double *part_asm, *part[2];
+    int nparts_asm, ntiles;
+    int kc, ntx, nty, nkz;
+    SolveCtl ctl;                // maxit, tolerances, mixed_inner, first_sweep, st
+};
+constexpr int kCg1BatchTx = 32, kCg1BatchTy = 8;
+// start-up on the map of the grid-stride kernels (grid_for(N) blocks per model)
+void launch_batched_cg1_init(hipStream_t s, bool fp32, const void* args, const BlkMap* map, int nblocks,
+                             const int* active);
+// iteration *it_base + k on the tile map
+void launch_batched_cg1(hipStream_t s, bool fp32, const void* args, const BlkMap* map, int nblocks,
+                        const int* active, const int* it_base, int k);
 enum BatchKind { BK_VG = 0, BK_VG_PICARD, BK_SURF_PRE, BK_SURF_POST, BK_ASM_F64, BK_ASM_MIXED,
                  BK_REFINE_ADD, BK_REFINE_RES, BK_END, BK_OLF_H, BK_OLF_FACES, BK_OLF_ASM, BK_OLF_POST };
 // args points to an array of the argument struct of the kind (VgArgs, SurfArgs, AsmArgs,

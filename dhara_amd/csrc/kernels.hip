@@ -1199,16 +1199,15 @@ __device__ __forceinline__ bool cg1_begin(const SolveCtl& sc, double& alpha, dou
 // the stopping test of pcg_begin on the four partials [rz, rr, bb, rr64], then
 // [gamma, rho, delta, eps, 0] of u = M^-1 r (eps against the kept direction p_{i-1}).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg1_init(Geo g, const T* __restrict__ K, const T* __restrict__ D,
-                                                     const T* __restrict__ r, const T* __restrict__ p_old,
-                                                     SolveCtl sc)
+__device__ __forceinline__ void b_cg1_init(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                           const T* __restrict__ r, const T* __restrict__ p_old, SolveCtl sc, Blk bk)
 {
     double unused;
-    if (!pcg_begin(sc, unused, lin_bid())) return;
+    if (!pcg_begin(sc, unused, bk.id)) return;
     const bool kept = sc.it > 0;
     auto uf = [&](int c) -> T { return r[c] * jac_inv(D[c]); };
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x) {
+    for (int c = bk.id * blockDim.x + threadIdx.x; c < g.n; c += bk.n * blockDim.x) {
         int i, j, k;
         decompose(g, c, i, j, k);
         const T rc = r[c];
@@ -1219,7 +1218,15 @@ __global__ __launch_bounds__(kBlock) void k_cg1_init(Geo g, const T* __restrict_
         acc[2] += (double)uc * (double)au;
         if (kept) acc[3] += (double)p_old[c] * (double)au;
     }
-    block_partials<5, false>(acc, sc.part_out, lin_bid());
+    block_partials<5, false>(acc, sc.part_out, bk.id);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_init(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                                     const T* __restrict__ r, const T* __restrict__ p_old,
+                                                     SolveCtl sc)
+{
+    b_cg1_init<T>(g, K, D, r, p_old, sc, hw_blk());
 }
 
 // Local position (li, lj) of halo cell h of a TX x TY tile: first the one-cell ring (top row,
@@ -1263,10 +1270,10 @@ struct Cg1Plane {
 // a direction costs four operations.  The plane loop is unrolled six times, so the LDS ring
 // slots are compile-time constants and the register pipeline shifts are renames.
 template <typename T, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K, const T* __restrict__ D,
-                                                 const T* __restrict__ r_in, T* __restrict__ r_out,
-                                                 const T* __restrict__ p_in, T* __restrict__ p_out,
-                                                 T* __restrict__ x, SolveCtl sc, int kc, int ntx, int nty, int nkz)
+__device__ __forceinline__ void b_cg1(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                      const T* __restrict__ r_in, T* __restrict__ r_out,
+                                      const T* __restrict__ p_in, T* __restrict__ p_out, T* __restrict__ x,
+                                      SolveCtl sc, int kc, int ntx, int nty, int nkz, Blk bk)
 {
     constexpr int NT = TX * TY;
     constexpr int W2 = TX + 4, A2 = W2 * (TY + 4);
@@ -1274,7 +1281,7 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
     constexpr int NH1 = A1 - NT, NH2 = A2 - NT;
     static_assert(NH1 <= NT && NH2 <= 2 * NT, "tile too small for its halo");
     double alpha_d, beta_d;
-    if (!cg1_begin(sc, alpha_d, beta_d, blockIdx.x)) return;
+    if (!cg1_begin(sc, alpha_d, beta_d, bk.id)) return;
     const bool first = sc.it == 0;
     const T alpha = (T)alpha_d, beta = (T)beta_d;
     const T hx = (T)(0.5 * g.wx), hy = (T)(0.5 * g.wy), hz = (T)(0.5 * g.wz);
@@ -1302,7 +1309,7 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
 
     const int nb = ntx * nty * nkz;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // gamma, rho, delta, eps, pi
-    for (int tw = blockIdx.x; tw < nb; tw += gridDim.x) {
+    for (int tw = bk.id; tw < nb; tw += bk.n) {
         const int t = xcd_tile(tw, nb);
         const int bz = t % nkz, bxy = t / nkz;
         const int i0 = (bxy % ntx) * TX, j0 = (bxy / ntx) * TY;
@@ -1484,7 +1491,51 @@ __global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K,
         }
         __syncthreads();   // the LDS rings are reused by the next tile
     }
-    block_partials<5, false>(acc, sc.part_out, blockIdx.x);
+    block_partials<5, false>(acc, sc.part_out, bk.id);
+}
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_cg1(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                                 const T* __restrict__ r_in, T* __restrict__ r_out,
+                                                 const T* __restrict__ p_in, T* __restrict__ p_out,
+                                                 T* __restrict__ x, SolveCtl sc, int kc, int ntx, int nty, int nkz)
+{
+    b_cg1<T, TX, TY>(g, K, D, r_in, r_out, p_in, p_out, x, sc, kc, ntx, nty, nkz, Blk{(int)blockIdx.x, (int)gridDim.x});
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kb_cg1_init(const Cg1Args<T>* __restrict__ args,
+                                                      const BlkMap* __restrict__ map, const int* __restrict__ active)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const Cg1Args<T>& a = args[e.m];
+    SolveCtl sc = a.ctl;
+    sc.it = 0;
+    sc.segment_start = 0;
+    sc.part_in = a.part_asm;
+    sc.nparts = a.nparts_asm;
+    sc.part_out = a.part[1];
+    b_cg1_init<T>(a.g, a.K, a.D, a.r[0], a.p[1], sc, Blk{e.id, e.n});
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kb_cg1(const Cg1Args<T>* __restrict__ args, const BlkMap* __restrict__ map,
+                                                 const int* __restrict__ active, const int* __restrict__ it_base, int k)
+{
+    static_assert(kCg1BatchTx * kCg1BatchTy == kBlock, "batched tile = one block of kBlock threads");
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const Cg1Args<T>& a = args[e.m];
+    SolveCtl sc = a.ctl;
+    const int it = *it_base + k;
+    sc.it = it;
+    sc.after_init = it == 0 ? 1 : 0;
+    sc.part_in = a.part[(it + 1) & 1];
+    sc.nparts = it == 0 ? a.nparts_asm : a.ntiles;
+    sc.part_out = a.part[it & 1];
+    b_cg1<T, kCg1BatchTx, kCg1BatchTy>(a.g, a.K, a.D, a.r[it & 1], a.r[(it + 1) & 1], a.p[(it + 1) & 1], a.p[it & 1],
+                                       a.x, sc, a.kc, a.ntx, a.nty, a.nkz, Blk{e.id, e.n});
 }
 
 }  // namespace
@@ -1818,6 +1869,28 @@ void launch_batched_update2(hipStream_t s, const void* kargs, const BlkMap* map,
 {
     hipLaunchKernelGGL(kb_update2, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<double>*)kargs, map,
                        active, it_base, k);
+}
+
+void launch_batched_cg1_init(hipStream_t s, bool fp32, const void* args, const BlkMap* map, int nblocks,
+                             const int* active)
+{
+    if (fp32)
+        hipLaunchKernelGGL(kb_cg1_init<float>, dim3(nblocks), dim3(kBlock), 0, s, (const Cg1Args<float>*)args, map,
+                           active);
+    else
+        hipLaunchKernelGGL(kb_cg1_init<double>, dim3(nblocks), dim3(kBlock), 0, s, (const Cg1Args<double>*)args, map,
+                           active);
+}
+
+void launch_batched_cg1(hipStream_t s, bool fp32, const void* args, const BlkMap* map, int nblocks,
+                        const int* active, const int* it_base, int k)
+{
+    if (fp32)
+        hipLaunchKernelGGL(kb_cg1<float>, dim3(nblocks), dim3(kBlock), 0, s, (const Cg1Args<float>*)args, map, active,
+                           it_base, k);
+    else
+        hipLaunchKernelGGL(kb_cg1<double>, dim3(nblocks), dim3(kBlock), 0, s, (const Cg1Args<double>*)args, map,
+                           active, it_base, k);
 }
 
 void launch_advance(hipStream_t s, int* it_base, int by)

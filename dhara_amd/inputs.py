@@ -194,18 +194,18 @@ def c5_instances(count=256, seed=100):
                  rain_mm_h=float(rain[i]), seed=1000 + i) for i in range(count)]
 
 
-def c5_member(spec, dt=30.0, precision=_abi.MIXED):
+def c5_member(spec, dt=30.0, precision=_abi.MIXED, krylov=_abi.KRYLOV_ONE_KERNEL):
     """One ensemble member.  As in the adjoint workload (per-cell random n < 2), the column
     starts unsaturated (hydrostatic, water table 2.5 m down) and the Picard iterate is relaxed
     (am = 0.7) so that the modified Picard iteration converges.  The batched ensemble runs the
-    two-kernel PCG and the Jacobi-preconditioned overland CG, so a member stepped on its own
-    uses the same forms; its FP32 solves restart at every refinement, for which an FP32
-    reduction of 1e-3 per round suits better than the single-model default."""
+    Jacobi-preconditioned overland CG, so a member stepped on its own uses it too; the ensemble's
+    FP32 solves restart at every refinement, for which an FP32 reduction of 1e-3 per round suits
+    better than the single-model default."""
     nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
     g = _abi.Grid(nx=nx, ny=ny, nz=nz, nroot_groups=1, dx=1.0, dy=1.0, dz=0.05)
     p = _abi.default_params(dt=dt, overland=1, top_bc=_abi.TOP_SURFACE, precision=precision,
                             am=0.7, picard_maxiter=100, olf_precond=_abi.OLF_PRECOND_JACOBI,
-                            krylov=_abi.KRYLOV_TWO_KERNEL, inner_tol=1e-3, **LOAM)
+                            krylov=krylov, inner_tol=1e-3, **LOAM)
     seed = spec["seed"]
     n3 = nx * ny * nz
     lnk = np.log(KS_LOAM) + grf((ny, nx), (1.0, 1.0), spec["sigma_lnk"], 10.0, seed)

@@ -1,8 +1,9 @@
 """Batched ensemble on the GPU.
 
 * FP64: the batched launches run the same kernel bodies with the same per-member block counts
-  as a single model with the untiled SpMV and the Jacobi-preconditioned overland solve, so
-  every member's state equals the member stepped alone, bit for bit.
+  as a single model -- the one-kernel PCG on 32 x 8 whole-column tiles with the untiled
+  start-up (or the two-kernel PCG with the untiled SpMV) and the Jacobi-preconditioned
+  overland solve -- so every member's state equals the member stepped alone, bit for bit.
 * The Krylov batches replayed from the captured hipGraph give the same states as the same
   batches launched one by one, replay after replay with host work in between.
 * Mixed precision: every member within 1e-6 m of the FP64 run of the same member.
@@ -21,8 +22,8 @@ SPECS = [dict(nx=33, ny=20, nz=10, sigma_lnk=0.7, rain_mm_h=30.0, seed=2001),
          dict(nx=9, ny=12, nz=20, sigma_lnk=0.5, rain_mm_h=50.0, seed=2004)]
 
 
-def member(spec, precision, untiled=True):
-    w = inputs.c5_member(spec, precision=precision)
+def member(spec, precision, untiled=True, krylov=_abi.KRYLOV_ONE_KERNEL):
+    w = inputs.c5_member(spec, precision=precision, krylov=krylov)
     m = dhara_amd.FlowModel(w.grid, _abi.Params.from_buffer_copy(w.params))
     if untiled:
         m.set_tile(8, 0)
@@ -31,13 +32,14 @@ def member(spec, precision, untiled=True):
     return m
 
 
-def test_fp64_ensemble_equals_members_alone():
+@pytest.mark.parametrize("krylov", [_abi.KRYLOV_ONE_KERNEL, _abi.KRYLOV_TWO_KERNEL])
+def test_fp64_ensemble_equals_members_alone(krylov):
     steps = 4
-    ens_members = [member(s, _abi.FP64) for s in SPECS]
+    ens_members = [member(s, _abi.FP64, krylov=krylov) for s in SPECS]
     ens = dhara_amd.Ensemble(ens_members)
     est = ens.step(steps)
     for i, spec in enumerate(SPECS):
-        alone = member(spec, _abi.FP64)
+        alone = member(spec, _abi.FP64, krylov=krylov)
         st = alone.step(steps)
         np.testing.assert_array_equal(ens_members[i].download("PSI"), alone.download("PSI"))
         np.testing.assert_array_equal(ens_members[i].download("PONDING"), alone.download("PONDING"))
