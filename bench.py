@@ -353,14 +353,17 @@ def bench_ensemble(args, dev):
     for name, prec in (("mixed", _abi.MIXED), ("fp64", _abi.FP64)):
         ms = build(prec)
         ens = dhara_amd.Ensemble(ms)
-        print(f"# ensemble {name}: {len(ms)} members, {ncell:.4e} cells, {K} steps", file=sys.stderr, flush=True)
+        tune_us, div = ens.autotune()   # planes per tile of the batched one-kernel PCG (untimed)
+        floor_us = ncell * 8 * (4 if prec == _abi.MIXED else 8) / 8e12 * 1e6   # 8 arrays at 8 TB/s
+        print(f"# ensemble {name}: {len(ms)} members, {ncell:.4e} cells, {K} steps; tiles 32x8 x ceil(nz/{div}) "
+              f"({tune_us:.0f} us per batched iteration, HBM floor {floor_us:.0f} us)", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         st = []
         for k in range(0, K, 20):
             st += ens.step(min(20, K - k))
             print(f"#   step {k + 20}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         secs = time.perf_counter() - t0
-        res[name] = dict(secs=secs, stats=st, psi=[m.download("PSI") for m in ms])
+        res[name] = dict(secs=secs, stats=st, psi=[m.download("PSI") for m in ms], div=div, tune_us=tune_us)
         ens.close()
         for m in ms:
             m.close()
@@ -374,7 +377,10 @@ def bench_ensemble(args, dev):
         "data": "synthetic (seeded GRF Ks and topography per member, uniform draws of alpha, n, rain)",
         "config": {"workload": f"batched ensemble of {len(ws)} members (nx, ny in 32..128, nz in 10..40), "
                                f"{ncell} cells in total, dt 30 s, {K} steps, one launch per kernel for all members, "
-                               f"Krylov batches replayed from a captured hipGraph", "device": dev},
+                               f"Krylov batches replayed from a captured hipGraph", "device": dev,
+                   "krylov_tiles": {n_: f"32x8 columns x ceil(nz/{res[n_]['div']}) planes, "
+                                        f"{res[n_]['tune_us']:.0f} us per batched iteration (ensemble autotuner)"
+                                    for n_ in res}},
         "fp64": {"cell_steps_per_s": ncell * K / res["fp64"]["secs"], "seconds": res["fp64"]["secs"]},
         "mixed": {"seconds": r["secs"], "picard_iters": sum(x["picard_iters"] for x in flat),
                   "krylov_iters": sum(x["lin_iters"] for x in flat),

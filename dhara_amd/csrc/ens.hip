@@ -75,6 +75,7 @@ struct dhara_ensemble {
     void* a_c1 = nullptr;
     std::vector<dh::Cg1Args<double>> h_c1d;
     std::vector<dh::Cg1Args<float>> h_c1f;
+    int kc_div = 0;                 // dhara_ensemble_autotune: planes per tile = ceil(nz / kc_div); 0: member's tile
 };
 
 namespace {
@@ -241,10 +242,11 @@ int upload_k3(dhara_ensemble* e)
 
 // Tiles of a member in the batched one-kernel PCG: 32 x 8 columns; the member's own (tuned)
 // planes per tile when its tile is 32 x 8, whole columns otherwise.
-void c1_tiles(const dhara_ensemble* e, const dhara_ctx* c, int& kc, int& ntx, int& nty, int& nkz)
+void c1_tiles(const dhara_ensemble* e, const dhara_ctx* c, int& kc, int& ntx, int& nty, int& nkz, int kc_div)
 {
     const dh::TileCfg& t = c->cg1_tile[e->mixed ? 1 : 0];
-    kc = (t.tx == dh::kCg1BatchTx && t.ty == dh::kCg1BatchTy && t.kc > 0) ? std::min(t.kc, c->g.nz) : c->g.nz;
+    if (kc_div > 0) kc = (c->g.nz + kc_div - 1) / kc_div;
+    else kc = (t.tx == dh::kCg1BatchTx && t.ty == dh::kCg1BatchTy && t.kc > 0) ? std::min(t.kc, c->g.nz) : c->g.nz;
     ntx = (c->g.nx + dh::kCg1BatchTx - 1) / dh::kCg1BatchTx;
     nty = (c->g.ny + dh::kCg1BatchTy - 1) / dh::kCg1BatchTy;
     nkz = (c->g.nz + kc - 1) / kc;
@@ -261,10 +263,24 @@ void fill_c1(dhara_ensemble* e, dhara_ctx* c, int m, bool first_sweep, const T* 
     a.part_asm = c->part[0];
     a.part[0] = c->part[0]; a.part[1] = c->part[1];
     a.nparts_asm = dh::grid_for(c->N);
-    c1_tiles(e, c, a.kc, a.ntx, a.nty, a.nkz);
+    c1_tiles(e, c, a.kc, a.ntx, a.nty, a.nkz, e->kc_div);
     a.ntiles = a.ntx * a.nty * a.nkz;
     const int maxit = e->mixed ? c->p.inner_maxiter : c->p.lin_maxiter;
     a.ctl = ctl(c, &e->st[m], maxit, c->p.lin_tol, e->mixed, first_sweep, nullptr, 0, nullptr);
+}
+
+// Block map of the batched one-kernel PCG for a planes-per-tile policy (kc_div, see c1_tiles).
+int c1_map(dhara_ensemble* e, int kc_div, std::vector<dh::BlkMap>& map)
+{
+    map.clear();
+    for (int m = 0; m < e->M; ++m) {
+        int kc, ntx, nty, nkz;
+        c1_tiles(e, e->mem[m], kc, ntx, nty, nkz, kc_div);
+        const int nt = ntx * nty * nkz;
+        if (nt > dh::kMaxBlocks) return eset(e, DHARA_E_UNSUPPORTED, "member has more tiles than partial slots");
+        for (int b = 0; b < nt; ++b) map.push_back(dh::BlkMap{m, b, nt});
+    }
+    return DHARA_OK;
 }
 
 // Arguments of the one-kernel PCG of every member (first_sweep: ||b||^2 from the assembly).
@@ -548,17 +564,14 @@ int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble**
     if (e->one) {
         e->h_c1d.resize(n);
         e->h_c1f.resize(n);
-        for (int m = 0; m < n; ++m) {
-            int kc, ntx, nty, nkz;
-            c1_tiles(e, e->mem[m], kc, ntx, nty, nkz);
-            const int nt = ntx * nty * nkz;
-            if (nt > dh::kMaxBlocks) {
-                dhara_ensemble_destroy(e);
-                return DHARA_E_UNSUPPORTED;
-            }
-            for (int b = 0; b < nt; ++b) mc1.push_back(dh::BlkMap{m, b, nt});
+        // room for the finest policy the autotuner tries (a third of the planes per tile)
+        std::vector<dh::BlkMap> finest;
+        if (c1_map(e, 0, mc1) || c1_map(e, 3, finest)) {
+            dhara_ensemble_destroy(e);
+            return DHARA_E_UNSUPPORTED;
         }
         e->nb_c1 = (int)mc1.size();
+        mc1.resize(std::max(mc1.size(), finest.size()), dh::BlkMap{0, 0, 1});
     }
     int rc = 0;
     if ((rc = ealloc(e, &e->map3, m3.size())) || (rc = ealloc(e, &e->map2, m2.size())) ||
@@ -576,7 +589,7 @@ int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble**
         if (e->mixed) rc = ealloc(e, (dh::Cg1Args<float>**)&e->a_c1, n);
         else rc = ealloc(e, (dh::Cg1Args<double>**)&e->a_c1, n);
         if (!rc) rc = ealloc(e, &e->map_c1, mc1.size());
-        if (!rc) rc = upload(e, e->map_c1, mc1);
+        if (!rc) rc = upload(e, e->map_c1, mc1);   // the first nb_c1 entries are live
     }
     if (rc) { dhara_ensemble_destroy(e); return rc; }
     if (hipHostMalloc((void**)&e->h_active, n * sizeof(int)) != hipSuccess ||
@@ -625,6 +638,115 @@ int dhara_ensemble_step(dhara_ensemble* e, int32_t nsteps, dhara_step_stats* sta
         for (int m = 0; m < e->M; ++m)
             if (st[m].nan_seen) return eset(e, DHARA_E_NUMERIC, "NaN/Inf in member " + std::to_string(m));
     }
+    return DHARA_OK;
+}
+
+// Ensemble-level tile autotuner of the batched one-kernel PCG: planes per tile = all, 1/2 or 1/3
+// of each member's columns (the 32 x 8 tile is fixed by the batched kernel), 3 + 10 launches of
+// every member's iteration kernel on scratch outputs and a never-converging scratch state; the
+// members' own arrays are only read.  Keeps the fastest (a later policy must be 3 % faster).
+int dhara_ensemble_autotune(dhara_ensemble* e, double* best_us, int32_t* planes_div)
+{
+    if (!e) return DHARA_E_ARG;
+    if (!e->one) return eset(e, DHARA_E_UNSUPPORTED, "autotune: members do not use the one-kernel PCG");
+    const int M = e->M;
+    const size_t es = e->mixed ? 4 : 8;
+    std::vector<void*> scratch;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(e->s);
+        for (void* v : scratch) (void)hipFree(v);
+    };
+    dh::PcgState* st = nullptr;
+    int* it1 = nullptr;
+    std::vector<void*> xs(M), r0s(M), r1s(M), ps(M), qs(M);   // x, r[0], r[1], p[0], p[1]
+    {
+        void* v = nullptr;
+        if (hipMalloc(&v, M * sizeof(dh::PcgState)) != hipSuccess) return eset(e, DHARA_E_ALLOC, "hipMalloc");
+        scratch.push_back(v);
+        st = (dh::PcgState*)v;
+        if (hipMalloc(&v, sizeof(int)) != hipSuccess) { cleanup(); return eset(e, DHARA_E_ALLOC, "hipMalloc"); }
+        scratch.push_back(v);
+        it1 = (int*)v;
+        for (int m = 0; m < M; ++m) {
+            const size_t nb = (size_t)e->mem[m]->N * es;
+            for (void** slot : {&xs[m], &r0s[m], &r1s[m], &ps[m], &qs[m]}) {
+                if (hipMalloc(slot, nb) != hipSuccess) { cleanup(); return eset(e, DHARA_E_ALLOC, "hipMalloc"); }
+                scratch.push_back(*slot);
+                (void)hipMemsetAsync(*slot, 0, nb, e->s);
+            }
+        }
+    }
+    dh::PcgState tune{};
+    tune.rz[0] = tune.rz[1] = 1.0;
+    tune.pi[0] = tune.pi[1] = 1.0;
+    tune.tol2 = -1.0;
+    std::vector<dh::PcgState> tunes(M, tune);
+    std::vector<double> ones(5 * dh::kMaxBlocks, 1.0);
+    const int one = 1;
+    EHIP(e, hipMemcpyAsync(st, tunes.data(), M * sizeof(dh::PcgState), hipMemcpyHostToDevice, e->s));
+    EHIP(e, hipMemcpyAsync(it1, &one, sizeof(int), hipMemcpyHostToDevice, e->s));
+    for (dhara_ctx* c : e->mem)
+        for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 5; ++v)
+                EHIP(e, hipMemcpyAsync(c->part[b] + v * dh::kMaxBlocks, ones.data(), dh::kMaxBlocks * sizeof(double),
+                                       hipMemcpyHostToDevice, e->s));
+    std::vector<int> all(M, 1);
+    int rc = set_active(e, all);
+    if (rc) { cleanup(); return rc; }
+    hipEvent_t e0, e1;
+    EHIP(e, hipEventCreate(&e0));
+    EHIP(e, hipEventCreate(&e1));
+    const int old_div = e->kc_div;
+    float best = 1e30f;
+    int best_div = 1;
+    for (int div : {1, 2, 3}) {
+        std::vector<dh::BlkMap> map;
+        if ((rc = c1_map(e, div, map))) break;
+        e->kc_div = div;
+        build_c1(e, false);
+        for (int m = 0; m < M; ++m) {   // scratch outputs and state
+            dhara_ctx* c = e->mem[m];
+            if (e->mixed) {
+                dh::Cg1Args<float>& a = e->h_c1f[m];
+                a.r[0] = (float*)r0s[m]; a.r[1] = (float*)r1s[m]; a.p[0] = (float*)ps[m]; a.p[1] = (float*)qs[m]; a.x = (float*)xs[m];
+                a.ctl.st = st + m;
+            } else {
+                dh::Cg1Args<double>& a = e->h_c1d[m];
+                a.r[0] = (double*)r0s[m]; a.r[1] = (double*)r1s[m]; a.p[0] = (double*)ps[m]; a.p[1] = (double*)qs[m]; a.x = (double*)xs[m];
+                a.ctl.st = st + m;
+            }
+            (void)c;
+        }
+        if ((rc = upload_c1(e)) || (rc = upload(e, e->map_c1, map))) break;
+        float ms = 0.f;
+        for (int rep = 0; rep < 13; ++rep) {
+            if (rep == 3) EHIP(e, hipEventRecord(e0, e->s));
+            // iteration 1 every time: reads part[0] (the ones, never written) and state slot 0,
+            // writes part[1] and slot 1, so every launch does the same full iteration
+            dh::launch_batched_cg1(e->s, e->mixed, e->a_c1, e->map_c1, (int)map.size(), e->active, it1, 0);
+        }
+        EHIP(e, hipEventRecord(e1, e->s));
+        EHIP(e, hipEventSynchronize(e1));
+        EHIP(e, hipGetLastError());
+        EHIP(e, hipEventElapsedTime(&ms, e0, e1));
+        const float us = ms * 1e3f / 10.f;
+        if (us < 0.97f * best || best >= 1e29f) { best = us; best_div = div; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cleanup();
+    if (rc) { e->kc_div = old_div; return rc; }
+    // keep the winner: its tile map (the solves rebuild their arguments from the members)
+    e->kc_div = best_div;
+    std::vector<dh::BlkMap> map;
+    if ((rc = c1_map(e, best_div, map)) || (rc = upload(e, e->map_c1, map))) return rc;
+    e->nb_c1 = (int)map.size();
+    if (e->graph3) {   // its launches carry the old grid size: captured again at the next solve
+        (void)hipGraphExecDestroy(e->graph3);
+        e->graph3 = nullptr;
+    }
+    if (best_us) *best_us = best;
+    if (planes_div) *planes_div = best_div;
     return DHARA_OK;
 }
 

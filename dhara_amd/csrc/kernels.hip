@@ -1360,11 +1360,11 @@ __device__ __forceinline__ void b_cg1(Geo g, const T* __restrict__ K, const T* _
         T RR1 = 0, RD1 = 1, RDI1 = 1, RK1 = 0;   // ring cell: r, D, jac_inv(D), K of plane m-1
 
         // step J (mod 6) of plane m.  The LDS ring slots are compile-time constants: sp/sq hold
-        // planes m, m-1, m-2 in slots a0, a1, a2, su/squ planes m-1, m-2 in b0, b1.  CHK =
-        // false: a step inside the steady range, where every plane test below is known true.
-        auto step = [&](auto J, auto CHK, int m) {
+        // planes m, m-1, m-2 in slots a0, a1, a2, su/squ planes m-1, m-2 in b0, b1.  (A second,
+        // check-free copy of the steps for the steady range measured slower: it raises the
+        // register count and lowers the occupancy more than it saves.)
+        auto step = [&](auto J, int m) {
             constexpr int j = decltype(J)::value;
-            constexpr bool chk = decltype(CHK)::value;
             constexpr int a0 = (3 - j % 3) % 3, a1 = (4 - j % 3) % 3, a2 = (5 - j % 3) % 3;
             constexpr int b0 = j % 2, b1 = 1 - j % 2;
             // phase 1: p_i, K p_i of plane m on the tile + two-cell halo
@@ -1385,9 +1385,7 @@ __device__ __forceinline__ void b_cg1(Geo g, const T* __restrict__ K, const T* _
                 sp[a0][hq2_1] = ph;
                 sq[a0][hq2_1] = L.hk1 * ph;
             }
-            if (!chk) {
-                load_plane(m + 1);
-            } else if (m + 1 <= k1 + 1) {
+            if (m + 1 <= k1 + 1) {
                 if (m + 1 >= 0 && m + 1 < g.nz) load_plane(m + 1);
                 else zero_plane();
             }
@@ -1395,8 +1393,8 @@ __device__ __forceinline__ void b_cg1(Geo g, const T* __restrict__ K, const T* _
             // phase 3: s_i, r_{i+1}, u_{i+1} of plane m-1 on the tile + one-cell halo
             const int pm = m - 1;
             T U1 = 0, QU1 = 0;
-            if (!chk || pm >= k0 - 1) {
-                const bool pv = !chk || (pm >= 0 && pm < g.nz);
+            if (pm >= k0 - 1) {
+                const bool pv = pm >= 0 && pm < g.nz;
                 if (pv) {
                     const T* pc = &sp[a1][obase];   // obase = oq2 - W2: neighbours at 0, W2-1, W2+1, 2 W2
                     const T* qc = &sq[a1][obase];
@@ -1410,7 +1408,7 @@ __device__ __forceinline__ void b_cg1(Geo g, const T* __restrict__ K, const T* _
                     const T rn = own_in ? fma(-alpha, sv, R1) : T(0);
                     U1 = rn * DI1;
                     QU1 = K1 * U1;
-                    if (!chk || (pm >= k0 && pm < k1)) {
+                    if (pm >= k0 && pm < k1) {
                         if (own_in) {
                             const unsigned c = (unsigned)pm * plane_bytes + obyte;
                             st_off<T>(r_out, c, rn);
@@ -1446,7 +1444,7 @@ __device__ __forceinline__ void b_cg1(Geo g, const T* __restrict__ K, const T* _
             __syncthreads();
             // phase 5: A u_{i+1} of plane m-2 on the tile (lanes without a cell have U2 = P2 = 0)
             const int pm2 = m - 2;
-            if (!chk || (pm2 >= k0 && pm2 < k1)) {
+            if (pm2 >= k0 && pm2 < k1) {
                 const T* uc = &su[b1][ubase];   // ubase = oq1 - W1
                 const T* quc = &squ[b1][ubase];
                 const T tx_ = fma(K2, uc[W1 - 1] + uc[W1 + 1], quc[W1 - 1] + quc[W1 + 1]);
@@ -1470,24 +1468,19 @@ __device__ __forceinline__ void b_cg1(Geo g, const T* __restrict__ K, const T* _
 
         zero_plane();
         if (k0 - 2 >= 0) load_plane(k0 - 2);
-        // (a second, check-free copy of the steps for the steady range measured slower: it
-        // raises the register count and lowers the occupancy more than it saves)
         const int mend = k1 + 1;
-        using Chk = std::integral_constant<bool, true>;
         for (int m = k0 - 2; m <= mend; m += 6) {
-            {
-                step(std::integral_constant<int, 0>{}, Chk{}, m);
-                if (m + 1 > mend) break;
-                step(std::integral_constant<int, 1>{}, Chk{}, m + 1);
-                if (m + 2 > mend) break;
-                step(std::integral_constant<int, 2>{}, Chk{}, m + 2);
-                if (m + 3 > mend) break;
-                step(std::integral_constant<int, 3>{}, Chk{}, m + 3);
-                if (m + 4 > mend) break;
-                step(std::integral_constant<int, 4>{}, Chk{}, m + 4);
-                if (m + 5 > mend) break;
-                step(std::integral_constant<int, 5>{}, Chk{}, m + 5);
-            }
+            step(std::integral_constant<int, 0>{}, m);
+            if (m + 1 > mend) break;
+            step(std::integral_constant<int, 1>{}, m + 1);
+            if (m + 2 > mend) break;
+            step(std::integral_constant<int, 2>{}, m + 2);
+            if (m + 3 > mend) break;
+            step(std::integral_constant<int, 3>{}, m + 3);
+            if (m + 4 > mend) break;
+            step(std::integral_constant<int, 4>{}, m + 4);
+            if (m + 5 > mend) break;
+            step(std::integral_constant<int, 5>{}, m + 5);
         }
         __syncthreads();   // the LDS rings are reused by the next tile
     }

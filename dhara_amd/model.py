@@ -246,6 +246,8 @@ def load_hip_library(path=HIP_LIB_PATH):
         lib.dhara_ensemble_set_graph.restype = C.c_int
         lib.dhara_ensemble_step.argtypes = [C.c_void_p, C.c_int32, P(_abi.StepStats)]
         lib.dhara_ensemble_step.restype = C.c_int
+        lib.dhara_ensemble_autotune.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int32)]
+        lib.dhara_ensemble_autotune.restype = C.c_int
         lib.dhara_profile_enable.argtypes = [C.c_void_p, C.c_int32]
         lib.dhara_profile_enable.restype = C.c_int
         lib.dhara_profile_read.argtypes = [C.c_void_p, P(_abi.KernelProfile)]
@@ -377,6 +379,13 @@ class Ensemble:
 
     def set_graph(self, on=True):
         self._check(self._lib.dhara_ensemble_set_graph(self._h, 1 if on else 0), "ensemble_set_graph")
+
+    def autotune(self):
+        """Pick the planes-per-tile policy of the batched one-kernel PCG (planes = ceil(nz / div));
+        returns (us per batched launch, div)."""
+        out, div = C.c_double(), C.c_int32()
+        self._check(self._lib.dhara_ensemble_autotune(self._h, C.byref(out), C.byref(div)), "ensemble_autotune")
+        return out.value, div.value
 
     def step(self, nsteps=1):
         """Returns stats[step][member] as dicts."""

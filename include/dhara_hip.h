@@ -377,6 +377,12 @@ int dhara_ensemble_set_graph(dhara_ensemble* ens, int32_t on);
 /* nsteps coupled steps of every member; stats (may be NULL) holds nsteps*n entries,
  * stats[step*n + member]. */
 int dhara_ensemble_step(dhara_ensemble* ens, int32_t nsteps, dhara_step_stats* stats);
+/* Tile autotuner of the ensemble's batched one-kernel PCG (members with DHARA_KRYLOV_ONE_KERNEL):
+ * times planes-per-tile policies (planes = ceil(nz / div), div = 1, 2, 3, on 32 x 8 tiles) on the
+ * members' current operators and keeps the fastest; best_us and planes_div (may be NULL) receive
+ * its time per batched launch and its div.  The members' states are not changed.  No reference
+ * counterpart. */
+int dhara_ensemble_autotune(dhara_ensemble* ens, double* best_us, int32_t* planes_div);
 
 #ifdef __cplusplus
 }

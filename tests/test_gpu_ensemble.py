@@ -85,3 +85,34 @@ def test_ensemble_rejects_mixed_precisions():
     b = member(SPECS[1], _abi.MIXED)
     with pytest.raises(dhara_amd.DharaError):
         dhara_amd.Ensemble([a, b])
+
+
+def test_autotune_keeps_states_and_parity():
+    steps = 3
+    ms = [member(s, _abi.FP64) for s in SPECS]
+    ens = dhara_amd.Ensemble(ms)
+    ens.step(1)
+    before = [m.download("PSI") for m in ms]
+    us, div = ens.autotune()
+    print("ensemble autotune:", us, "us, planes = ceil(nz /", div, ")", flush=True)
+    assert us > 0 and div in (1, 2, 3)
+    for m, b in zip(ms, before):
+        np.testing.assert_array_equal(m.download("PSI"), b)
+    st = ens.step(steps)
+    assert all(x["lin_converged"] for row in st for x in row)
+    for i, spec in enumerate(SPECS):
+        alone = member(spec, _abi.FP64)
+        alone.step(steps + 1)
+        assert np.max(np.abs(ms[i].download("PSI") - alone.download("PSI"))) <= 1e-8, i
+
+
+def test_autotune_times_full_iterations():
+    """Every timed launch of the ensemble autotuner is a full PCG iteration over all members: its
+    time per launch is at least the algorithmic bytes (8 FP64 arrays per cell) at 8 TB/s."""
+    specs = [dict(nx=128, ny=128, nz=32, sigma_lnk=1.0, rain_mm_h=20.0, seed=2100 + i) for i in range(8)]
+    ms = [member(s, _abi.FP64, untiled=False) for s in specs]
+    ens = dhara_amd.Ensemble(ms)
+    us, div = ens.autotune()
+    floor_us = sum(s["nx"] * s["ny"] * s["nz"] for s in specs) * 8 * 8 / 8e12 * 1e6
+    print(f"ensemble autotune {us:.1f} us (HBM floor {floor_us:.1f} us), planes = ceil(nz / {div})", flush=True)
+    assert us >= floor_us
