@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--refine-max", type=int, default=None, help="mixed: FP64 refinement sweeps per Picard iteration")
     ap.add_argument("--inner-tol", type=float, default=None, help="mixed: FP32 inner relative tolerance")
     ap.add_argument("--mixed-lin-tol", type=float, default=None, help="mixed: FP64 residual target")
+    ap.add_argument("--precond", choices=["jacobi", "column"], default="jacobi",
+                    help="subsurface PCG preconditioner (column: exact solve of each column's vertical "
+                         "tridiagonal part, two-kernel recurrences with a column-solve kernel)")
     ap.add_argument("--krylov", choices=["one", "two"], default="one",
                     help="subsurface PCG: one launch per iteration (single-reduction recurrences) or the "
                          "two-kernel Hestenes-Stiefel form")
@@ -111,7 +114,7 @@ def calibrate_copy():
     print(f"# calibration: 3 device-to-device copies of {n} bytes", file=sys.stderr, flush=True)
 
 
-def counted_bytes(stats, n3, n2, mixed, olf_mg=True, one_kernel=False):
+def counted_bytes(stats, n3, n2, mixed, olf_mg=True, one_kernel=False, column=False):
     """Algorithmic HBM bytes of the steps (per-kernel byte model of the solver)."""
     w = 4 if mixed else 8
     # overland CG iteration per surface cell: SpMV 6 and update 7 arrays; with the multigrid,
@@ -123,7 +126,11 @@ def counted_bytes(stats, n3, n2, mixed, olf_mg=True, one_kernel=False):
         P, I, R, Io = s["picard_iters"], s["lin_iters"], s["refine_sweeps"], s["olf_iters"]
         vg = 56 + (4 if mixed else 0)                  # read x, psi_m, Ks; write theta, K, C, psi_m
         asm = 80 if not mixed else 84                  # read 7 fields; write D, b, r (or D, b, D32, r32, d32)
-        if one_kernel:
+        if column:
+            # SpMV (read z, p, K, D; write p, q) + column kernel (read x, p, q, r, K, D; write
+            # x, r, z) per iteration, a column start-up (read r, K, D; write z) per segment
+            krylov = I * 15 * w + (P + R) * 4 * w
+        elif one_kernel:
             # one launch per iteration: read r, D, p, K, x; write r, p, x (8 arrays), plus a
             # start-up pass per segment (read r, D, K, p: 4 arrays)
             krylov = I * 8 * w + (P + R) * 4 * w
@@ -179,9 +186,11 @@ def main():
 
     w64 = inputs.c2_coupled(nx=nx, ny=ny, nz=nz)
     w32 = inputs.c3_coupled_mixed(nx=nx, ny=ny, nz=nz)
-    one_kernel = args.krylov == "one"
+    column = args.precond == "column"
+    one_kernel = args.krylov == "one" and not column
     for w_ in (w64, w32):
-        w_.params.krylov = _abi.KRYLOV_ONE_KERNEL if one_kernel else _abi.KRYLOV_TWO_KERNEL
+        w_.params.krylov = _abi.KRYLOV_ONE_KERNEL if args.krylov == "one" else _abi.KRYLOV_TWO_KERNEL
+        w_.params.precond = _abi.PRECOND_COLUMN if column else _abi.PRECOND_JACOBI
     if args.refine_max is not None:
         w32.params.refine_maxiter = args.refine_max
     if args.inner_tol is not None:
@@ -211,7 +220,8 @@ def main():
                            "kernel": "k_cg1 (tx, ty, planes per tile, grid cap)"}
         else:
             tiles[name] = {"fp64": m.get_tile(8), "fp32": m.get_tile(4), "tune_us": tune_us,
-                           "kernel": "k_pcg3_spmv_tiled (tx, ty, planes per block)"}
+                           "kernel": "k_pcg3_spmv_tiled (tx, ty, planes per block)"
+                                     + ("; column solve: one thread per column" if column else "")}
         w.apply(m)
         print(f"# {name}: {K} steps", file=sys.stderr, flush=True)
         stats, secs, snaps = run_timed(m, w, K, marks=cpu_marks if name == "fp64" else ())
@@ -240,7 +250,8 @@ def main():
         "data": "synthetic (seeded GRF topography and log-normal Ks; no external data)",
         "config": {"workload": f"coupled {nx}x{ny}x{nz} (surface {nx}x{ny}), 20 mm/h rain 3600 s + "
                                f"3600 s recession, dt 30 s; {head} run timed over steps 0..{K - 1}",
-                   "precision": head, "device": dev, "krylov": args.krylov, "tiles": tiles,
+                   "precision": head, "device": dev, "precond": args.precond,
+                   "krylov": "two (column-solve kernel)" if column else args.krylov, "tiles": tiles,
                    "mixed_solver": {"refine_maxiter": w32.params.refine_maxiter, "inner_tol": w32.params.inner_tol,
                                     "lin_tol": w32.params.lin_tol},
                    "fp64_solver": {"lin_tol": w64.params.lin_tol}},
@@ -256,7 +267,7 @@ def main():
             "overland_iters": sum(s["olf_iters"] for s in st),
             "picard_unconverged_steps": sum(1 for s in st if not s["picard_converged"]),
             "max_mb_rel_converged_steps": max((s["mb_rel"] for s in conv), default=None),
-            "counted_hbm_bytes": counted_bytes(st, n3, n2, name == "mixed", one_kernel=one_kernel),
+            "counted_hbm_bytes": counted_bytes(st, n3, n2, name == "mixed", one_kernel=one_kernel, column=column),
         }
         out[name]["step_hbm_frac"] = out[name]["counted_hbm_bytes"] / (rr["secs"] * HBM_PEAK)
     if "fp64" in runs and "mixed" in runs:
@@ -268,7 +279,9 @@ def main():
     # roofline of the dominant kernel, from the library's HIP-event timing
     pr = r["prof"]
     kern = []
-    for kname, key in ((("cg1" if one_kernel else "pcg3_spmv"), "spmv"), ("pcg_update", "update")):
+    names = ((("pcg3_spmv_z" if column else "cg1" if one_kernel else "pcg3_spmv"), "spmv"),
+             (("line" if column else "pcg_update"), "update"))
+    for kname, key in names:
         nl = pr[f"{key}_launches"]
         if nl:
             avg = pr[f"{key}_ms"] / nl * 1e-3
@@ -301,7 +314,8 @@ def main():
     if not args.no_cpu and "fp64" in runs and cpu_start in runs["fp64"]["snaps"]:
         from oracle import oracle
         threads = oracle.set_threads(0)
-        om = oracle.OracleModel(w64.grid, cp(w64.params))
+        om = oracle.OracleModel(w64.grid, cp(w64.params),
+                                lin_solver=oracle.LIN_PCG_LINE if column else oracle.LIN_PCG_JACOBI)
         om.upload_many(w64.fields)
         psi0, h0 = runs["fp64"]["snaps"][cpu_start]
         om.upload("PSI", psi0)
@@ -318,7 +332,8 @@ def main():
         out["cpu_baseline"] = {
             "value": n3 * args.cpu_steps / cs, "unit": "cell-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle FP64 (C++/OpenMP) on steps {cpu_start}..{cpu_start + args.cpu_steps - 1} of the "
+            "sample": f"oracle FP64 (C++/OpenMP, {'column' if column else 'Jacobi'}-preconditioned CG) on steps "
+                      f"{cpu_start}..{cpu_start + args.cpu_steps - 1} of the "
                       f"{nx}x{ny}x{nz} coupled workload, started from the GPU FP64 state",
             "seconds": cs, "gpu_fp64_same_steps_s": gpu_win,
             "gpu_fp64_speedup_same_steps": cs / gpu_win if gpu_win > 0 else None,

@@ -5,7 +5,7 @@ else that implements the same entry points) is driven by the same definitions.
 """
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK, E_ARG, E_DEVICE, E_ALLOC, E_STATE, E_NUMERIC, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
 
@@ -14,6 +14,7 @@ FP64, MIXED = 0, 1
 THETA_CARRY_CONSISTENT, THETA_CARRY_REFERENCE = 0, 1
 OLF_PRECOND_JACOBI, OLF_PRECOND_MG = 0, 1
 KRYLOV_TWO_KERNEL, KRYLOV_ONE_KERNEL = 0, 1
+PRECOND_JACOBI, PRECOND_COLUMN = 0, 1
 TOP_SURFACE, TOP_PRESCRIBED = 0, 1
 INIT_FROM_THETA, INIT_FROM_PSI = 0, 1
 BC_DIRICHLET, BC_FLUX = 0, 1
@@ -60,6 +61,7 @@ class Params(C.Structure):
         ("theta_carry", C.c_int32), ("dirichlet_face_quirk", C.c_int32),
         ("top_bc", C.c_int32), ("olf_precond", C.c_int32),
         ("krylov", C.c_int32),
+        ("precond", C.c_int32),
     ]
 
 
@@ -112,7 +114,8 @@ def default_params(**overrides):
         overland=1, olf_maxiter=2000, olf_tol=1e-12,
         hmin=1e-5, delta=1e-7, K0=1e-5,
         theta_carry=THETA_CARRY_CONSISTENT, dirichlet_face_quirk=1,
-        top_bc=TOP_SURFACE, olf_precond=OLF_PRECOND_MG, krylov=KRYLOV_ONE_KERNEL)
+        top_bc=TOP_SURFACE, olf_precond=OLF_PRECOND_MG, krylov=KRYLOV_ONE_KERNEL,
+        precond=PRECOND_JACOBI)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(f"unknown parameter {k!r}")

@@ -94,6 +94,18 @@ int alloc_mixed(dhara_ctx* c)
     return DHARA_OK;
 }
 
+int alloc_line(dhara_ctx* c, bool mixed)
+{
+    int rc;
+    if (!c->part_line && (rc = dalloc(c, &c->part_line, (size_t)4 * dh::kMaxBlocks))) return rc;
+    if (mixed) {
+        if (!c->zl32 && (rc = dalloc(c, &c->zl32, (size_t)c->N))) return rc;
+    } else if (!c->zl && (rc = dalloc(c, &c->zl, (size_t)c->N))) {
+        return rc;
+    }
+    return DHARA_OK;
+}
+
 int validate_params(dhara_ctx* c, const dhara_params* p)
 {
     if (!(p->dt > 0) || !(p->theta_s > p->theta_r) || !(p->alpha > 0) || !(p->n_vg > 1.0) ||
@@ -106,6 +118,8 @@ int validate_params(dhara_ctx* c, const dhara_params* p)
         return set_err(c, DHARA_E_ARG, "bad top_bc");
     if (p->krylov != DHARA_KRYLOV_TWO_KERNEL && p->krylov != DHARA_KRYLOV_ONE_KERNEL)
         return set_err(c, DHARA_E_ARG, "bad krylov");
+    if (p->precond != DHARA_PRECOND_JACOBI && p->precond != DHARA_PRECOND_COLUMN)
+        return set_err(c, DHARA_E_ARG, "bad precond");
     return DHARA_OK;
 }
 
@@ -420,6 +434,143 @@ int solve_fp64_cg1(dhara_ctx* c, dhara_step_stats* st)
     return DHARA_OK;
 }
 
+// Column-preconditioned PCG (no Dirichlet rows): the Hestenes-Stiefel iteration of solve_fp64
+// with z = T^-1 r (T: the vertical tridiagonal part of A) from the column kernel, which also
+// does the x / r update.  Partials: assembly -> part[0] -> column start-up -> part_line ->
+// SpMV (first iteration) -> part[1] -> column update -> part[0] -> SpMV -> ...
+int solve_fp64_line(dhara_ctx* c, dhara_step_stats* st)
+{
+    int rc;
+    if ((rc = alloc_line(c, false))) return rc;
+    dh::launch_assemble(c->s, asm_args(c, false), false);
+    LAUNCHCHK(c);
+    double* const pp[2] = {c->pv[0], c->pv[1]};
+    dh::SolveCtl sc = solve_ctl(c, c->st_sub, c->p.lin_maxiter, dh::grid_for(c->N), false, true);
+    const dh::TileCfg tc = c->tile[0];
+    const int spmv_parts = tc.tx ? dh::tile_blocks(c->g, tc) : dh::grid_for(c->N);
+    const int lparts = dh::line_blocks(c->g);
+    {
+        dh::SolveCtl k = sc;   // assembly partials in part[0]
+        k.part_out = c->part_line;
+        dh::launch_line_init_f64(c->s, c->g, c->K, c->D, c->r, c->zl, k);
+    }
+    rc = run_krylov(
+        c, c->st_sub, c->p.lin_maxiter, c->pcg_hint, c->prof, 8.0,
+        [&](int it, const dh::LaunchEv& ev) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            k.part_in = it == 0 ? c->part_line : c->part[0];
+            k.nparts = lparts;
+            dh::launch_pcg3_spmv_z_f64(c->s, c->g, tc, c->K, c->D, c->zl, pp, c->q, k, ev);
+        },
+        [&](int it, const dh::LaunchEv& ev) {
+            dh::SolveCtl k = sc;
+            k.it = it;
+            k.part_in = c->part[1];
+            k.nparts = spmv_parts;
+            k.part_out = c->part[0];
+            dh::launch_line_update_f64(c->s, c->g, c->K, c->D, c->x, c->r, pp[(it + 1) & 1], c->q, c->zl, k, ev);
+        });
+    if (rc) return rc;
+    c->prof_acc.update_bytes = 9.0 * 8.0 * c->N;   // x, p, q, r, K, D read; x, r, z written
+    st->lin_iters += c->h_st->iter;
+    st->lin_converged &= c->h_st->conv;
+    if (c->h_st->nan) st->nan_seen = 1;
+    return DHARA_OK;
+}
+
+// Mixed precision with the column preconditioner: the reliable-update scheme of solve_mixed,
+// FP32 column solves; every segment starts with the column start-up kernel on the fresh
+// residual, whose partials the segment's first SpMV tests (outer FP64 test included).
+int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
+{
+    int rc;
+    if ((rc = alloc_line(c, true))) return rc;
+    dh::launch_assemble(c->s, asm_args(c, false), true);
+    LAUNCHCHK(c);
+    float* const pp[2] = {c->p32[0], c->p32[1]};
+    dh::RefineArgs ra{};
+    ra.g = c->g;
+    ra.K = c->K;
+    ra.D = c->D;
+    ra.b = c->b;
+    ra.x = c->x;
+    ra.flags = nullptr;
+    ra.r32 = c->r32;
+    ra.D32 = c->D32;
+    ra.d32 = c->d32;
+    ra.part_out = c->part[0];
+    ra.st = c->st_sub;
+    const int maxit = c->p.inner_maxiter;
+    const dh::TileCfg tc = c->tile[1];
+    const int spmv_parts = tc.tx ? dh::tile_blocks(c->g, tc) : dh::grid_for(c->N);
+    const int lparts = dh::line_blocks(c->g);
+    dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, true);
+    int it = 0, updates = 0, refine_parts = dh::grid_for(c->N);
+    bool outer_conv = false;
+    auto start = [&](int at) {
+        dh::SolveCtl k = sc;   // fresh residual partials in part[0]
+        k.it = at;
+        k.nparts = refine_parts;
+        k.part_out = c->part_line;
+        dh::launch_line_init_f32(c->s, c->g, c->K32, c->D32, c->r32, c->zl32, k);
+    };
+    for (;;) {
+        start(it);
+        const int seg = it;
+        rc = run_krylov(
+            c, c->st_sub, maxit, c->pcg_hint, c->prof, 4.0,
+            [&](int j, const dh::LaunchEv& ev) {
+                dh::SolveCtl k = sc;
+                k.it = j;
+                k.segment_start = (j == seg) ? 1 : 0;
+                k.part_in = j == seg ? c->part_line : c->part[0];
+                k.nparts = lparts;
+                dh::launch_pcg3_spmv_z_f32(c->s, c->g, tc, c->K32, c->D32, c->zl32, pp, c->q32, k, ev);
+            },
+            [&](int j, const dh::LaunchEv& ev) {
+                dh::SolveCtl k = sc;
+                k.it = j;
+                k.part_in = c->part[1];
+                k.nparts = spmv_parts;
+                k.part_out = c->part[0];
+                dh::launch_line_update_f32(c->s, c->g, c->K32, c->D32, c->d32, c->r32, pp[(j + 1) & 1], c->q32,
+                                           c->zl32, k, ev);
+            },
+            it);
+        if (rc) return rc;
+        c->prof_acc.update_bytes = 9.0 * 4.0 * c->N;
+        if (c->h_st->nan) { st->nan_seen = 1; break; }
+        if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
+        const int it_end = c->h_st->iter;
+        refine_parts = dh::launch_refine_tiled(c->s, ra, c->tile[0]);
+        LAUNCHCHK(c);
+        ++updates;
+        if (it_end >= maxit || updates > c->p.refine_maxiter) {
+            // final convergence check only: a segment start at the limit ends the solve
+            start(it_end);
+            dh::SolveCtl k = sc;
+            k.it = it_end;
+            k.maxit = it_end;
+            k.segment_start = 1;
+            k.first_sweep = 0;
+            k.part_in = c->part_line;
+            k.nparts = lparts;
+            dh::launch_pcg3_spmv_z_f32(c->s, c->g, tc, c->K32, c->D32, c->zl32, pp, c->q32, k);
+            LAUNCHCHK(c);
+            if ((rc = read_state(c, c->st_sub))) return rc;
+            outer_conv = c->h_st->outer_conv != 0;
+            break;
+        }
+        it = it_end;
+        sc.first_sweep = 0;
+    }
+    st->lin_iters += c->h_st->iter;
+    st->refine_sweeps += updates;
+    st->lin_converged &= outer_conv ? 1 : 0;
+    return DHARA_OK;
+}
+
 // Mixed precision with the one-kernel PCG: the reliable-update scheme of solve_mixed; every
 // segment (first sweep, and after each FP64 update) starts with the start-up kernel, which
 // also makes the outer convergence test.
@@ -531,8 +682,10 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
     while (!conv && niter < c->p.picard_maxiter) {
         // (the one-kernel form addresses planes with 32-bit byte offsets)
         const bool one = c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !dir && (size_t)c->N * 8 < (size_t(1) << 32);
-        int rc = mixed ? (one ? solve_mixed_cg1(c, st) : solve_mixed(c, st))
-                       : (one ? solve_fp64_cg1(c, st) : solve_fp64(c, st));
+        const bool line = c->p.precond == DHARA_PRECOND_COLUMN && !dir && dh::line_supported(c->g);
+        int rc = line ? (mixed ? solve_mixed_line(c, st) : solve_fp64_line(c, st))
+                      : mixed ? (one ? solve_mixed_cg1(c, st) : solve_mixed(c, st))
+                              : (one ? solve_fp64_cg1(c, st) : solve_fp64(c, st));
         if (rc) return rc;
         if (st->nan_seen) { c->theta_m_current = false; break; }
         dh::launch_vg_picard(c->s, vg_args(c, c->x, c->theta_new));

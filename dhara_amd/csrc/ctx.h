@@ -49,6 +49,11 @@ struct dhara_ctx {
     double* z2 = nullptr;
     // reductions: partial buffers A (written by assembly/update kernels) and B (SpMV kernels)
     double* part[2] = {nullptr, nullptr};
+    // column-preconditioned PCG (allocated on first use): z = T^-1 r in FP64 / FP32 and the
+    // partials of its segment start-up kernel
+    double* zl = nullptr;
+    float* zl32 = nullptr;
+    double* part_line = nullptr;
     dh::PcgState* st_sub = nullptr;
     dh::PcgState* st_olf = nullptr;
     // pinned host mirrors
@@ -116,6 +121,7 @@ dh::OlfArgs olf_args(dhara_ctx* c);
 dh::SurfArgs surf_args(dhara_ctx* c);
 dh::AsmArgs asm_args(dhara_ctx* c, bool dir);
 int alloc_mixed(dhara_ctx* c);
+int alloc_line(dhara_ctx* c, bool mixed);
 dh::SolveCtl solve_ctl(dhara_ctx* c, dh::PcgState* st, int maxit, int nparts, bool mixed_inner,
                        bool first_sweep);
 int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st);

@@ -216,6 +216,33 @@ void launch_cg1_init_f64(hipStream_t s, const Geo& g, const TileCfg& t, const do
                          const double* r, const double* p_old, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
 void launch_cg1_init_f32(hipStream_t s, const Geo& g, const TileCfg& t, const float* K, const float* D,
                          const float* r, const float* p_old, const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
+// Column-preconditioned PCG (z = T^-1 r, T the vertical tridiagonal part of A; no Dirichlet rows,
+// nz <= kLineMaxNz), two kernels per iteration:
+//   spmv_z: p = z + beta p (p[it&1] -> p[(it+1)&1]), q = A p, partial (p, q) -- tiled on t
+//           (t.tx = 0: untiled); its partial count is tile_blocks / grid_for(n) as for Jacobi
+//   line update: x += alpha p, r -= alpha q, z = T^-1 r, partials [(r, z), (r, r)]
+//   line init (segment start): z = T^-1 r, partials [(r, z), (r, r)] + slots 2, 3 of the
+//           assembly / reliable-update partials
+// The line kernels write line_blocks(g) partials.
+constexpr int kLineMaxNz = 64;
+bool line_supported(const Geo& g);
+int line_blocks(const Geo& g);
+void launch_pcg3_spmv_z_f64(hipStream_t s, const Geo& g, const TileCfg& t, const double* K, const double* D,
+                            const double* z, double* const p[2], double* q, const SolveCtl& sc,
+                            const LaunchEv& ev = LaunchEv{});
+void launch_pcg3_spmv_z_f32(hipStream_t s, const Geo& g, const TileCfg& t, const float* K, const float* D,
+                            const float* z, float* const p[2], float* q, const SolveCtl& sc,
+                            const LaunchEv& ev = LaunchEv{});
+void launch_line_init_f64(hipStream_t s, const Geo& g, const double* K, const double* D, const double* r, double* z,
+                          const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
+void launch_line_init_f32(hipStream_t s, const Geo& g, const float* K, const float* D, const float* r, float* z,
+                          const SolveCtl& sc, const LaunchEv& ev = LaunchEv{});
+void launch_line_update_f64(hipStream_t s, const Geo& g, const double* K, const double* D, double* x, double* r,
+                            const double* p, const double* q, double* z, const SolveCtl& sc,
+                            const LaunchEv& ev = LaunchEv{});
+void launch_line_update_f32(hipStream_t s, const Geo& g, const float* K, const float* D, float* x, float* r,
+                            const float* p, const float* q, float* z, const SolveCtl& sc,
+                            const LaunchEv& ev = LaunchEv{});
 // y = A x (tests)
 void launch_apply3_f64(hipStream_t s, const Geo& g, const uint8_t* flags, const double* K,
                        const double* D, const double* x, double* y);

@@ -461,7 +461,8 @@ __device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha, 
     return true;
 }
 
-template <typename T, bool DIR>
+// ZIN: r holds the preconditioned residual z (column preconditioner) instead of r.
+template <typename T, bool DIR, bool ZIN = false>
 __device__ __forceinline__ void b_pcg3_spmv(Geo g, const uint8_t* __restrict__ flags,
                                                       const T* __restrict__ K, const T* __restrict__ D,
                                                       const T* __restrict__ r, const T* __restrict__ pold,
@@ -472,7 +473,7 @@ __device__ __forceinline__ void b_pcg3_spmv(Geo g, const uint8_t* __restrict__ f
     const bool first = (sc.it == 0);
     const T beta = (T)beta_d;
     auto pf = [&](int c) -> T {
-        const T z = r[c] / D[c];
+        const T z = ZIN ? r[c] : r[c] / D[c];
         return first ? z : z + beta * pold[c];
     };
     double acc[1] = {0.0};
@@ -489,13 +490,13 @@ __device__ __forceinline__ void b_pcg3_spmv(Geo g, const uint8_t* __restrict__ f
     block_partials<1, false>(acc, sc.part_out, bk.id);
 }
 
-template <typename T, bool DIR>
+template <typename T, bool DIR, bool ZIN = false>
 __global__ __launch_bounds__(kBlock) void k_pcg3_spmv(Geo g, const uint8_t* __restrict__ flags,
                                                       const T* __restrict__ K, const T* __restrict__ D,
                                                       const T* __restrict__ r, const T* __restrict__ pold,
                                                       T* __restrict__ pnew, T* __restrict__ q, SolveCtl sc)
 {
-    b_pcg3_spmv<T, DIR>(g, flags, K, D, r, pold, pnew, q, sc, hw_blk());
+    b_pcg3_spmv<T, DIR, ZIN>(g, flags, K, D, r, pold, pnew, q, sc, hw_blk());
 }
 
 template <typename T>
@@ -527,14 +528,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
     b_pcg_update<T>(n, x, r, D, p, q, sc, hw_blk());
 }
 
-// LDS-tiled 2.5D SpMV.  Block = TX x TY threads owning TX x TY columns; the block marches
-// through planes k0..k0+kc-1.  For each plane the new search direction p = M^-1 r + beta p
-// and the conductivity K are written into a (TY+2) x (TX+2) LDS tile (the one-cell halo is
-// computed by the first 2(TX+TY) threads); the k-1 / k+1 values of the thread's own column
-// come from registers (one plane of look-ahead).  Two LDS buffers alternate by plane parity,
-// so one barrier per plane suffices.  Blocks are numbered so that each XCD (workgroups are
-// dealt to the 8 XCDs round-robin) gets a contiguous band of tiles: neighbouring tiles, whose
-// halos overlap, then share that XCD's L2.
 // Jacobi scaling of the one-kernel PCG: u = r * jac_inv(D), the hardware reciprocal (FP32) or
 // the reciprocal refined by two Newton steps (FP64).  It is a fixed diagonal preconditioner like
 // r / D, cheaper than an IEEE division; every kernel of the one-kernel form uses this function,
@@ -546,6 +539,104 @@ __device__ __forceinline__ double jac_inv(double d)
     y = fma(y, fma(-d, y, 1.0), y);
     return fma(y, fma(-d, y, 1.0), y);
 }
+
+// Column (vertical line) preconditioner: z = T^-1 r with T the vertical tridiagonal part of the
+// 7-point operator -- diagonal D, coupling to the cell above / below -(wz/2)(K_k + K_k-+1), as
+// in the SpMV --, i.e. one independent tridiagonal system per column of cells, solved by the
+// Thomas algorithm with the column's forward factors in registers (NZM >= nz).  The flow
+// operator's vertical couplings exceed the horizontal ones by (dx/dz)^2 (400 at dz = 5 cm);
+// Jacobi scaling leaves that stiffness to the Krylov iteration, the column solve removes it.
+// Thread = column: the plane-major layout makes every plane access coalesced.
+//   INIT:   start of a solve segment: z from r; partials [(r, z), (r, r), S2, S3] with S2, S3
+//           the totals of slots 2, 3 of the previous kernel's partials (||b||^2 and the FP64
+//           residual of the assembly or reliable update), carried over by block 0.
+//   update: alpha = (r, z) / (p, q) from the SpMV partials, x += alpha p, r -= alpha q, then z
+//           from the new r; partials [(r, z), (r, r)].
+// (r, z) is accumulated as sum_k w_k y_k with w_k = r_k - a_k y_k-1 = den_k y_k (T = L diag(den)
+// L^T): the same value as sum_k r_k z_k, as a sum of positive terms.  The reciprocal of the
+// pivot is jac_inv, a fixed function of the operator, so every application is the same
+// linear map.
+template <typename T, int NZM, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                                 T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                                                 const T* __restrict__ q, T* __restrict__ z, SolveCtl sc)
+{
+    double alpha_d = 0.0;
+    double carry[2] = {0.0, 0.0};
+    if constexpr (INIT) {
+        if (sc.st->done) return;
+        double in[4];
+        sum_partials<4>(sc.part_in, sc.nparts, 4, in);
+        carry[0] = in[2];
+        carry[1] = in[3];
+    } else {
+        if (!update_begin(sc, alpha_d, blockIdx.x)) return;
+    }
+    const T alpha = (T)alpha_d;
+    const T hz = (T)(0.5 * g.wz);
+    const int nz = g.nz, nxy = g.nxy;
+    double acc[INIT ? 4 : 2] = {};
+    for (int col = blockIdx.x * kBlock + threadIdx.x; col < nxy; col += gridDim.x * kBlock) {
+        T y[NZM], cp[NZM];
+        T kprev = T(0), yprev = T(0), inv_prev = T(0);
+#pragma unroll
+        for (int k = 0; k < NZM; ++k) {
+            if (k < nz) {
+                const int c = k * nxy + col;
+                T rc = r[c];
+                if constexpr (!INIT) {
+                    rc -= alpha * q[c];
+                    r[c] = rc;
+                    x[c] += alpha * p[c];
+                }
+                const T kc = K[c];
+                T den = D[c], w = rc;
+                if (k > 0) {
+                    const T a = -hz * (kc + kprev);   // T[k][k-1] = T[k-1][k]
+                    const T cpk = a * inv_prev;       // c'_{k-1}
+                    cp[k - 1] = cpk;
+                    den -= a * cpk;
+                    w -= a * yprev;
+                }
+                const T inv = jac_inv(den);
+                const T yk = w * inv;
+                y[k] = yk;
+                acc[0] += (double)w * (double)yk;
+                acc[1] += (double)rc * (double)rc;
+                kprev = kc;
+                yprev = yk;
+                inv_prev = inv;
+            }
+        }
+        T zn = T(0);
+#pragma unroll
+        for (int k = NZM - 1; k >= 0; --k) {
+            if (k < nz) {
+                const T zk = (k == nz - 1) ? y[k] : y[k] - cp[k] * zn;
+                z[k * nxy + col] = zk;
+                zn = zk;
+            }
+        }
+    }
+    if constexpr (INIT) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            acc[2] = carry[0];
+            acc[3] = carry[1];
+        }
+        block_partials<4, false>(acc, sc.part_out);
+    } else {
+        block_partials<2, false>(acc, sc.part_out);
+    }
+}
+
+// LDS-tiled 2.5D SpMV.  Block = TX x TY threads owning TX x TY columns; the block marches
+// through planes k0..k0+kc-1.  For each plane the new search direction p = M^-1 r + beta p
+// and the conductivity K are written into a (TY+2) x (TX+2) LDS tile (the one-cell halo is
+// computed by the first 2(TX+TY) threads); the k-1 / k+1 values of the thread's own column
+// come from registers (one plane of look-ahead).  Two LDS buffers alternate by plane parity,
+// so one barrier per plane suffices.  Blocks are numbered so that each XCD (workgroups are
+// dealt to the 8 XCDs round-robin) gets a contiguous band of tiles: neighbouring tiles, whose
+// halos overlap, then share that XCD's L2.
 
 // Load / store at a 32-bit byte offset from a kernel-argument base (the scalar-base plus vector
 // offset addressing form; the caller guarantees that every offset fits in 32 bits).
@@ -572,7 +663,8 @@ __device__ __forceinline__ int xcd_tile(int w, int nb)
 
 // INIT: start-up of a one-kernel PCG segment instead -- u = M^-1 r, A u, no stores, partials
 // [(r, u), (r, r), (u, A u), (p_old, A u), 0] (p_old: the kept direction, it > 0).
-template <typename T, bool DIR, int TX, int TY, bool INIT = false>
+// ZIN: r holds the preconditioned residual z (column preconditioner): p = z + beta p.
+template <typename T, bool DIR, int TX, int TY, bool INIT = false, bool ZIN = false>
 __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_t* __restrict__ flags,
                                                              const T* __restrict__ K, const T* __restrict__ D,
                                                              const T* __restrict__ r, const T* __restrict__ pold,
@@ -611,7 +703,10 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
     const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
     const int hcol = hj * g.nx + hi;
 
-    auto jac = [&](T rv, T dv) -> T { return INIT ? rv * jac_inv(dv) : rv / dv; };
+    auto jac = [&](T rv, T dv) -> T {
+        if constexpr (ZIN) return rv;
+        else return INIT ? rv * jac_inv(dv) : rv / dv;
+    };
     auto pval = [&](int c) -> T {
         const T z = jac(r[c], D[c]);
         return first ? z : z + beta * pold[c];
@@ -1687,13 +1782,16 @@ int tile_blocks(const Geo& g, const TileCfg& t)
     return ntx * nty * nkz;
 }
 
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, bool ZIN = false>
 static void tiled_launch(hipStream_t s, const Geo& g, int kc, const uint8_t* flags, const T* K, const T* D,
                          const T* r, const T* pold, T* pnew, T* q, const SolveCtl& sc, const LaunchEv& ev)
 {
     const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, nkz = (g.nz + kc - 1) / kc;
     const dim3 gr(ntx * nty * nkz), bl(TX, TY);
-    if (flags)
+    if constexpr (ZIN)
+        launch_ev(k_pcg3_spmv_tiled<T, false, TX, TY, false, true>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q,
+                  sc, kc, ntx, nty, nkz);
+    else if (flags)
         launch_ev(k_pcg3_spmv_tiled<T, true, TX, TY>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q, sc, kc,
                   ntx, nty, nkz);
     else
@@ -1701,17 +1799,81 @@ static void tiled_launch(hipStream_t s, const Geo& g, int kc, const uint8_t* fla
                   ntx, nty, nkz);
 }
 
-template <typename T>
+template <typename T, bool ZIN = false>
 static void tiled_dispatch(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags, const T* K,
                            const T* D, const T* r, T* const p[2], T* q, const SolveCtl& sc, const LaunchEv& ev)
 {
     const T* pold = p[sc.it & 1];
     T* pnew = p[(sc.it + 1) & 1];
-    if (t.tx == 32 && t.ty == 8) tiled_launch<T, 32, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
-    else if (t.tx == 64 && t.ty == 4) tiled_launch<T, 64, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
-    else if (t.tx == 16 && t.ty == 16) tiled_launch<T, 16, 16>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
-    else if (t.tx == 32 && t.ty == 4) tiled_launch<T, 32, 4>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
-    else tiled_launch<T, 64, 8>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    if (t.tx == 32 && t.ty == 8) tiled_launch<T, 32, 8, ZIN>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else if (t.tx == 64 && t.ty == 4) tiled_launch<T, 64, 4, ZIN>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else if (t.tx == 16 && t.ty == 16) tiled_launch<T, 16, 16, ZIN>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else if (t.tx == 32 && t.ty == 4) tiled_launch<T, 32, 4, ZIN>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+    else tiled_launch<T, 64, 8, ZIN>(s, g, t.kc, flags, K, D, r, pold, pnew, q, sc, ev);
+}
+
+// SpMV kernel of the column-preconditioned PCG: p = z + beta p, q = A p, (p, q); tiled on t, or
+// untiled when t.tx = 0.  No Dirichlet rows.
+template <typename T>
+static void spmv_z(hipStream_t s, const Geo& g, const TileCfg& t, const T* K, const T* D, const T* z, T* const p[2],
+                   T* q, const SolveCtl& sc, const LaunchEv& ev)
+{
+    if (t.tx) {
+        tiled_dispatch<T, true>(s, g, t, nullptr, K, D, z, p, q, sc, ev);
+    } else {
+        launch_ev(k_pcg3_spmv<T, false, true>, dim3(grid_for(g.n)), dim3(kBlock), s, ev, g, (const uint8_t*)nullptr,
+                  K, D, z, (const T*)p[sc.it & 1], p[(sc.it + 1) & 1], q, sc);
+    }
+}
+
+void launch_pcg3_spmv_z_f64(hipStream_t s, const Geo& g, const TileCfg& t, const double* K, const double* D,
+                            const double* z, double* const p[2], double* q, const SolveCtl& sc, const LaunchEv& ev)
+{
+    spmv_z<double>(s, g, t, K, D, z, p, q, sc, ev);
+}
+
+void launch_pcg3_spmv_z_f32(hipStream_t s, const Geo& g, const TileCfg& t, const float* K, const float* D,
+                            const float* z, float* const p[2], float* q, const SolveCtl& sc, const LaunchEv& ev)
+{
+    spmv_z<float>(s, g, t, K, D, z, p, q, sc, ev);
+}
+
+bool line_supported(const Geo& g) { return g.nz >= 1 && g.nz <= kLineMaxNz; }
+
+int line_blocks(const Geo& g) { return std::min((g.nxy + kBlock - 1) / kBlock, kMaxBlocks / 2); }
+
+template <typename T, bool INIT>
+static void line_launch(hipStream_t s, const Geo& g, const T* K, const T* D, T* x, T* r, const T* p, const T* q,
+                        T* z, const SolveCtl& sc, const LaunchEv& ev)
+{
+    const dim3 gr(line_blocks(g)), bl(kBlock);
+    if (g.nz <= 16) launch_ev(k_line<T, 16, INIT>, gr, bl, s, ev, g, K, D, x, r, p, q, z, sc);
+    else if (g.nz <= 32) launch_ev(k_line<T, 32, INIT>, gr, bl, s, ev, g, K, D, x, r, p, q, z, sc);
+    else launch_ev(k_line<T, 64, INIT>, gr, bl, s, ev, g, K, D, x, r, p, q, z, sc);
+}
+
+void launch_line_init_f64(hipStream_t s, const Geo& g, const double* K, const double* D, const double* r, double* z,
+                          const SolveCtl& sc, const LaunchEv& ev)
+{
+    line_launch<double, true>(s, g, K, D, nullptr, const_cast<double*>(r), nullptr, nullptr, z, sc, ev);
+}
+
+void launch_line_init_f32(hipStream_t s, const Geo& g, const float* K, const float* D, const float* r, float* z,
+                          const SolveCtl& sc, const LaunchEv& ev)
+{
+    line_launch<float, true>(s, g, K, D, nullptr, const_cast<float*>(r), nullptr, nullptr, z, sc, ev);
+}
+
+void launch_line_update_f64(hipStream_t s, const Geo& g, const double* K, const double* D, double* x, double* r,
+                            const double* p, const double* q, double* z, const SolveCtl& sc, const LaunchEv& ev)
+{
+    line_launch<double, false>(s, g, K, D, x, r, p, q, z, sc, ev);
+}
+
+void launch_line_update_f32(hipStream_t s, const Geo& g, const float* K, const float* D, float* x, float* r,
+                            const float* p, const float* q, float* z, const SolveCtl& sc, const LaunchEv& ev)
+{
+    line_launch<float, false>(s, g, K, D, x, r, p, q, z, sc, ev);
 }
 
 void launch_pcg3_spmv_tiled_f64(hipStream_t s, const Geo& g, const TileCfg& t, const uint8_t* flags,

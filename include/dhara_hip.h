@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define DHARA_ABI_VERSION 2
+#define DHARA_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------------------- */
 #define DHARA_OK            0
@@ -78,6 +78,15 @@ extern "C" {
 #define DHARA_KRYLOV_ONE_KERNEL 1   /* single-reduction recurrences (Chronopoulos-Gear): one
                                        launch per iteration (default); without Dirichlet rows
                                        only, the two-kernel form is used otherwise          */
+
+/* Preconditioner of the subsurface PCG (the reference ran unpreconditioned BiCGStab,
+ * src/subsurface.cu:750-751; the converged solution is the same, ||r|| <= lin_tol ||b||)    */
+#define DHARA_PRECOND_JACOBI 0      /* diagonal scaling (default)                            */
+#define DHARA_PRECOND_COLUMN 1      /* exact solve of each column's vertical tridiagonal part
+                                       (the stiff dz^-2 couplings): ~4x fewer iterations on
+                                       thin layers; two-kernel recurrences with a column-solve
+                                       kernel; needs nz <= 64 and no Dirichlet rows (Jacobi is
+                                       used otherwise)                                      */
 
 /* State initialisation. */
 #define DHARA_INIT_FROM_THETA 0   /* psi <- vG^-1(theta), then theta,K,C <- vG(psi)
@@ -157,6 +166,7 @@ typedef struct dhara_params {
     int32_t top_bc;                               /* DHARA_TOP_*                          */
     int32_t olf_precond;                          /* DHARA_OLF_PRECOND_* (overland CG)    */
     int32_t krylov;                               /* DHARA_KRYLOV_* (subsurface PCG)      */
+    int32_t precond;                              /* DHARA_PRECOND_* (subsurface PCG)     */
 } dhara_params;
 
 typedef struct dhara_step_stats {
@@ -210,6 +220,7 @@ static inline void dhara_params_defaults(dhara_params* p)
     p->top_bc = DHARA_TOP_SURFACE;
     p->olf_precond = DHARA_OLF_PRECOND_MG;
     p->krylov = DHARA_KRYLOV_ONE_KERNEL;
+    p->precond = DHARA_PRECOND_JACOBI;
 }
 
 /* ---- lifecycle ------------------------------------------------------------------------ */

@@ -16,7 +16,8 @@
  *   CG / BiCGStab / monitor      CUSP testing/cg.cu, bicgstab.cu (poisson5pt(10,10), b=1,
  *                                20 iterations, |r| < 1e-4|b|; zero-residual start = 0
  *                                iterations), testing/monitor.cu (tolerance = atol+rtol|b|).
- *   Jacobi-PCG                   dense LU solve on <= 300 unknowns.
+ *   Jacobi-PCG, column-PCG       dense LU solve on <= 300 unknowns; the column solve
+ *                                against a dense solve of each column's tridiagonal system.
  *   subsurface assembly + step   symmetry of the assembled matrix, row sums, laterally
  *                                uniform 3D run == independent 1D Thomas column run,
  *                                saturated linear steady state (closed form), hydrostatic
@@ -253,6 +254,58 @@ void pcg_jacobi(long n, int nd, const int* off, const double* a, const double* b
     }
 }
 
+// Column (line) preconditioner: z = T^-1 r, T the vertical tridiagonal part of A (diagonals
+// -nxy, 0, +nxy of the 7-point matrix), i.e. one independent tridiagonal system per column of
+// cells, solved by the Thomas algorithm.  T is symmetric and diagonally dominant like A, so the
+// elimination needs no pivoting.  (Not in the reference, which runs unpreconditioned
+// BiCGStab; it is a preconditioner choice of this solver, like Jacobi.)
+void line_solve(long n, long nxy, int nz, const double* a, const double* r, double* z, double* cp)
+{
+    const double *lo = a, *dg = a + 3 * n, *up = a + 6 * n;
+#pragma omp parallel for schedule(static)
+    for (long col = 0; col < nxy; ++col) {
+        double cprev = 0.0, yprev = 0.0;
+        for (int k = 0; k < nz; ++k) {
+            const long row = (long)k * nxy + col;
+            const double l = k > 0 ? lo[row] : 0.0;
+            const double den = dg[row] - l * cprev;
+            cprev = (k + 1 < nz ? up[row] : 0.0) / den;
+            yprev = (r[row] - l * yprev) / den;
+            cp[row] = cprev;
+            z[row] = yprev;
+        }
+        for (int k = nz - 2; k >= 0; --k) {
+            const long row = (long)k * nxy + col;
+            z[row] -= cp[row] * z[row + nxy];
+        }
+    }
+}
+
+// PCG with the column preconditioner: the iteration and stopping rule of pcg_jacobi with
+// z = T^-1 r in place of z = D^-1 r.
+void pcg_line(long n, long nxy, int nz, const int* off, const double* a, const double* b, double* x, Monitor& mon)
+{
+    std::vector<double> r(n), z(n), p(n), q(n), cp(n);
+    dia_spmv(n, 7, off, a, x, q.data());
+    for (long i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    line_solve(n, nxy, nz, a, r.data(), z.data(), cp.data());
+    p = z;
+    double rz = dot(n, r.data(), z.data());
+    while (!mon.finished(n, r.data())) {
+        dia_spmv(n, 7, off, a, p.data(), q.data());
+        const double alpha = rz / dot(n, p.data(), q.data());
+        for (long i = 0; i < n; ++i) x[i] += alpha * p[i];
+        for (long i = 0; i < n; ++i) r[i] -= alpha * q[i];
+        line_solve(n, nxy, nz, a, r.data(), z.data(), cp.data());
+        const double rz_new = dot(n, r.data(), z.data());
+        const double beta = rz_new / rz;
+        rz = rz_new;
+        ++mon.count;
+        if (mon.finished(n, r.data())) break;
+        for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+}
+
 int dense_solve(int n, double* A, const double* b, double* x)
 {
     std::vector<double> rhs(b, b + n);
@@ -442,6 +495,9 @@ int solve_subsurface(dor_ctx& c, int* iters, int* conv)
         break;
     case DOR_LIN_BICGSTAB_REF:
         cusp_bicgstab(N, 7, off, c.a3d.data(), c.rhs3d.data(), c.x.data(), mon);
+        break;
+    case DOR_LIN_PCG_LINE:
+        pcg_line(N, (long)c.g.nx * c.g.ny, c.g.nz, off, c.a3d.data(), c.rhs3d.data(), c.x.data(), mon);
         break;
     case DOR_LIN_DENSE: {
         int rc = dia_dense_solve(N, 7, off, c.a3d.data(), c.rhs3d.data(), c.x.data());
@@ -1096,6 +1152,29 @@ int dor_dia_bicgstab(int32_t nrows, int32_t ndiag, const int32_t* offsets,
     if (nrows < 1 || !offsets || !values || !b || !x) return DHARA_E_ARG;
     Monitor mon(nrm2(nrows, b), maxit, rtol, atol);
     cusp_bicgstab(nrows, ndiag, offsets, values, b, x, mon);
+    if (iters) *iters = (int)mon.count;
+    if (rnorm) *rnorm = mon.r_norm;
+    if (converged) *converged = mon.converged();
+    return DHARA_OK;
+}
+
+int dor_line_solve(int32_t nx, int32_t ny, int32_t nz, const double* values, const double* r, double* z)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || !values || !r || !z) return DHARA_E_ARG;
+    const long n = (long)nx * ny * nz;
+    std::vector<double> cp(n);
+    line_solve(n, (long)nx * ny, nz, values, r, z, cp.data());
+    return DHARA_OK;
+}
+
+int dor_dia_pcg_line(int32_t nx, int32_t ny, int32_t nz, const double* values, const double* b, double* x,
+                     int32_t maxit, double rtol, int32_t* iters, double* rnorm, int32_t* converged)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || !values || !b || !x) return DHARA_E_ARG;
+    const long n = (long)nx * ny * nz, nxy = (long)nx * ny;
+    const int off[7] = {-(int)nxy, -nx, -1, 0, 1, nx, (int)nxy};
+    Monitor mon(nrm2(n, b), maxit, rtol, 0.0);
+    pcg_line(n, nxy, nz, off, values, b, x, mon);
     if (iters) *iters = (int)mon.count;
     if (rnorm) *rnorm = mon.r_norm;
     if (converged) *converged = mon.converged();

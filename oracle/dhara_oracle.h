@@ -26,6 +26,7 @@ extern "C" {
 #define DOR_LIN_BICGSTAB_REF 1  /* CUSP bicgstab, identity preconditioner
                                    (cusp/krylov/detail/bicgstab.inl:64-146)                */
 #define DOR_LIN_DENSE        2  /* dense LU (tiny grids only)                               */
+#define DOR_LIN_PCG_LINE     3  /* PCG with the column (vertical line) preconditioner        */
 /* Overland linear solver of the oracle. */
 #define DOR_OLF_PCG_JACOBI   0  /* Dirichlet ring eliminated, Jacobi-PCG (tight mode)       */
 #define DOR_OLF_CG_REF       1  /* CUSP cg on the full ring-identity DIA matrix
@@ -93,6 +94,11 @@ int dor_dia_bicgstab(int32_t nrows, int32_t ndiag, const int32_t* offsets,
 int dor_dia_pcg_jacobi(int32_t nrows, int32_t ndiag, const int32_t* offsets,
                        const double* values, const double* b, double* x, int32_t maxit,
                        double rtol, int32_t* iters, double* rnorm, int32_t* converged);
+/* Column preconditioner and PCG with it on a 7-point DIA matrix of an nx x ny x nz grid
+ * (offsets -nx*ny, -nx, -1, 0, 1, nx, nx*ny; values 7 x n, pitch n). */
+int dor_line_solve(int32_t nx, int32_t ny, int32_t nz, const double* values, const double* r, double* z);
+int dor_dia_pcg_line(int32_t nx, int32_t ny, int32_t nz, const double* values, const double* b, double* x,
+                     int32_t maxit, double rtol, int32_t* iters, double* rnorm, int32_t* converged);
 /* Dense LU with partial pivoting, A row-major n*n (overwritten), b -> x. */
 int dor_dense_solve(int32_t n, double* A, const double* b, double* x);
 /* Thomas algorithm: a sub-, b main-, c super-diagonal, d rhs, x out (all length n). */

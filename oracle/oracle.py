@@ -15,7 +15,7 @@ from dhara_amd.model import BaseModel, DharaError
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdhara_oracle.so")
 
-LIN_PCG_JACOBI, LIN_BICGSTAB_REF, LIN_DENSE = 0, 1, 2
+LIN_PCG_JACOBI, LIN_BICGSTAB_REF, LIN_DENSE, LIN_PCG_LINE = 0, 1, 2, 3
 OLF_PCG_JACOBI, OLF_CG_REF, OLF_DENSE = 0, 1, 2
 
 _lib = None
@@ -49,6 +49,9 @@ def lib():
                                       C.c_double, ip, dp, ip]
         L.dor_dia_pcg_jacobi.argtypes = [C.c_int32, C.c_int32, ip, dp, dp, dp, C.c_int32,
                                          C.c_double, ip, dp, ip]
+        L.dor_line_solve.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, dp, dp]
+        L.dor_dia_pcg_line.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, dp, dp, C.c_int32, C.c_double,
+                                       ip, dp, ip]
         L.dor_dense_solve.argtypes = [C.c_int32, dp, dp, dp]
         L.dor_thomas.argtypes = [C.c_int32, dp, dp, dp, dp, dp]
         L.dor_column_run.argtypes = [C.c_int32, C.c_double, P(_abi.Params), C.c_double, C.c_double,
@@ -177,6 +180,27 @@ def dia_bicgstab(off, val, b, x0, maxit, rtol, atol=0.0):
 
 def dia_pcg_jacobi(off, val, b, x0, maxit, rtol):
     return _krylov("dor_dia_pcg_jacobi", off, val, b, x0, maxit, rtol)
+
+
+def line_solve(shape, val, r):
+    """z = T^-1 r, T the vertical tridiagonal part of a 7-point DIA matrix (shape nx, ny, nz)."""
+    nx, ny, nz = shape
+    val = np.ascontiguousarray(val, np.float64)
+    r = np.ascontiguousarray(r, np.float64)
+    z = np.empty_like(r)
+    assert lib().dor_line_solve(nx, ny, nz, _dp(val), _dp(r), _dp(z)) == 0
+    return z
+
+
+def dia_pcg_line(shape, val, b, x0, maxit, rtol):
+    nx, ny, nz = shape
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    it, cv = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    rn = np.zeros(1)
+    assert lib().dor_dia_pcg_line(nx, ny, nz, _dp(val), _dp(b), _dp(x), maxit, rtol, _ip(it), _dp(rn), _ip(cv)) == 0
+    return x, int(it[0]), float(rn[0]), bool(cv[0])
 
 
 def dense_solve(A, b):

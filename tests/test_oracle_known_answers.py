@@ -121,3 +121,61 @@ def test_thomas_matches_dense():
     A = np.diag(bdiag) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
     x = oracle.thomas(a, bdiag, c, d)
     assert np.allclose(x, np.linalg.solve(A, d), rtol=1e-12, atol=1e-13)
+
+
+def _random_spd_7pt(nx, ny, nz, seed, vscale=1.0):
+    """SPD 7-point DIA matrix on an nx x ny x nz grid: a Poisson operator with per-row positive
+    scaling, vertical couplings times vscale and a positive diagonal shift."""
+    rng = np.random.default_rng(seed)
+    off, val = oracle.poisson7pt(nx, ny, nz)
+    n = val.shape[1]
+    A = oracle.dia_to_dense(off, val)
+    nxy = nx * ny
+    for r in range(n):
+        for c in (r - nxy, r + nxy):
+            if 0 <= c < n:
+                A[r, c] *= vscale
+    A = A - np.diag(np.diag(A))
+    A += np.diag(-A.sum(axis=1))                 # weakly dominant, like the flow operator
+    w = rng.uniform(0.2, 3.0, n)
+    A = A * np.sqrt(np.outer(w, w))
+    A += np.diag(rng.uniform(0.01, 1.0, n))
+    dense_val = np.zeros_like(val)
+    for d, o in enumerate(off):
+        for r in range(n):
+            c = r + o
+            if 0 <= c < n:
+                dense_val[d, r] = A[r, c]
+    return off, dense_val, A
+
+
+def test_line_solve_matches_dense_column_systems():
+    nx, ny, nz = 4, 3, 7
+    off, val, A = _random_spd_7pt(nx, ny, nz, seed=3)
+    n, nxy = nx * ny * nz, nx * ny
+    T = np.zeros_like(A)                          # the +-nxy diagonals and the main diagonal only
+    for r in range(n):
+        for c in (r - nxy, r, r + nxy):
+            if 0 <= c < n:
+                T[r, c] = A[r, c]
+    r = np.random.default_rng(4).standard_normal(n)
+    z = oracle.line_solve((nx, ny, nz), val, r)
+    np.testing.assert_allclose(z, np.linalg.solve(T, r), rtol=1e-12, atol=1e-12 * np.max(np.abs(z)))
+    # columns are independent: a residual in one column leaves the others at zero
+    r1 = np.zeros(n)
+    r1[np.arange(nz) * nxy + 5] = 1.0
+    z1 = oracle.line_solve((nx, ny, nz), val, r1)
+    assert np.all(z1[np.setdiff1d(np.arange(n), np.arange(nz) * nxy + 5)] == 0.0)
+
+
+def test_pcg_line_matches_dense_lu_and_beats_jacobi_on_anisotropy():
+    nx, ny, nz = 5, 4, 9
+    off, val, A = _random_spd_7pt(nx, ny, nz, seed=5, vscale=400.0)   # dz = dx / 20
+    n = nx * ny * nz
+    b = np.random.default_rng(6).standard_normal(n)
+    x_lu = oracle.dense_solve(A, b)
+    x, it, rn, conv = oracle.dia_pcg_line((nx, ny, nz), val, b, np.zeros(n), 1000, 1e-13)
+    assert conv
+    assert np.max(np.abs(x - x_lu)) < 1e-10 * np.max(np.abs(x_lu))
+    _, it_j, _, conv_j = oracle.dia_pcg_jacobi(off, val, b, np.zeros(n), 1000, 1e-13)
+    assert conv_j and it < it_j / 2, (it, it_j)
