@@ -48,7 +48,9 @@ def parse():
     ap.add_argument("--refine-max", type=int, default=None, help="mixed: FP64 refinement sweeps per Picard iteration")
     ap.add_argument("--inner-tol", type=float, default=None, help="mixed: FP32 inner relative tolerance")
     ap.add_argument("--mixed-lin-tol", type=float, default=None, help="mixed: FP64 residual target")
-    ap.add_argument("--precond", choices=["jacobi", "column"], default="jacobi",
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 Jacobi-PCG reference run (with --precond column)")
+    ap.add_argument("--precond", choices=["jacobi", "column"], default="column",
                     help="subsurface PCG preconditioner (column: exact solve of each column's vertical "
                          "tridiagonal part, two-kernel recurrences with a column-solve kernel)")
     ap.add_argument("--krylov", choices=["one", "two"], default="one",
@@ -239,6 +241,29 @@ def main():
               f"{sum(s['picard_iters'] for s in stats)}, krylov {sum(s['lin_iters'] for s in stats)}",
               file=sys.stderr, flush=True)
 
+    # the plain method (FP64 end to end, Jacobi-PCG, one-kernel form) on the same workload, for
+    # the speed-up of the fast path over it
+    plain = None
+    if column and not args.no_plain and args.only == "both":
+        wj = inputs.c2_coupled(nx=nx, ny=ny, nz=nz)
+        wj.params.krylov = _abi.KRYLOV_ONE_KERNEL
+        wj.params.precond = _abi.PRECOND_JACOBI
+        m = dhara_amd.FlowModel(wj.grid, cp(wj.params))
+        wj.apply(m)
+        wj.run(m, max(W, 1))
+        m.autotune()
+        m.close()
+        m = dhara_amd.FlowModel(wj.grid, cp(wj.params))
+        wj.apply(m)
+        print(f"# plain fp64 (Jacobi): {K} steps", file=sys.stderr, flush=True)
+        st_j, secs_j, _ = run_timed(m, wj, K)
+        m.close()
+        plain = {"cell_steps_per_s": n3 * K / secs_j, "seconds": secs_j,
+                 "picard_iters": sum(s_["picard_iters"] for s_ in st_j),
+                 "krylov_iters": sum(s_["lin_iters"] for s_ in st_j),
+                 "solver": "FP64 Jacobi-PCG, one-kernel form, tuned tile"}
+        print(f"# plain fp64 (Jacobi): {secs_j:.3f} s, {plain['cell_steps_per_s']:.4e} cell-steps/s",
+              file=sys.stderr, flush=True)
     head = "mixed" if "mixed" in runs else "fp64"
     r = runs[head]
     value = n3 * K / r["secs"]
@@ -276,6 +301,10 @@ def main():
         out["head_error_vs_fp64"] = {"max_m": float(np.max(np.abs(d))),
                                      "rms_m": float(np.sqrt(np.mean(d * d))),
                                      "ponding_max_m": float(np.max(np.abs(runs["mixed"]["h"] - runs["fp64"]["h"])))}
+    if plain is not None:
+        out["plain_fp64_jacobi"] = plain
+        if "mixed" in runs:
+            out["mixed_over_plain_fp64"] = plain["seconds"] / runs["mixed"]["secs"]
     # roofline of the dominant kernel, from the library's HIP-event timing
     pr = r["prof"]
     kern = []

@@ -1327,6 +1327,8 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
     const int shapes[5][2] = {{32, 8}, {64, 4}, {16, 16}, {32, 4}, {64, 8}};
     std::vector<int> kcs = {c->g.nz, (c->g.nz + 1) / 2, (c->g.nz + 2) / 3, (c->g.nz + 3) / 4, 8, 4};
     const uint8_t* fl = has_dirichlet(c) ? c->flags : nullptr;
+    // the column-preconditioned PCG runs the SpMV kernel that reads z (= r here)
+    const bool zin = c->p.precond == DHARA_PRECOND_COLUMN && !fl && dh::line_supported(c->g);
     double best[2] = {0.0, 0.0};
     for (int e = 0; e < 2; ++e) {
         if (e == 1 && !mixed) continue;
@@ -1346,7 +1348,8 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
                 r2 = time_spmv<double>(c, [&](int it) {
                     dh::SolveCtl k = sc;
                     k.it = it;
-                    if (t.tx) dh::launch_pcg3_spmv_tiled_f64(c->s, c->g, t, fl, c->K, c->D, c->r, pp, c->q, k);
+                    if (zin) dh::launch_pcg3_spmv_z_f64(c->s, c->g, t, c->K, c->D, c->r, pp, c->q, k);
+                    else if (t.tx) dh::launch_pcg3_spmv_tiled_f64(c->s, c->g, t, fl, c->K, c->D, c->r, pp, c->q, k);
                     else dh::launch_pcg3_spmv_f64(c->s, c->g, fl, c->K, c->D, c->r, pp, c->q, k);
                 }, &us);
             } else {
@@ -1354,27 +1357,30 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
                 r2 = time_spmv<float>(c, [&](int it) {
                     dh::SolveCtl k = sc;
                     k.it = it;
-                    if (t.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, t, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
+                    if (zin) dh::launch_pcg3_spmv_z_f32(c->s, c->g, t, c->K32, c->D32, c->r32, pp, c->q32, k);
+                    else if (t.tx) dh::launch_pcg3_spmv_tiled_f32(c->s, c->g, t, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
                     else dh::launch_pcg3_spmv_f32(c->s, c->g, fl, c->K32, c->D32, c->r32, pp, c->q32, k);
                 }, &us);
             }
             if (r2) return r2;
-            if (us < best_t) { best_t = us; best_cfg = t; }
+            // keep the earlier candidate unless another is 3 % faster, so the choice (and the
+            // traffic measured for it) is the same from run to run
+            if (us < 0.97f * best_t || best_t >= 1e29f) { best_t = us; best_cfg = t; }
             return DHARA_OK;
         };
-        if ((rc = consider(dh::TileCfg{0, 0, 0}))) return rc;
         for (const auto& sh : shapes)
             for (int kc : kcs) {
                 dh::TileCfg t{sh[0], sh[1], std::max(1, std::min(kc, c->g.nz))};
                 if (dh::tile_blocks(c->g, t) > dh::kMaxBlocks) continue;
                 if ((rc = consider(t))) return rc;
             }
+        if ((rc = consider(dh::TileCfg{0, 0, 0}))) return rc;
         c->tile[e] = best_cfg;
         best[e] = best_t;
         remember_tile(TuneKey{c->g.nx, c->g.ny, c->g.nz, e == 0 ? 8 : 4, 0}, best_cfg);
     }
     // one-kernel PCG: tile shape x planes per chunk x grid cap, on scratch output buffers
-    if (c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !fl) {
+    if (c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !fl && !zin) {
         HIPCHK(c, hipMemcpyAsync(st, &tune, sizeof(tune), hipMemcpyHostToDevice, c->s));
         const size_t N = c->N;
         void *xs = nullptr, *rs = nullptr, *ps = nullptr;

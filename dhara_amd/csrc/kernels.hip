@@ -555,7 +555,9 @@ __device__ __forceinline__ double jac_inv(double d)
 // (r, z) is accumulated as sum_k w_k y_k with w_k = r_k - a_k y_k-1 = den_k y_k (T = L diag(den)
 // L^T): the same value as sum_k r_k z_k, as a sum of positive terms.  The reciprocal of the
 // pivot is jac_inv, a fixed function of the operator, so every application is the same
-// linear map.
+// linear map.  In FP64 the upper factors c'_k wait in z itself (written in the forward sweep,
+// overwritten by the back substitution) instead of registers: 2 x 32 doubles per thread would
+// leave 2 waves per SIMD, too few for the 2.5 waves of columns of a 400 x 400 surface.
 template <typename T, int NZM, bool INIT>
 __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K, const T* __restrict__ D,
                                                  T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
@@ -576,8 +578,9 @@ __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K,
     const T hz = (T)(0.5 * g.wz);
     const int nz = g.nz, nxy = g.nxy;
     double acc[INIT ? 4 : 2] = {};
+    constexpr bool kCpInZ = sizeof(T) == 8;
     for (int col = blockIdx.x * kBlock + threadIdx.x; col < nxy; col += gridDim.x * kBlock) {
-        T y[NZM], cp[NZM];
+        T y[NZM], cp[kCpInZ ? 1 : NZM];
         T kprev = T(0), yprev = T(0), inv_prev = T(0);
 #pragma unroll
         for (int k = 0; k < NZM; ++k) {
@@ -594,7 +597,8 @@ __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K,
                 if (k > 0) {
                     const T a = -hz * (kc + kprev);   // T[k][k-1] = T[k-1][k]
                     const T cpk = a * inv_prev;       // c'_{k-1}
-                    cp[k - 1] = cpk;
+                    if constexpr (kCpInZ) z[c - nxy] = cpk;
+                    else cp[k - 1] = cpk;
                     den -= a * cpk;
                     w -= a * yprev;
                 }
@@ -612,8 +616,13 @@ __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K,
 #pragma unroll
         for (int k = NZM - 1; k >= 0; --k) {
             if (k < nz) {
-                const T zk = (k == nz - 1) ? y[k] : y[k] - cp[k] * zn;
-                z[k * nxy + col] = zk;
+                const int c = k * nxy + col;
+                T zk = y[k];
+                if (k < nz - 1) {
+                    if constexpr (kCpInZ) zk -= z[c] * zn;
+                    else zk -= cp[k] * zn;
+                }
+                z[c] = zk;
                 zn = zk;
             }
         }

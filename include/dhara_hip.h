@@ -291,10 +291,11 @@ int dhara_set_krylov_tile(dhara_ctx* ctx, int32_t elem_bytes, int32_t tx, int32_
 int dhara_get_krylov_tile(const dhara_ctx* ctx, int32_t elem_bytes, int32_t* tx, int32_t* ty, int32_t* kc,
                           int32_t* grid);
 /* Times the candidate tile shapes on the context's current operator (HIP events, a few
- * launches each) for the precisions the context uses -- of the SpMV kernel and, with
- * DHARA_KRYLOV_ONE_KERNEL, of the one-kernel PCG -- keeps the fastest, and records it in a
- * process-wide table keyed by (nx, ny, nz, elem_bytes) that later contexts of the same shape
- * reuse without timing.  best_us (may be NULL) receives the winner's time per launch for
+ * launches each) for the precisions the context uses -- of the SpMV kernel (its z-input form
+ * with DHARA_PRECOND_COLUMN) and, with DHARA_KRYLOV_ONE_KERNEL and Jacobi, of the one-kernel
+ * PCG -- keeps the fastest (the earlier candidate unless another is 3 % faster), and records it
+ * in a process-wide table keyed by (nx, ny, nz, elem_bytes) that later contexts of the same
+ * shape reuse without timing.  best_us (may be NULL) receives the winner's time per launch for
  * [FP64, FP32]. */
 int dhara_autotune(dhara_ctx* ctx, double* best_us);
 
@@ -377,7 +378,8 @@ int dhara_adjoint_jvp(dhara_ctx* ctx, const double* psi_new, int32_t transpose, 
  * when their own stopping rule is met, and batches of Krylov iterations are replayed from a
  * captured hipGraph.  All members must have the same precision and flux faces only.  Mixed
  * precision is FP32 Jacobi-PCG with FP64 iterative refinement (the FP32 recurrence restarts
- * after each refinement); the overland solve is Jacobi-PCG.  The members stay usable through
+ * after each refinement); the subsurface solves are Jacobi-preconditioned whatever the
+ * members' precond; the overland solve is Jacobi-PCG.  The members stay usable through
  * the other entry points between ensemble steps and must outlive the ensemble. */
 typedef struct dhara_ensemble dhara_ensemble;
 int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble** out);
