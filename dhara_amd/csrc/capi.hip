@@ -403,7 +403,7 @@ int solve_mixed(dhara_ctx* c, dhara_step_stats* st)
 // p[it&1], x in place.
 int solve_fp64_cg1(dhara_ctx* c, dhara_step_stats* st)
 {
-    dh::launch_assemble(c->s, asm_args(c, false), false);
+    const int aparts = dh::launch_assemble_tiled(c->s, asm_args(c, false), false, c->tile[0]);
     LAUNCHCHK(c);
     double* const rb[2] = {c->r, c->q};
     double* const pb[2] = {c->pv[0], c->pv[1]};
@@ -412,6 +412,7 @@ int solve_fp64_cg1(dhara_ctx* c, dhara_step_stats* st)
     dh::SolveCtl sc = solve_ctl(c, c->st_sub, c->p.lin_maxiter, dh::grid_for(c->N), false, true);
     {
         dh::SolveCtl k = sc;   // reads the assembly partials in part[0]
+        k.nparts = aparts;
         k.part_out = c->part[1];
         dh::launch_cg1_init_f64(c->s, c->g, c->tile[0], c->K, c->D, rb[0], pb[1], k);
     }
@@ -442,7 +443,7 @@ int solve_fp64_line(dhara_ctx* c, dhara_step_stats* st)
 {
     int rc;
     if ((rc = alloc_line(c, false))) return rc;
-    dh::launch_assemble(c->s, asm_args(c, false), false);
+    const int aparts = dh::launch_assemble_tiled(c->s, asm_args(c, false), false, c->tile[0]);
     LAUNCHCHK(c);
     double* const pp[2] = {c->pv[0], c->pv[1]};
     dh::SolveCtl sc = solve_ctl(c, c->st_sub, c->p.lin_maxiter, dh::grid_for(c->N), false, true);
@@ -451,6 +452,7 @@ int solve_fp64_line(dhara_ctx* c, dhara_step_stats* st)
     const int lparts = dh::line_blocks(c->g);
     {
         dh::SolveCtl k = sc;   // assembly partials in part[0]
+        k.nparts = aparts;
         k.part_out = c->part_line;
         dh::launch_line_init_f64(c->s, c->g, c->K, c->D, c->r, c->zl, k);
     }
@@ -486,7 +488,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
 {
     int rc;
     if ((rc = alloc_line(c, true))) return rc;
-    dh::launch_assemble(c->s, asm_args(c, false), true);
+    const int aparts = dh::launch_assemble_tiled(c->s, asm_args(c, false), true, c->tile[0]);
     LAUNCHCHK(c);
     float* const pp[2] = {c->p32[0], c->p32[1]};
     dh::RefineArgs ra{};
@@ -506,7 +508,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
     const int spmv_parts = tc.tx ? dh::tile_blocks(c->g, tc) : dh::grid_for(c->N);
     const int lparts = dh::line_blocks(c->g);
     dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, true);
-    int it = 0, updates = 0, refine_parts = dh::grid_for(c->N);
+    int it = 0, updates = 0, refine_parts = aparts;
     bool outer_conv = false;
     auto start = [&](int at) {
         dh::SolveCtl k = sc;   // fresh residual partials in part[0]
@@ -576,7 +578,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
 // also makes the outer convergence test.
 int solve_mixed_cg1(dhara_ctx* c, dhara_step_stats* st)
 {
-    dh::launch_assemble(c->s, asm_args(c, false), true);
+    const int aparts = dh::launch_assemble_tiled(c->s, asm_args(c, false), true, c->tile[0]);
     LAUNCHCHK(c);
     float* const rb[2] = {c->r32, c->q32};
     float* const pb[2] = {c->p32[0], c->p32[1]};
@@ -596,14 +598,14 @@ int solve_mixed_cg1(dhara_ctx* c, dhara_step_stats* st)
     const int nb = dh::cg1_blocks(c->g, tc);
     const int init_parts = dh::cg1_init_blocks(c->g, c->tile[1]);
     dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, true);
-    int it = 0, updates = 0, refine_parts = dh::grid_for(c->N);
+    int it = 0, updates = 0, refine_parts = aparts;
     bool outer_conv = false;
     for (;;) {
         {
             dh::SolveCtl k = sc;   // fresh residual partials in part[0]
             k.it = it;
             k.segment_start = it > 0 ? 1 : 0;
-            k.nparts = it > 0 ? refine_parts : dh::grid_for(c->N);
+            k.nparts = refine_parts;
             k.part_out = c->part[1];
             dh::launch_cg1_init_f32(c->s, c->g, c->tile[1], c->K32, c->D32, rb[it & 1], pb[(it + 1) & 1], k);
         }

@@ -163,6 +163,10 @@ void launch_group_scatter(hipStream_t s, int nxy, const int* procmap, const doub
 void launch_surface_pre(hipStream_t s, const SurfArgs& a);
 void launch_surface_post(hipStream_t s, const SurfArgs& a);
 void launch_assemble(hipStream_t s, const AsmArgs& a, bool mixed);
+// the same on an SpMV tile (LDS-staged K and x; untiled with Dirichlet rows or tx = 0);
+// returns the number of partials written
+struct TileCfg;
+int launch_assemble_tiled(hipStream_t s, const AsmArgs& a, bool mixed, const TileCfg& t);
 void launch_refine(hipStream_t s, const RefineArgs& a);
 // the same on an SpMV tile (LDS-tiled FP64 residual; untiled with Dirichlet rows or tx = 0);
 // returns the number of partials written

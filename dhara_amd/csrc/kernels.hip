@@ -208,6 +208,16 @@ __global__ __launch_bounds__(kBlock) void k_surface_post(SurfArgs a)
     b_surface_post(a, hw_blk());
 }
 
+constexpr int kXcd = 8;
+
+__device__ __forceinline__ int xcd_tile(int w, int nb)
+{
+    // workgroup w runs on XCD w % 8; give XCD x the tiles [x*q + min(x, rem), ...)
+    const int x = w % kXcd, slot = w / kXcd;
+    const int q = nb / kXcd, rem = nb % kXcd;
+    return x * q + (x < rem ? x : rem) + slot;
+}
+
 // ---------------------------------------------------------------------------------------
 // Matrix-free assembly: diagonal D and right-hand side b of the mixed-form modified Picard
 // system (src/subsurface.cu:381-429 with 26-353), fused with the initial residual of the
@@ -331,6 +341,122 @@ template <bool DIR, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
 {
     b_assemble<DIR, MIXED>(a, hw_blk());
+}
+
+// The assembly of b_assemble (flux faces only) on an SpMV tile: a TX x TY block of columns
+// marches through kc planes with the K and x planes staged in LDS (one-cell halo, double
+// buffered, one barrier per plane) and the vertical neighbours in registers, so every array
+// is read once per launch instead of through the caches for each of the six neighbours.  The
+// arithmetic of a cell is that of b_assemble, term for term, so D, b and r are the same.
+template <bool MIXED, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_assemble_tiled(AsmArgs a, int kc, int ntx, int nty, int nkz)
+{
+    const Geo g = a.g;
+    if (lin_bid() == 0 && lin_tid() == 0) a.st->done = 0;   // a new solve starts
+    __shared__ double sx[2][TY + 2][TX + 2];
+    __shared__ double sk[2][TY + 2][TX + 2];
+    const int nb = ntx * nty * nkz;
+    const int t = xcd_tile(blockIdx.x, nb);
+    const int bz = t % nkz, bxy = t / nkz;
+    const int bx = bxy % ntx, by = bxy / ntx;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * TX + tx, j = by * TY + ty;
+    const int k0 = bz * kc, k1 = min(k0 + kc, g.nz);
+    const bool in = (i < g.nx) && (j < g.ny);
+    const int col = j * g.nx + i;
+    const int lid = ty * TX + tx;
+    int hx = -2, hy = -2;
+    if (lid < TX) { hx = lid; hy = -1; }
+    else if (lid < 2 * TX) { hx = lid - TX; hy = TY; }
+    else if (lid < 2 * TX + TY) { hx = -1; hy = lid - 2 * TX; }
+    else if (lid < 2 * TX + 2 * TY) { hx = TX; hy = lid - 2 * TX - TY; }
+    const int hi = bx * TX + hx, hj = by * TY + hy;
+    const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
+    const int hcol = hj * g.nx + hi;
+    const double* __restrict__ x = a.x;
+    const double* __restrict__ K = a.K;
+    const double dx = g.dx, dy = g.dy, dz = g.dz, dt = a.dt;
+    const double dx2inv = -1.0 / (dx * dx), dy2inv = -1.0 / (dy * dy), dz2inv = -1.0 / (dz * dz);
+
+    double xm = 0.0, km = 0.0, xc = 0.0, kcur = 0.0;
+    if (in) {
+        if (k0 > 0) { xm = x[(k0 - 1) * g.nxy + col]; km = K[(k0 - 1) * g.nxy + col]; }
+        if (k0 < k1) { xc = x[k0 * g.nxy + col]; kcur = K[k0 * g.nxy + col]; }
+    }
+    double tr = 0.0;
+    int rgrp = 0;
+    if (in && a.has_transp) {
+        tr = a.transp[col];
+        rgrp = a.procmap[col];
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz, rr (solver precision), bb, rr (FP64)
+    for (int k = k0; k < k1; ++k) {
+        double xn = 0.0, kn = 0.0;
+        if (in && k + 1 < g.nz) { xn = x[(k + 1) * g.nxy + col]; kn = K[(k + 1) * g.nxy + col]; }
+        const int buf = k & 1;
+        sx[buf][ty + 1][tx + 1] = xc;
+        sk[buf][ty + 1][tx + 1] = kcur;
+        if (has_halo) {
+            sx[buf][hy + 1][hx + 1] = x[k * g.nxy + hcol];
+            sk[buf][hy + 1][hx + 1] = K[k * g.nxy + hcol];
+        }
+        __syncthreads();
+        if (in) {
+            const int c = k * g.nxy + col;
+            const double Kc = kcur;
+            const double kd = (k == 0) ? 0.0 : 0.5 * (Kc + km);
+            const double ku = (k == g.nz - 1) ? 0.0 : 0.5 * (Kc + kn);
+            const double ks = (j == 0) ? 0.0 : 0.5 * (Kc + sk[buf][ty][tx + 1]);
+            const double kno = (j == g.ny - 1) ? 0.0 : 0.5 * (Kc + sk[buf][ty + 2][tx + 1]);
+            const double kw = (i == 0) ? 0.0 : 0.5 * (Kc + sk[buf][ty + 1][tx]);
+            const double ke = (i == g.nx - 1) ? 0.0 : 0.5 * (Kc + sk[buf][ty + 1][tx + 2]);
+            const double thm = a.theta_m[c];
+            const double Cc = a.C[c];
+            const double D = a.Ss / dt * thm / a.poros + Cc / dt - (ke + kw) * dx2inv - (kno + ks) * dy2inv -
+                             (ku + kd) * dz2inv;
+            double sink = 0.0;
+            if (tr > 0.0) sink = -tr * a.root[rgrp * g.nz + k] / dz;
+            double b = a.Ss / dt * thm / a.poros * a.psi_n[c] + Cc / dt * a.psi_m[c] - (ku - kd) / dz +
+                       (a.theta_n[c] - thm) / dt + sink;
+            const int bi = k * g.ny + j, bj = k * g.nx + i;
+            if (k == 0) b += (-a.bcq[FT][col] * dz) * dz2inv;
+            if (k == g.nz - 1) {
+                a.bcq[FB][col] = Kc;
+                b += (Kc * dz) * dz2inv;
+            }
+            if (j == 0) b += (-a.bcq[FS][bj] * dy) * dy2inv;
+            if (j == g.ny - 1) b += (a.bcq[FN][bj] * dy) * dy2inv;
+            if (i == 0) b += (-a.bcq[FW][bi] * dx) * dx2inv;
+            if (i == g.nx - 1) b += (a.bcq[FE][bi] * dx) * dx2inv;
+            a.D[c] = D;
+            a.b[c] = b;
+            double Ax = D * xc;
+            if (k > 0) Ax += dz2inv * kd * xm;
+            if (j > 0) Ax += dy2inv * ks * sx[buf][ty][tx + 1];
+            if (i > 0) Ax += dx2inv * kw * sx[buf][ty + 1][tx];
+            if (i < g.nx - 1) Ax += dx2inv * ke * sx[buf][ty + 1][tx + 2];
+            if (j < g.ny - 1) Ax += dy2inv * kno * sx[buf][ty + 2][tx + 1];
+            if (k < g.nz - 1) Ax += dz2inv * ku * xn;
+            const double r = b - Ax;
+            acc[2] += b * b;
+            acc[3] += r * r;
+            if (MIXED) {
+                const float r32 = (float)r, D32 = (float)D;
+                a.D32[c] = D32;
+                a.r32[c] = r32;
+                a.d32[c] = 0.0f;
+                const float z32 = r32 / D32;
+                acc[0] += (double)r32 * (double)z32;
+                acc[1] += (double)r32 * (double)r32;
+            } else {
+                a.r[c] = r;
+                acc[0] += r * (r / D);
+                acc[1] += r * r;
+            }
+        }
+        xm = xc; km = kcur; xc = xn; kcur = kn;
+    }
+    block_partials<4, false>(acc, a.part_out);
 }
 
 // FP64 iterative refinement: x <- x + d (pointwise), then r = b - A x in FP64 with the
@@ -660,15 +786,6 @@ __device__ __forceinline__ void st_off(T* base, unsigned byte_off, T v)
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
-constexpr int kXcd = 8;
-
-__device__ __forceinline__ int xcd_tile(int w, int nb)
-{
-    // workgroup w runs on XCD w % 8; give XCD x the tiles [x*q + min(x, rem), ...)
-    const int x = w % kXcd, slot = w / kXcd;
-    const int q = nb / kXcd, rem = nb % kXcd;
-    return x * q + (x < rem ? x : rem) + slot;
-}
 
 // INIT: start-up of a one-kernel PCG segment instead -- u = M^-1 r, A u, no stores, partials
 // [(r, u), (r, r), (u, A u), (p_old, A u), 0] (p_old: the kept direction, it > 0).
@@ -1688,6 +1805,35 @@ void launch_assemble(hipStream_t s, const AsmArgs& a, bool mixed)
         if (mixed) hipLaunchKernelGGL((k_assemble<false, true>), gr, bl, 0, s, a);
         else hipLaunchKernelGGL((k_assemble<false, false>), gr, bl, 0, s, a);
     }
+}
+
+template <bool MIXED, int TX, int TY>
+static void assemble_tiled(hipStream_t s, const AsmArgs& a, int kc)
+{
+    const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY, nkz = (a.g.nz + kc - 1) / kc;
+    hipLaunchKernelGGL((k_assemble_tiled<MIXED, TX, TY>), dim3(ntx * nty * nkz), dim3(TX, TY), 0, s, a, kc, ntx,
+                       nty, nkz);
+}
+
+template <bool MIXED>
+static void assemble_tiled_dispatch(hipStream_t s, const AsmArgs& a, const TileCfg& t)
+{
+    if (t.tx == 32 && t.ty == 8) assemble_tiled<MIXED, 32, 8>(s, a, t.kc);
+    else if (t.tx == 64 && t.ty == 4) assemble_tiled<MIXED, 64, 4>(s, a, t.kc);
+    else if (t.tx == 16 && t.ty == 16) assemble_tiled<MIXED, 16, 16>(s, a, t.kc);
+    else if (t.tx == 32 && t.ty == 4) assemble_tiled<MIXED, 32, 4>(s, a, t.kc);
+    else assemble_tiled<MIXED, 64, 8>(s, a, t.kc);
+}
+
+int launch_assemble_tiled(hipStream_t s, const AsmArgs& a, bool mixed, const TileCfg& t)
+{
+    if (a.flags || !t.tx) {
+        launch_assemble(s, a, mixed);
+        return grid_for(a.g.n);
+    }
+    if (mixed) assemble_tiled_dispatch<true>(s, a, t);
+    else assemble_tiled_dispatch<false>(s, a, t);
+    return tile_blocks(a.g, t);
 }
 
 void launch_refine(hipStream_t s, const RefineArgs& a)
