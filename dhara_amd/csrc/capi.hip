@@ -55,6 +55,7 @@ dh::TileCfg default_tile(const dh::Geo& g)
 {
     dh::TileCfg t{32, 8, 1};
     const int ntiles = ((g.nx + 31) / 32) * ((g.ny + 7) / 8);
+    if (ntiles > dh::kMaxBlocks) return dh::TileCfg{0, 0, 0};   // untiled (grid-stride) kernels
     int nkz = std::max(1, std::min(g.nz, (2048 + ntiles - 1) / ntiles));
     t.kc = (g.nz + nkz - 1) / nkz;
     while (dh::tile_blocks(g, t) > dh::kMaxBlocks) ++t.kc;
@@ -508,17 +509,17 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
     const int spmv_parts = tc.tx ? dh::tile_blocks(c->g, tc) : dh::grid_for(c->N);
     const int lparts = dh::line_blocks(c->g);
     dh::SolveCtl sc = solve_ctl(c, c->st_sub, maxit, dh::grid_for(c->N), true, true);
-    int it = 0, updates = 0, refine_parts = aparts;
+    int it = 0, updates = 0;
     bool outer_conv = false;
-    auto start = [&](int at) {
-        dh::SolveCtl k = sc;   // fresh residual partials in part[0]
-        k.it = at;
-        k.nparts = refine_parts;
+    {
+        dh::SolveCtl k = sc;   // the assembly's partials in part[0]
+        k.nparts = aparts;
         k.part_out = c->part_line;
         dh::launch_line_init_f32(c->s, c->g, c->K32, c->D32, c->r32, c->zl32, k);
-    };
+    }
+    int seg_parts = lparts;   // partials of the segment start-up in part_line
+    ra.part_out = c->part_line;
     for (;;) {
-        start(it);
         const int seg = it;
         rc = run_krylov(
             c, c->st_sub, maxit, c->pcg_hint, c->prof, 4.0,
@@ -527,7 +528,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
                 k.it = j;
                 k.segment_start = (j == seg) ? 1 : 0;
                 k.part_in = j == seg ? c->part_line : c->part[0];
-                k.nparts = lparts;
+                k.nparts = j == seg ? seg_parts : lparts;
                 dh::launch_pcg3_spmv_z_f32(c->s, c->g, tc, c->K32, c->D32, c->zl32, pp, c->q32, k, ev);
             },
             [&](int j, const dh::LaunchEv& ev) {
@@ -545,19 +546,28 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
         if (c->h_st->nan) { st->nan_seen = 1; break; }
         if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
         const int it_end = c->h_st->iter;
-        refine_parts = dh::launch_refine_tiled(c->s, ra, c->tile[0]);
+        // x += d, FP64 residual, and the next segment's z = T^-1 r (partials to part_line)
+        seg_parts = dh::launch_refine_line(c->s, ra, c->zl32);
+        if (!seg_parts) {   // more column tiles than partial slots: residual, then the start-up
+            ra.part_out = c->part[0];
+            dh::SolveCtl k = sc;
+            k.nparts = dh::launch_refine_tiled(c->s, ra, c->tile[0]);
+            k.part_out = c->part_line;
+            dh::launch_line_init_f32(c->s, c->g, c->K32, c->D32, c->r32, c->zl32, k);
+            ra.part_out = c->part_line;
+            seg_parts = lparts;
+        }
         LAUNCHCHK(c);
         ++updates;
         if (it_end >= maxit || updates > c->p.refine_maxiter) {
             // final convergence check only: a segment start at the limit ends the solve
-            start(it_end);
             dh::SolveCtl k = sc;
             k.it = it_end;
             k.maxit = it_end;
             k.segment_start = 1;
             k.first_sweep = 0;
             k.part_in = c->part_line;
-            k.nparts = lparts;
+            k.nparts = seg_parts;
             dh::launch_pcg3_spmv_z_f32(c->s, c->g, tc, c->K32, c->D32, c->zl32, pp, c->q32, k);
             LAUNCHCHK(c);
             if ((rc = read_state(c, c->st_sub))) return rc;
@@ -950,7 +960,7 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
     for (int e = 0; e < 2; ++e) {
         c->tile[e] = default_tile(g);
         c->cg1_tile[e] = dh::TileCfg{32, 8, g.nz, 0};
-        while (dh::cg1_blocks(g, c->cg1_tile[e]) > dh::kMaxBlocks) c->cg1_tile[e].kc *= 2;
+        if (dh::cg1_blocks(g, c->cg1_tile[e]) > dh::kMaxBlocks) c->cg1_tile[e].grid = dh::kMaxBlocks;   // blocks loop
         for (const TuneEntry& te : g_tune_table) {
             if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4, 0}) c->tile[e] = te.cfg;
             if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4, 1}) c->cg1_tile[e] = te.cfg;

@@ -172,6 +172,11 @@ void launch_refine(hipStream_t s, const RefineArgs& a);
 // returns the number of partials written
 struct TileCfg;
 int launch_refine_tiled(hipStream_t s, const RefineArgs& a, const TileCfg& t);
+// reliable update of the mixed column-preconditioned PCG (flux faces, nz <= 64): x += d, then
+// r32 = b - A x, d = 0 and z32 = T32^-1 r32 in one launch; partials [rz, rr, 0, rr64] to
+// a.part_out; returns their number, or 0 without launching anything when the surface has more
+// column tiles than partial slots
+int launch_refine_line(hipStream_t s, const RefineArgs& a, float* z32);
 void launch_end_step(hipStream_t s, const EndArgs& a);
 
 // Jacobi-PCG iteration `sc.it` on the 3D operator, split in its two kernels:

@@ -764,6 +764,118 @@ __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K,
     }
 }
 
+// Reliable update of the mixed column-preconditioned PCG after k_refine_add (x += d): the true
+// residual r = b - A x in FP64, r32 = (float) r, d = 0, and the start of the next FP32 segment,
+// z32 = T32^-1 r32 with T32 the column operator of the FP32 solve ((float) K, (float) D, the
+// arithmetic of k_line) -- k_refine_residual_tiled and the k_line start-up in one launch.  (x
+// += d stays a launch of its own: a block reads x on its halo, which another block would be
+// updating.)  A block owns TX x TY whole columns (the column solve needs them); x and K of each
+// plane are staged in LDS with a one-cell halo, the vertical neighbours stay in registers.
+// Partials [(r32, z32), (r32, r32), 0, (r, r)] per block, the layout of a segment start.
+template <int NZM, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __restrict__ z32, int ntx, int nty)
+{
+    const Geo g = a.g;
+    if (lin_bid() == 0 && lin_tid() == 0) a.st->done = 0;   // the next inner solve starts
+    __shared__ double sx[2][TY + 2][TX + 2];
+    __shared__ double sk[2][TY + 2][TX + 2];
+    const int t = xcd_tile(blockIdx.x, ntx * nty);
+    const int bx = t % ntx, by = t / ntx;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * TX + tx, j = by * TY + ty;
+    const bool in = (i < g.nx) && (j < g.ny);
+    const int col = j * g.nx + i;
+    const int lid = ty * TX + tx;
+    int hx = -2, hy = -2;
+    if (lid < TX) { hx = lid; hy = -1; }
+    else if (lid < 2 * TX) { hx = lid - TX; hy = TY; }
+    else if (lid < 2 * TX + TY) { hx = -1; hy = lid - 2 * TX; }
+    else if (lid < 2 * TX + 2 * TY) { hx = TX; hy = lid - 2 * TX - TY; }
+    const int hi = bx * TX + hx, hj = by * TY + hy;
+    const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
+    const int hcol = hj * g.nx + hi;
+    const double* __restrict__ x = a.x;
+    const double* __restrict__ K = a.K;
+    const double wx = g.wx, wy = g.wy, wz = g.wz, half = 0.5;
+    const float hz = (float)(0.5 * g.wz);
+    const int nz = g.nz, nxy = g.nxy;
+
+    auto xnew = [&](int c) { return x[c]; };
+    double xm = 0.0, km = 0.0, xc = 0.0, kcur = 0.0;
+    if (in) {
+        xc = xnew(col);
+        kcur = K[col];
+    }
+    float y[NZM], cp[NZM];
+    float kprev = 0.0f, yprev = 0.0f, inv_prev = 0.0f;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NZM; ++k) {
+        if (k < nz) {
+            double xn = 0.0, kn = 0.0;
+            if (in && k + 1 < nz) {
+                xn = xnew((k + 1) * nxy + col);
+                kn = K[(k + 1) * nxy + col];
+            }
+            const int buf = k & 1;
+            sx[buf][ty + 1][tx + 1] = xc;
+            sk[buf][ty + 1][tx + 1] = kcur;
+            if (has_halo) {
+                sx[buf][hy + 1][hx + 1] = xnew(k * nxy + hcol);
+                sk[buf][hy + 1][hx + 1] = K[k * nxy + hcol];
+            }
+            __syncthreads();
+            if (in) {
+                const int c = k * nxy + col;
+                const double Dc = a.D[c];
+                double ax = Dc * xc;
+                if (k > 0) ax -= wz * (half * (kcur + km)) * xm;
+                if (j > 0) ax -= wy * (half * (kcur + sk[buf][ty][tx + 1])) * sx[buf][ty][tx + 1];
+                if (i > 0) ax -= wx * (half * (kcur + sk[buf][ty + 1][tx])) * sx[buf][ty + 1][tx];
+                if (i < g.nx - 1) ax -= wx * (half * (kcur + sk[buf][ty + 1][tx + 2])) * sx[buf][ty + 1][tx + 2];
+                if (j < g.ny - 1) ax -= wy * (half * (kcur + sk[buf][ty + 2][tx + 1])) * sx[buf][ty + 2][tx + 1];
+                if (k < nz - 1) ax -= wz * (half * (kcur + kn)) * xn;
+                const double r = a.b[c] - ax;
+                const float r32 = (float)r;
+                a.r32[c] = r32;
+                a.d32[c] = 0.0f;
+                acc[3] += r * r;
+                // forward elimination of the FP32 column system (k_line's arithmetic)
+                const float k32 = (float)kcur;
+                float den = (float)Dc, w = r32;
+                if (k > 0) {
+                    const float av = -hz * (k32 + kprev);
+                    const float cpk = av * inv_prev;
+                    cp[k - 1] = cpk;
+                    den -= av * cpk;
+                    w -= av * yprev;
+                }
+                const float inv = jac_inv(den);
+                const float yk = w * inv;
+                y[k] = yk;
+                acc[0] += (double)w * (double)yk;
+                acc[1] += (double)r32 * (double)r32;
+                kprev = k32;
+                yprev = yk;
+                inv_prev = inv;
+            }
+            xm = xc; km = kcur; xc = xn; kcur = kn;
+        }
+    }
+    if (in) {
+        float zn = 0.0f;
+#pragma unroll
+        for (int k = NZM - 1; k >= 0; --k) {
+            if (k < nz) {
+                const float zk = (k == nz - 1) ? y[k] : y[k] - cp[k] * zn;
+                z32[k * nxy + col] = zk;
+                zn = zk;
+            }
+        }
+    }
+    block_partials<4, false>(acc, a.part_out);
+}
+
 // LDS-tiled 2.5D SpMV.  Block = TX x TY threads owning TX x TY columns; the block marches
 // through planes k0..k0+kc-1.  For each plane the new search direction p = M^-1 r + beta p
 // and the conductivity K are written into a (TY+2) x (TX+2) LDS tile (the one-cell halo is
@@ -1850,6 +1962,19 @@ static void refine_tiled(hipStream_t s, const RefineArgs& a, int kc)
     const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY, nkz = (a.g.nz + kc - 1) / kc;
     hipLaunchKernelGGL((k_refine_residual_tiled<TX, TY>), dim3(ntx * nty * nkz), dim3(TX, TY), 0, s, a, kc, ntx,
                        nty, nkz);
+}
+
+int launch_refine_line(hipStream_t s, const RefineArgs& a, float* z32)
+{
+    constexpr int TX = 32, TY = 8;
+    const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
+    if (ntx * nty > kMaxBlocks || a.g.nz > kLineMaxNz) return 0;
+    hipLaunchKernelGGL(k_refine_add, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a.g.n, a.x, (const float*)a.d32);
+    const dim3 gr(ntx * nty), bl(TX, TY);
+    if (a.g.nz <= 16) hipLaunchKernelGGL((k_refine_line<16, TX, TY>), gr, bl, 0, s, a, z32, ntx, nty);
+    else if (a.g.nz <= 32) hipLaunchKernelGGL((k_refine_line<32, TX, TY>), gr, bl, 0, s, a, z32, ntx, nty);
+    else hipLaunchKernelGGL((k_refine_line<64, TX, TY>), gr, bl, 0, s, a, z32, ntx, nty);
+    return ntx * nty;
 }
 
 int launch_refine_tiled(hipStream_t s, const RefineArgs& a, const TileCfg& t)
