@@ -797,17 +797,6 @@ int ensure_mg(dhara_ctx* c)
         }
     }
     c->mg_ready = true;
-    // capture (nothing runs) the per-iteration V-cycle: fixed hierarchy, state and partials
-    hipGraph_t gr = nullptr;
-    HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-    dh::launch_mg_vcycle(c->s, c->mg, c->st_olf, c->part[0], nullptr);
-    HIPCHK(c, hipStreamEndCapture(c->s, &gr));
-    hipError_t e = hipGraphInstantiate(&c->mg_graph, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    if (e != hipSuccess) {
-        c->mg_graph = nullptr;
-        return set_err(c, DHARA_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    }
     return DHARA_OK;
 }
 
@@ -845,7 +834,7 @@ int do_overland_step(dhara_ctx* c, dhara_step_stats* st)
             k.nparts = dh::grid_for(c->S);
             k.part_out = c->part[0];
             dh::launch_pcg_update_f64(c->s, c->S, c->H, c->r2, c->D2, pp[(it + 1) & 1], c->q2, k);
-            if (mg) (void)hipGraphLaunch(c->mg_graph, c->s);   // launch_mg_vcycle(.., part[0], nullptr)
+            if (mg) dh::launch_mg_vcycle(c->s, c->mg, c->st_olf, c->part[0], nullptr);
         });
     if (rc) return rc;
     st->olf_iters = c->h_st->iter;
@@ -1031,7 +1020,6 @@ int dhara_destroy(dhara_ctx* c)
     if (!c) return DHARA_OK;
     if (c->s) (void)hipStreamSynchronize(c->s);
     free_adjoint(c);
-    if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void* v : c->allocs) (void)hipFree(v);
     if (c->h_st) (void)hipHostFree(c->h_st);

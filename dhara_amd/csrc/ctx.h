@@ -46,9 +46,6 @@ struct dhara_ctx {
     // overland multigrid hierarchy (allocated on first use) and the preconditioned residual
     dh::MgHier mg{};
     bool mg_ready = false;
-    // the V-cycle of every overland CG iteration (same launches, same arguments) as a graph, so
-    // its nine short kernels go to the device in one submission
-    hipGraphExec_t mg_graph = nullptr;
     double* z2 = nullptr;
     // reductions: partial buffers A (written by assembly/update kernels) and B (SpMV kernels)
     double* part[2] = {nullptr, nullptr};

@@ -201,19 +201,16 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
         int o = 0;
         for (int l = lc; l < nl; ++l) { off[l] = o; o += h.lv[l].nx * h.lv[l].ny; }
     }
-    // all levels in one flat pass, so that every thread's loads are independent (one memory
-    // latency for the whole hierarchy rather than one per level)
-    const int ntot = off[nl - 1] + h.lv[nl - 1].nx * h.lv[nl - 1].ny;
-    for (int u = threadIdx.x; u < ntot; u += blockDim.x) {
-        int l = lc;
-        while (l + 1 < nl && u >= off[l + 1]) ++l;
+    for (int l = lc; l < nl; ++l) {
         const MgLevel& L = h.lv[l];
-        const int t = u - off[l];
-        sd[u] = L.d[t];
-        swe[u] = L.we[t];
-        swn[u] = L.wn[t];
-        sb[u] = (l == lc) ? L.b[t] : 0.0;
-        sx[u] = 0.0;
+        const int n = L.nx * L.ny;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            sd[off[l] + t] = L.d[t];
+            swe[off[l] + t] = L.we[t];
+            swn[off[l] + t] = L.wn[t];
+            sb[off[l] + t] = (l == lc) ? L.b[t] : 0.0;
+            sx[off[l] + t] = 0.0;
+        }
     }
     __syncthreads();
     auto half = [&](int l, int color) {   // cells of the colour only: row j holds i = (color - j) & 1, +2, ...
