@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--refine-max", type=int, default=None, help="mixed: FP64 refinement sweeps per Picard iteration")
     ap.add_argument("--inner-tol", type=float, default=None, help="mixed: FP32 inner relative tolerance")
     ap.add_argument("--mixed-lin-tol", type=float, default=None, help="mixed: FP64 residual target")
+    ap.add_argument("--tile-f64", default=None, metavar="TX,TY,KC",
+                    help="SpMV tile of the FP64 solves instead of the autotuner's (profiler runs, whose "
+                         "instrumentation distorts the tuning)")
+    ap.add_argument("--tile-f32", default=None, metavar="TX,TY,KC", help="the same for the FP32 solves")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 Jacobi-PCG reference run (with --precond column)")
     ap.add_argument("--precond", choices=["jacobi", "column"], default="column",
@@ -209,12 +213,16 @@ def main():
     for name, w in order:
         # warm-up on a throw-away context, then the tile autotuner on its operator; the tuned
         # shape goes to the library's process-wide table, which the timed context reads
+        pinned = {8: args.tile_f64, 4: args.tile_f32}
         m = dhara_amd.FlowModel(w.grid, cp(w.params))
         w.apply(m)
         w.run(m, max(W, 1))
-        tune_us = m.autotune()
+        tune_us = m.autotune() if not any(pinned.values()) else (0.0, 0.0)
         m.close()
         m = dhara_amd.FlowModel(w.grid, cp(w.params))
+        for eb, spec in pinned.items():
+            if spec:
+                m.set_tile(eb, *[int(v) for v in spec.split(",")])
         m_tile_fp64 = m.get_krylov_tile(8) if one_kernel else m.get_tile(8)
         m_tile_fp32 = m.get_krylov_tile(4) if one_kernel else m.get_tile(4)
         if one_kernel:
