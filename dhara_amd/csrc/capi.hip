@@ -780,7 +780,7 @@ int ensure_mg(dhara_ctx* c)
     const int nl = dh::mg_levels(c->grid.nx, c->grid.ny, dims);
     if (nl > dh::kMgMaxLevels) return set_err(c, DHARA_E_UNSUPPORTED, "surface grid too large for the multigrid");
     int rc;
-    if ((rc = dalloc(c, &c->z2, c->S))) return rc;
+    if ((rc = dalloc(c, &c->z2, c->S)) || (rc = dalloc(c, &c->r2b, c->S))) return rc;
     c->mg.nlev = nl;
     for (int l = 0; l < nl; ++l) {
         dh::MgLevel& L = c->mg.lv[l];
@@ -833,8 +833,13 @@ int do_overland_step(dhara_ctx* c, dhara_step_stats* st)
             k.part_in = c->part[1];
             k.nparts = dh::grid_for(c->S);
             k.part_out = c->part[0];
-            dh::launch_pcg_update_f64(c->s, c->S, c->H, c->r2, c->D2, pp[(it + 1) & 1], c->q2, k);
-            if (mg) dh::launch_mg_vcycle(c->s, c->mg, c->st_olf, c->part[0], nullptr);
+            if (mg) {   // H += alpha p, r -= alpha q inside the V-cycle's first kernel (r2 / r2b alternate)
+                double* const rb[2] = {c->r2, c->r2b};
+                const dh::MgUpd u{k, c->H, pp[(it + 1) & 1], c->q2, rb[it & 1], rb[(it + 1) & 1]};
+                dh::launch_mg_vcycle_upd(c->s, c->mg, c->st_olf, c->part[0], u);
+            } else {
+                dh::launch_pcg_update_f64(c->s, c->S, c->H, c->r2, c->D2, pp[(it + 1) & 1], c->q2, k);
+            }
         });
     if (rc) return rc;
     st->olf_iters = c->h_st->iter;

@@ -56,6 +56,29 @@ struct SolveCtl {
     PcgState* st;
 };
 
+// Start of a PCG update kernel (x += alpha p, r -= alpha q): alpha = (r, z) / (p, A p) from the
+// SpMV kernel's (p, A p) partials and the state's (r, z); false when the solve is over (or, with
+// a zero or non-finite step, marks it over).  Every block takes the same decision.
+__device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha, int bid)
+{
+    PcgState* st = sc.st;
+    if (st->done) return false;
+    double pq[1];
+    sum_partials<1>(sc.part_in, sc.nparts, 1, pq);
+    const double a = st->rz[sc.it & 1] / pq[0];
+    if (!(pq[0] != 0.0) || !isfinite(a)) {
+        if (bid == 0 && lin_tid() == 0) {
+            st->iter = sc.it;
+            st->conv = 0;
+            st->nan = 1;
+            st->done = 1;
+        }
+        return false;
+    }
+    alpha = a;
+    return true;
+}
+
 struct VgArgs {
     Geo g;
     Soil soil;
@@ -349,5 +372,15 @@ void launch_mg_setup(hipStream_t s, const Geo2& g, const double* ke, const doubl
 // y_0 = V-cycle(b_0); with `part`, partial sums [(b0, y0), (b0, b0), (bb_src, bb_src), (b0, b0)].
 // Every kernel returns at once when st->done is set (st may be NULL).
 void launch_mg_vcycle(hipStream_t s, const MgHier& h, const PcgState* st, double* part, const double* bb_src);
+// The overland CG's update fused into the V-cycle's first kernel: alpha from the SpMV partials
+// (sc), H += alpha p and r_new = r_old - alpha q (the tile's halo recomputes r_new from r_old
+// and q, so r_old and r_new must be different buffers), then the V-cycle of r_new.
+struct MgUpd {
+    SolveCtl sc;
+    double* x;
+    const double *p, *q, *r_old;
+    double* r_new;
+};
+void launch_mg_vcycle_upd(hipStream_t s, const MgHier& h, const PcgState* st, double* part, const MgUpd& u);
 
 }  // namespace dh

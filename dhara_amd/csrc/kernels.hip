@@ -566,26 +566,6 @@ __device__ __forceinline__ bool pcg_begin(const SolveCtl& sc, double& beta, int 
     return true;
 }
 
-// Start of the update kernel: alpha = (r, z) / (p, A p).
-__device__ __forceinline__ bool update_begin(const SolveCtl& sc, double& alpha, int bid)
-{
-    PcgState* st = sc.st;
-    if (st->done) return false;
-    double pq[1];
-    sum_partials<1>(sc.part_in, sc.nparts, 1, pq);
-    const double a = st->rz[sc.it & 1] / pq[0];
-    if (!(pq[0] != 0.0) || !isfinite(a)) {
-        if (bid == 0 && lin_tid() == 0) {
-            st->iter = sc.it;
-            st->conv = 0;
-            st->nan = 1;
-            st->done = 1;
-        }
-        return false;
-    }
-    alpha = a;
-    return true;
-}
 
 // ZIN: r holds the preconditioned residual z (column preconditioner) instead of r.
 template <typename T, bool DIR, bool ZIN = false>

@@ -95,10 +95,17 @@ __device__ __forceinline__ void lds_half_sweep(int w, int ox, int oy, int color,
     __syncthreads();
 }
 
-// Down: x = 2 forward RBGS sweeps from 0 on b, r = b - A x, b_coarse = P^T r.
-__global__ __launch_bounds__(kMgThreads) void k_mg_down(MgHier h, int l, const PcgState* st)
+// Down: x = 2 forward RBGS sweeps from 0 on b, r = b - A x, b_coarse = P^T r.  UPD (level 0):
+// b is the CG residual after this iteration's update, done here (see MgUpd).
+template <bool UPD>
+__global__ __launch_bounds__(kMgThreads) void k_mg_down(MgHier h, int l, const PcgState* st, MgUpd u)
 {
-    if (st && st->done) return;
+    double alpha = 0.0;
+    if constexpr (UPD) {
+        if (!update_begin(u.sc, alpha, blockIdx.x)) return;
+    } else if (st && st->done) {
+        return;
+    }
     constexpr int W = kMgT + 2 * kMgHd;
     __shared__ double sb[W * W], sd[W * W], swe[W * W], swn[W * W], sx[W * W], sr[kMgT * kMgT];
     const MgLevel L = h.lv[l], C = h.lv[l + 1];
@@ -110,7 +117,8 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_down(MgHier h, int l, const P
         const int gi = ox + li, gj = oy + lj;
         const bool in = gi >= 0 && gj >= 0 && gi < L.nx && gj < L.ny;
         const int g = gj * L.nx + gi;
-        sb[t] = in ? L.b[g] : 0.0;
+        if constexpr (UPD) sb[t] = in ? u.r_old[g] - alpha * u.q[g] : 0.0;
+        else sb[t] = in ? L.b[g] : 0.0;
         sd[t] = in ? L.d[g] : 1.0;
         swe[t] = in ? L.we[g] : 0.0;
         swn[t] = in ? L.wn[g] : 0.0;
@@ -128,7 +136,14 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_down(MgHier h, int l, const P
         const double Ax = sd[q] * sx[q] - swe[q] * sx[q + 1] - swe[q - 1] * sx[q - 1] - swn[q] * sx[q + W] -
                           swn[q - W] * sx[q - W];
         const int gi = gx0 + ti, gj = gy0 + tj;
-        if (gi < L.nx && gj < L.ny) L.x[gj * L.nx + gi] = sx[q];
+        if (gi < L.nx && gj < L.ny) {
+            const int g = gj * L.nx + gi;
+            L.x[g] = sx[q];
+            if constexpr (UPD) {
+                u.r_new[g] = sb[q];
+                u.x[g] = u.x[g] + alpha * u.p[g];
+            }
+        }
         sr[t] = sb[q] - Ax;
     }
     __syncthreads();
@@ -284,6 +299,17 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
     if (part) block_partials<4, false>(acc, part);
 }
 
+// The update of MgUpd as a kernel of its own (surfaces handled entirely by the coarse kernel).
+__global__ __launch_bounds__(kBlock) void k_pcg_update_mg(int n, MgUpd u)
+{
+    double alpha;
+    if (!update_begin(u.sc, alpha, blockIdx.x)) return;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        u.x[c] = u.x[c] + alpha * u.p[c];
+        u.r_new[c] = u.r_old[c] - alpha * u.q[c];
+    }
+}
+
 inline int mg_tiles(const MgLevel& L) { return ((L.nx + kMgT - 1) / kMgT) * ((L.ny + kMgT - 1) / kMgT); }
 
 }  // namespace
@@ -328,12 +354,32 @@ void launch_mg_vcycle(hipStream_t s, const MgHier& h, const PcgState* st, double
 {
     const int lc = mg_coarse_level(h);
     for (int l = 0; l < lc; ++l)
-        hipLaunchKernelGGL(k_mg_down, dim3(mg_tiles(h.lv[l])), dim3(kMgThreads), 0, s, h, l, st);
+        hipLaunchKernelGGL(k_mg_down<false>, dim3(mg_tiles(h.lv[l])), dim3(kMgThreads), 0, s, h, l, st, MgUpd{});
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kMgThreads), 0, s, h, st, lc == 0 ? part : nullptr,
                        lc == 0 ? bb_src : nullptr);
     for (int l = lc - 1; l >= 0; --l)
         hipLaunchKernelGGL(k_mg_up, dim3(mg_tiles(h.lv[l])), dim3(kMgThreads), 0, s, h, l, h.lv[l].y, st,
                            l == 0 ? part : nullptr, l == 0 ? bb_src : nullptr);
+}
+
+void launch_mg_vcycle_upd(hipStream_t s, const MgHier& h, const PcgState* st, double* part, const MgUpd& u)
+{
+    MgHier hh = h;
+    hh.lv[0].b = u.r_new;   // level 0 of this V-cycle: the updated residual
+    const int lc = mg_coarse_level(hh);
+    if (lc == 0) {   // a surface of <= 32 x 32 cells: the update as its own kernel
+        hipLaunchKernelGGL(k_pcg_update_mg, dim3(grid_for(h.lv[0].nx * h.lv[0].ny)), dim3(kBlock), 0, s,
+                           h.lv[0].nx * h.lv[0].ny, u);
+        launch_mg_vcycle(s, hh, st, part, nullptr);
+        return;
+    }
+    hipLaunchKernelGGL(k_mg_down<true>, dim3(mg_tiles(hh.lv[0])), dim3(kMgThreads), 0, s, hh, 0, st, u);
+    for (int l = 1; l < lc; ++l)
+        hipLaunchKernelGGL(k_mg_down<false>, dim3(mg_tiles(hh.lv[l])), dim3(kMgThreads), 0, s, hh, l, st, MgUpd{});
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kMgThreads), 0, s, hh, st, nullptr, nullptr);
+    for (int l = lc - 1; l >= 0; --l)
+        hipLaunchKernelGGL(k_mg_up, dim3(mg_tiles(hh.lv[l])), dim3(kMgThreads), 0, s, hh, l, hh.lv[l].y, st,
+                           l == 0 ? part : nullptr, nullptr);
 }
 
 }  // namespace dh
