@@ -393,11 +393,14 @@ def bench_ensemble(args, dev):
     ws = [inputs.c5_member(sp) for sp in specs]
     ncell = sum(w.ncells for w in ws)
 
+    column = args.precond == "column"
+
     def build(prec):
         ms = []
         for w in ws:
             p = _abi.Params.from_buffer_copy(w.params)
             p.precision = prec
+            p.precond = _abi.PRECOND_COLUMN if column else _abi.PRECOND_JACOBI
             m = dhara_amd.FlowModel(w.grid, p)
             w.apply(m)
             ms.append(m)
@@ -407,10 +410,16 @@ def bench_ensemble(args, dev):
     for name, prec in (("mixed", _abi.MIXED), ("fp64", _abi.FP64)):
         ms = build(prec)
         ens = dhara_amd.Ensemble(ms)
-        tune_us, div = ens.autotune()   # planes per tile of the batched one-kernel PCG (untimed)
-        floor_us = ncell * 8 * (4 if prec == _abi.MIXED else 8) / 8e12 * 1e6   # 8 arrays at 8 TB/s
-        print(f"# ensemble {name}: {len(ms)} members, {ncell:.4e} cells, {K} steps; tiles 32x8 x ceil(nz/{div}) "
-              f"({tune_us:.0f} us per batched iteration, HBM floor {floor_us:.0f} us)", file=sys.stderr, flush=True)
+        if column:   # the column form's kernels are untiled: nothing to tune
+            tune_us, div = 0.0, 0
+            print(f"# ensemble {name}: {len(ms)} members, {ncell:.4e} cells, {K} steps; column-preconditioned PCG",
+                  file=sys.stderr, flush=True)
+        else:
+            tune_us, div = ens.autotune()   # planes per tile of the batched one-kernel PCG (untimed)
+            floor_us = ncell * 8 * (4 if prec == _abi.MIXED else 8) / 8e12 * 1e6   # 8 arrays at 8 TB/s
+            print(f"# ensemble {name}: {len(ms)} members, {ncell:.4e} cells, {K} steps; tiles 32x8 x ceil(nz/{div}) "
+                  f"({tune_us:.0f} us per batched iteration, HBM floor {floor_us:.0f} us)", file=sys.stderr,
+                  flush=True)
         t0 = time.perf_counter()
         st = []
         for k in range(0, K, 20):
@@ -432,9 +441,12 @@ def bench_ensemble(args, dev):
         "config": {"workload": f"batched ensemble of {len(ws)} members (nx, ny in 32..128, nz in 10..40), "
                                f"{ncell} cells in total, dt 30 s, {K} steps, one launch per kernel for all members, "
                                f"Krylov batches replayed from a captured hipGraph", "device": dev,
-                   "krylov_tiles": {n_: f"32x8 columns x ceil(nz/{res[n_]['div']}) planes, "
-                                        f"{res[n_]['tune_us']:.0f} us per batched iteration (ensemble autotuner)"
-                                    for n_ in res}},
+                   "precond": args.precond,
+                   "krylov_tiles": ({n_: "untiled SpMV + one thread per column (column-preconditioned PCG)"
+                                     for n_ in res} if column else
+                                    {n_: f"32x8 columns x ceil(nz/{res[n_]['div']}) planes, "
+                                         f"{res[n_]['tune_us']:.0f} us per batched iteration (ensemble autotuner)"
+                                     for n_ in res})},
         "fp64": {"cell_steps_per_s": ncell * K / res["fp64"]["secs"], "seconds": res["fp64"]["secs"]},
         "mixed": {"seconds": r["secs"], "picard_iters": sum(x["picard_iters"] for x in flat),
                   "krylov_iters": sum(x["lin_iters"] for x in flat),

@@ -76,6 +76,11 @@ struct dhara_ensemble {
     std::vector<dh::Cg1Args<double>> h_c1d;
     std::vector<dh::Cg1Args<float>> h_c1f;
     int kc_div = 0;                 // dhara_ensemble_autotune: planes per tile = ceil(nz / kc_div); 0: member's tile
+    // column-preconditioned PCG (every member with DHARA_PRECOND_COLUMN): the column kernel's
+    // block maps, one per register-array size (nz <= 16, 32, 64)
+    bool line = false;
+    dh::BlkMap* map_l[3] = {nullptr, nullptr, nullptr};
+    int nb_l[3] = {0, 0, 0};
 };
 
 namespace {
@@ -141,6 +146,12 @@ void krylov_batch(dhara_ensemble* e, bool two_d)
         if (two_d) {
             dh::launch_batched_spmv2(e->s, e->a_k2, e->map2, e->nb2, e->active, e->it_base, k);
             dh::launch_batched_update2(e->s, e->a_k2, e->map2, e->nb2, e->active, e->it_base, k);
+        } else if (e->line) {
+            dh::launch_batched_spmv_z(e->s, e->mixed, e->a_k3, e->map3, e->nb3, e->active, e->it_base, k);
+            for (int b = 0; b < 3; ++b)
+                if (e->nb_l[b])
+                    dh::launch_batched_line(e->s, e->mixed, false, 16 << b, e->a_k3, e->map_l[b], e->nb_l[b],
+                                            e->active, e->it_base, k);
         } else if (e->one) {
             dh::launch_batched_cg1(e->s, e->mixed, e->a_c1, e->map_c1, e->nb_c1, e->active, e->it_base, k);
         } else {
@@ -171,6 +182,13 @@ int krylov_solve(dhara_ensemble* e, bool two_d, const std::vector<int>& on, int 
     EHIP(e, hipMemsetAsync(e->it_base, 0, sizeof(int), e->s));   // outside any captured graph
     if (!two_d && e->one) {
         dh::launch_batched_cg1_init(e->s, e->mixed, e->a_c1, e->map3, e->nb3, e->active);
+        EHIP(e, hipGetLastError());
+    }
+    if (!two_d && e->line) {   // z = T^-1 r of the fresh residual
+        for (int b = 0; b < 3; ++b)
+            if (e->nb_l[b])
+                dh::launch_batched_line(e->s, e->mixed, true, 16 << b, e->a_k3, e->map_l[b], e->nb_l[b], e->active,
+                                        e->it_base, 0);
         EHIP(e, hipGetLastError());
     }
     if (e->use_graph) {
@@ -222,6 +240,13 @@ void build_k3(dhara_ensemble* e, bool first_sweep)
             k.x = c->d32; k.res = c->r32; k.p[0] = c->p32[0]; k.p[1] = c->p32[1]; k.q = c->q32;
             k.spmv = ctl(c, &e->st[m], c->p.inner_maxiter, c->p.lin_tol, true, first_sweep, c->part[0], np, c->part[1]);
             k.upd = ctl(c, &e->st[m], c->p.inner_maxiter, c->p.lin_tol, true, first_sweep, c->part[1], np, c->part[0]);
+            if (e->line) {
+                k.z = c->zl32;
+                k.init = ctl(c, &e->st[m], c->p.inner_maxiter, c->p.lin_tol, true, first_sweep, c->part[0], np,
+                             c->part_line);
+                k.part_first = c->part_line;
+                k.spmv.nparts = dh::line_blocks(c->g);
+            }
         } else {
             dh::KrylovArgs<double>& k = e->h_k3d[m];
             k = dh::KrylovArgs<double>{};
@@ -230,6 +255,12 @@ void build_k3(dhara_ensemble* e, bool first_sweep)
             k.x = c->x; k.res = c->r; k.p[0] = c->pv[0]; k.p[1] = c->pv[1]; k.q = c->q;
             k.spmv = ctl(c, &e->st[m], c->p.lin_maxiter, c->p.lin_tol, false, true, c->part[0], np, c->part[1]);
             k.upd = ctl(c, &e->st[m], c->p.lin_maxiter, c->p.lin_tol, false, true, c->part[1], np, c->part[0]);
+            if (e->line) {
+                k.z = c->zl;
+                k.init = ctl(c, &e->st[m], c->p.lin_maxiter, c->p.lin_tol, false, true, c->part[0], np, c->part_line);
+                k.part_first = c->part_line;
+                k.spmv.nparts = dh::line_blocks(c->g);
+            }
         }
     }
 }
@@ -556,8 +587,25 @@ int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble**
     }
     e->nb3 = (int)m3.size();
     e->nb2 = (int)m2.size();
-    // one-kernel PCG when every member asks for it (32-bit plane offsets: N * 8 < 4 GiB)
-    e->one = true;
+    // column preconditioner when every member asks for it (and its columns fit the kernel)
+    e->line = true;
+    for (dhara_ctx* c : e->mem)
+        if (c->p.precond != DHARA_PRECOND_COLUMN || !dh::line_supported(c->g)) e->line = false;
+    std::vector<dh::BlkMap> ml[3];
+    if (e->line) {
+        for (int m = 0; m < n; ++m) {
+            dhara_ctx* c = e->mem[m];
+            if (alloc_line(c, e->mixed) || hipStreamSynchronize(c->s) != hipSuccess) {
+                dhara_ensemble_destroy(e);
+                return DHARA_E_ALLOC;
+            }
+            const int b = c->g.nz <= 16 ? 0 : c->g.nz <= 32 ? 1 : 2;
+            const int nb = dh::line_blocks(c->g);
+            for (int i = 0; i < nb; ++i) ml[b].push_back(dh::BlkMap{m, i, nb});
+        }
+    }
+    // otherwise the one-kernel Jacobi PCG when every member asks for it (32-bit plane offsets)
+    e->one = !e->line;
     for (dhara_ctx* c : e->mem)
         if (c->p.krylov != DHARA_KRYLOV_ONE_KERNEL || (size_t)c->N * 8 >= (size_t(1) << 32)) e->one = false;
     std::vector<dh::BlkMap> mc1;
@@ -590,6 +638,10 @@ int dhara_ensemble_create(int32_t n, dhara_ctx* const* members, dhara_ensemble**
         else rc = ealloc(e, (dh::Cg1Args<double>**)&e->a_c1, n);
         if (!rc) rc = ealloc(e, &e->map_c1, mc1.size());
         if (!rc) rc = upload(e, e->map_c1, mc1);   // the first nb_c1 entries are live
+    }
+    for (int b = 0; b < 3 && !rc; ++b) {
+        e->nb_l[b] = (int)ml[b].size();
+        if (e->nb_l[b] && !(rc = ealloc(e, &e->map_l[b], ml[b].size()))) rc = upload(e, e->map_l[b], ml[b]);
     }
     if (rc) { dhara_ensemble_destroy(e); return rc; }
     if (hipHostMalloc((void**)&e->h_active, n * sizeof(int)) != hipSuccess ||
@@ -648,7 +700,9 @@ int dhara_ensemble_step(dhara_ensemble* e, int32_t nsteps, dhara_step_stats* sta
 int dhara_ensemble_autotune(dhara_ensemble* e, double* best_us, int32_t* planes_div)
 {
     if (!e) return DHARA_E_ARG;
-    if (!e->one) return eset(e, DHARA_E_UNSUPPORTED, "autotune: members do not use the one-kernel PCG");
+    if (!e->one)
+        return eset(e, DHARA_E_UNSUPPORTED, "autotune: members do not use the one-kernel Jacobi PCG (the column "
+                                            "form's column kernel and untiled SpMV have no tile to tune)");
     const int M = e->M;
     const size_t es = e->mixed ? 4 : 8;
     std::vector<void*> scratch;

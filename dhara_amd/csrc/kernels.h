@@ -309,6 +309,11 @@ struct KrylovArgs {
     T *x, *res, *p[2], *q;
     const double *ke, *kn;       // overland operator (2D solves)
     SolveCtl spmv, upd;          // part_in/part_out/st of the two kernels; `it` is ignored
+    // column-preconditioned form: z = T^-1 r, the start-up kernel's control (reads the assembly /
+    // refinement partials, writes part_first) and the partials the first SpMV reads
+    T* z;
+    SolveCtl init;
+    const double* part_first;
 };
 // One-kernel PCG of one model in a batched launch: tiles of kCg1BatchTx x kCg1BatchTy columns
 // and kc planes, one block per tile.  Iteration it reads r[it & 1], p[(it + 1) & 1] and the
@@ -338,6 +343,13 @@ enum BatchKind { BK_VG = 0, BK_VG_PICARD, BK_SURF_PRE, BK_SURF_POST, BK_ASM_F64,
 // RefineArgs, EndArgs or OlfArgs), one per model.
 void launch_batched(hipStream_t s, BatchKind kind, const void* args, const BlkMap* map, int nblocks,
                     const int* active);
+// Column-preconditioned form (untiled SpMV with z as input over map3; the column kernel over a map
+// of line_blocks(g) blocks per member, one launch per register-array size nzm = 16, 32, 64; init:
+// the start-up z = T^-1 r of a fresh residual).
+void launch_batched_spmv_z(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,
+                           const int* active, const int* it_base, int k);
+void launch_batched_line(hipStream_t s, bool fp32, bool init, int nzm, const void* kargs, const BlkMap* map,
+                         int nblocks, const int* active, const int* it_base, int k);
 void launch_batched_spmv3(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,
                           const int* active, const int* it_base, int k);
 void launch_batched_update3(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,

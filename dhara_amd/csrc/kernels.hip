@@ -665,9 +665,9 @@ __device__ __forceinline__ double jac_inv(double d)
 // overwritten by the back substitution) instead of registers: 2 x 32 doubles per thread would
 // leave 2 waves per SIMD, too few for the 2.5 waves of columns of a 400 x 400 surface.
 template <typename T, int NZM, bool INIT>
-__global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K, const T* __restrict__ D,
-                                                 T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-                                                 const T* __restrict__ q, T* __restrict__ z, SolveCtl sc)
+__device__ __forceinline__ void b_line(Geo g, const T* __restrict__ K, const T* __restrict__ D, T* __restrict__ x,
+                                       T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ q,
+                                       T* __restrict__ z, SolveCtl sc, Blk bk)
 {
     double alpha_d = 0.0;
     double carry[2] = {0.0, 0.0};
@@ -678,14 +678,14 @@ __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K,
         carry[0] = in[2];
         carry[1] = in[3];
     } else {
-        if (!update_begin(sc, alpha_d, blockIdx.x)) return;
+        if (!update_begin(sc, alpha_d, bk.id)) return;
     }
     const T alpha = (T)alpha_d;
     const T hz = (T)(0.5 * g.wz);
     const int nz = g.nz, nxy = g.nxy;
     double acc[INIT ? 4 : 2] = {};
     constexpr bool kCpInZ = sizeof(T) == 8;
-    for (int col = blockIdx.x * kBlock + threadIdx.x; col < nxy; col += gridDim.x * kBlock) {
+    for (int col = bk.id * kBlock + threadIdx.x; col < nxy; col += bk.n * kBlock) {
         T y[NZM], cp[kCpInZ ? 1 : NZM];
         T kprev = T(0), yprev = T(0), inv_prev = T(0);
 #pragma unroll
@@ -734,14 +734,22 @@ __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K,
         }
     }
     if constexpr (INIT) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (bk.id == 0 && threadIdx.x == 0) {
             acc[2] = carry[0];
             acc[3] = carry[1];
         }
-        block_partials<4, false>(acc, sc.part_out);
+        block_partials<4, false>(acc, sc.part_out, bk.id);
     } else {
-        block_partials<2, false>(acc, sc.part_out);
+        block_partials<2, false>(acc, sc.part_out, bk.id);
     }
+}
+
+template <typename T, int NZM, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                                 T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                                                 const T* __restrict__ q, T* __restrict__ z, SolveCtl sc)
+{
+    b_line<T, NZM, INIT>(g, K, D, x, r, p, q, z, sc, hw_blk());
 }
 
 // Reliable update of the mixed column-preconditioned PCG after k_refine_add (x += d): the true
@@ -1383,6 +1391,34 @@ __global__ __launch_bounds__(kBlock) void kb_update3(const KrylovArgs<T>* __rest
     SolveCtl sc = a.upd;
     sc.it = *it_base + k;
     b_pcg_update<T>(a.g.n, a.x, a.res, a.D, a.p[(sc.it + 1) & 1], a.q, sc, Blk{e.id, e.n});
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kb_spmv_z(const KrylovArgs<T>* __restrict__ args,
+                                                    const BlkMap* __restrict__ map, const int* __restrict__ active,
+                                                    const int* __restrict__ it_base, int k)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const KrylovArgs<T>& a = args[e.m];
+    SolveCtl sc = a.spmv;
+    sc.it = *it_base + k;
+    if (sc.it == 0) sc.part_in = a.part_first;
+    b_pcg3_spmv<T, false, true>(a.g, nullptr, a.K, a.D, a.z, a.p[sc.it & 1], a.p[(sc.it + 1) & 1], a.q, sc,
+                                Blk{e.id, e.n});
+}
+
+template <typename T, int NZM, bool INIT>
+__global__ __launch_bounds__(kBlock) void kb_line(const KrylovArgs<T>* __restrict__ args,
+                                                  const BlkMap* __restrict__ map, const int* __restrict__ active,
+                                                  const int* __restrict__ it_base, int k)
+{
+    const BlkMap e = map[blockIdx.x];
+    if (!active[e.m]) return;
+    const KrylovArgs<T>& a = args[e.m];
+    SolveCtl sc = INIT ? a.init : a.upd;
+    sc.it = INIT ? 0 : *it_base + k;
+    b_line<T, NZM, INIT>(a.g, a.K, a.D, a.x, a.res, a.p[(sc.it + 1) & 1], a.q, a.z, sc, Blk{e.id, e.n});
 }
 
 __global__ __launch_bounds__(kBlock) void kb_spmv2(const KrylovArgs<double>* __restrict__ args,
@@ -2247,6 +2283,39 @@ void launch_batched(hipStream_t s, BatchKind kind, const void* args, const BlkMa
     case BK_OLF_POST:
         hipLaunchKernelGGL((kb_args<OlfArgs, b_olf_post>), gr, bl, 0, s, (const OlfArgs*)args, map, active);
         break;
+    }
+}
+
+void launch_batched_spmv_z(hipStream_t s, bool fp32, const void* kargs, const BlkMap* map, int nblocks,
+                           const int* active, const int* it_base, int k)
+{
+    if (fp32)
+        hipLaunchKernelGGL(kb_spmv_z<float>, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<float>*)kargs, map,
+                           active, it_base, k);
+    else
+        hipLaunchKernelGGL(kb_spmv_z<double>, dim3(nblocks), dim3(kBlock), 0, s, (const KrylovArgs<double>*)kargs,
+                           map, active, it_base, k);
+}
+
+template <typename T, bool INIT>
+static void batched_line(hipStream_t s, int nzm, const void* kargs, const BlkMap* map, int nblocks,
+                         const int* active, const int* it_base, int k)
+{
+    const KrylovArgs<T>* a = (const KrylovArgs<T>*)kargs;
+    if (nzm <= 16) hipLaunchKernelGGL((kb_line<T, 16, INIT>), dim3(nblocks), dim3(kBlock), 0, s, a, map, active, it_base, k);
+    else if (nzm <= 32) hipLaunchKernelGGL((kb_line<T, 32, INIT>), dim3(nblocks), dim3(kBlock), 0, s, a, map, active, it_base, k);
+    else hipLaunchKernelGGL((kb_line<T, 64, INIT>), dim3(nblocks), dim3(kBlock), 0, s, a, map, active, it_base, k);
+}
+
+void launch_batched_line(hipStream_t s, bool fp32, bool init, int nzm, const void* kargs, const BlkMap* map,
+                         int nblocks, const int* active, const int* it_base, int k)
+{
+    if (fp32) {
+        if (init) batched_line<float, true>(s, nzm, kargs, map, nblocks, active, it_base, k);
+        else batched_line<float, false>(s, nzm, kargs, map, nblocks, active, it_base, k);
+    } else {
+        if (init) batched_line<double, true>(s, nzm, kargs, map, nblocks, active, it_base, k);
+        else batched_line<double, false>(s, nzm, kargs, map, nblocks, active, it_base, k);
     }
 }
 

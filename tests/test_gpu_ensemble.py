@@ -2,8 +2,9 @@
 
 * FP64: the batched launches run the same kernel bodies with the same per-member block counts
   as a single model -- the one-kernel PCG on 32 x 8 whole-column tiles with the untiled
-  start-up (or the two-kernel PCG with the untiled SpMV) and the Jacobi-preconditioned
-  overland solve -- so every member's state equals the member stepped alone, bit for bit.
+  start-up, the two-kernel PCG with the untiled SpMV, or the column-preconditioned PCG with the
+  untiled SpMV and the column kernel, and the Jacobi-preconditioned overland solve -- so every
+  member's state equals the member stepped alone, bit for bit.
 * The Krylov batches replayed from the captured hipGraph give the same states as the same
   batches launched one by one, replay after replay with host work in between.
 * Mixed precision: every member within 1e-6 m of the FP64 run of the same member.
@@ -22,9 +23,11 @@ SPECS = [dict(nx=33, ny=20, nz=10, sigma_lnk=0.7, rain_mm_h=30.0, seed=2001),
          dict(nx=9, ny=12, nz=20, sigma_lnk=0.5, rain_mm_h=50.0, seed=2004)]
 
 
-def member(spec, precision, untiled=True, krylov=_abi.KRYLOV_ONE_KERNEL):
+def member(spec, precision, untiled=True, krylov=_abi.KRYLOV_ONE_KERNEL, precond=_abi.PRECOND_JACOBI):
     w = inputs.c5_member(spec, precision=precision, krylov=krylov)
-    m = dhara_amd.FlowModel(w.grid, _abi.Params.from_buffer_copy(w.params))
+    p = _abi.Params.from_buffer_copy(w.params)
+    p.precond = precond
+    m = dhara_amd.FlowModel(w.grid, p)
     if untiled:
         m.set_tile(8, 0)
         m.set_tile(4, 0)
@@ -32,14 +35,16 @@ def member(spec, precision, untiled=True, krylov=_abi.KRYLOV_ONE_KERNEL):
     return m
 
 
-@pytest.mark.parametrize("krylov", [_abi.KRYLOV_ONE_KERNEL, _abi.KRYLOV_TWO_KERNEL])
-def test_fp64_ensemble_equals_members_alone(krylov):
+@pytest.mark.parametrize("krylov,precond", [(_abi.KRYLOV_ONE_KERNEL, _abi.PRECOND_JACOBI),
+                                            (_abi.KRYLOV_TWO_KERNEL, _abi.PRECOND_JACOBI),
+                                            (_abi.KRYLOV_ONE_KERNEL, _abi.PRECOND_COLUMN)])
+def test_fp64_ensemble_equals_members_alone(krylov, precond):
     steps = 4
-    ens_members = [member(s, _abi.FP64, krylov=krylov) for s in SPECS]
+    ens_members = [member(s, _abi.FP64, krylov=krylov, precond=precond) for s in SPECS]
     ens = dhara_amd.Ensemble(ens_members)
     est = ens.step(steps)
     for i, spec in enumerate(SPECS):
-        alone = member(spec, _abi.FP64, krylov=krylov)
+        alone = member(spec, _abi.FP64, krylov=krylov, precond=precond)
         st = alone.step(steps)
         np.testing.assert_array_equal(ens_members[i].download("PSI"), alone.download("PSI"))
         np.testing.assert_array_equal(ens_members[i].download("PONDING"), alone.download("PONDING"))
@@ -54,10 +59,11 @@ def test_fp64_ensemble_equals_members_alone(krylov):
             assert abs(st[k]["mb_rel"] - est[k][i]["mb_rel"]) <= 1e-14
 
 
-def test_graph_replays_equal_eager_launches():
+@pytest.mark.parametrize("precond", [_abi.PRECOND_JACOBI, _abi.PRECOND_COLUMN])
+def test_graph_replays_equal_eager_launches(precond):
     steps = 3
-    a = [member(s, _abi.MIXED) for s in SPECS]
-    b = [member(s, _abi.MIXED) for s in SPECS]
+    a = [member(s, _abi.MIXED, precond=precond) for s in SPECS]
+    b = [member(s, _abi.MIXED, precond=precond) for s in SPECS]
     ea, eb = dhara_amd.Ensemble(a), dhara_amd.Ensemble(b)
     eb.set_graph(False)
     for _ in range(steps):              # one step at a time: host work between the replays
@@ -68,13 +74,14 @@ def test_graph_replays_equal_eager_launches():
             np.testing.assert_array_equal(x.download("PONDING"), y.download("PONDING"))
 
 
-def test_mixed_ensemble_head_error():
+@pytest.mark.parametrize("precond", [_abi.PRECOND_JACOBI, _abi.PRECOND_COLUMN])
+def test_mixed_ensemble_head_error(precond):
     steps = 6
-    mixed = [member(s, _abi.MIXED) for s in SPECS]
+    mixed = [member(s, _abi.MIXED, precond=precond) for s in SPECS]
     stats = dhara_amd.Ensemble(mixed).step(steps)
     assert all(x["lin_converged"] for row in stats for x in row)
     for i, spec in enumerate(SPECS):
-        ref = member(spec, _abi.FP64)
+        ref = member(spec, _abi.FP64, precond=precond)
         ref.step(steps)
         err = np.max(np.abs(mixed[i].download("PSI") - ref.download("PSI")))
         assert err <= 1e-6, (i, err)
