@@ -9,7 +9,8 @@
 //     piecewise constant -- on a structured grid these stay 5-point (d, we, wn), so every
 //     level uses the same kernels;
 //   * smoothing: two red-black Gauss-Seidel sweeps before (red first) and after (black first)
-//     the coarse correction, which keeps the V-cycle symmetric, so CG stays valid;
+//     the coarse correction, which keeps the V-cycle symmetric, so CG stays valid; the cell
+//     update multiplies by the reciprocal of the diagonal (held in LDS);
 //   * coarse correction scaled by 1.5 (over-correction for piecewise-constant interpolation);
 //   * large levels: one tiled kernel down (smooth, residual, restrict) and one up (prolong,
 //     smooth) per level; a tile of T x T cells works in LDS with a halo of 5 (down) or 4 (up)
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(MgLevel F, MgLevel C)
 // updating the cells of `color` ((gi + gj) & 1) that are not on the LDS border.
 // Threads are dealt only the cells of the colour: row lj holds the interior cells li = p, p+2, ...
 // with p fixed by the colour, so a half-sweep of a 38 x 38 tile is one pass of 648 threads.
-__device__ __forceinline__ void lds_half_sweep(int w, int ox, int oy, int color, const double* sb, const double* sd,
+__device__ __forceinline__ void lds_half_sweep(int w, int ox, int oy, int color, const double* sb, const double* sdi,
                                                const double* swe, const double* swn, double* sx)
 {
     const int hw = (w - 1) / 2;   // colour cells per interior row, at most
@@ -90,7 +91,7 @@ __device__ __forceinline__ void lds_half_sweep(int w, int ox, int oy, int color,
         if (li > w - 2) continue;
         const int t = lj * w + li;
         const double s = sb[t] + swe[t] * sx[t + 1] + swe[t - 1] * sx[t - 1] + swn[t] * sx[t + w] + swn[t - w] * sx[t - w];
-        sx[t] = s / sd[t];
+        sx[t] = s * sdi[t];
     }
     __syncthreads();
 }
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_down(MgHier h, int l, const P
         return;
     }
     constexpr int W = kMgT + 2 * kMgHd;
-    __shared__ double sb[W * W], sd[W * W], swe[W * W], swn[W * W], sx[W * W], sr[kMgT * kMgT];
+    __shared__ double sb[W * W], sd[W * W], sdi[W * W], swe[W * W], swn[W * W], sx[W * W], sr[kMgT * kMgT];
     const MgLevel L = h.lv[l], C = h.lv[l + 1];
     const int ntx = (L.nx + kMgT - 1) / kMgT;
     const int gx0 = (blockIdx.x % ntx) * kMgT, gy0 = (blockIdx.x / ntx) * kMgT;
@@ -119,16 +120,18 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_down(MgHier h, int l, const P
         const int g = gj * L.nx + gi;
         if constexpr (UPD) sb[t] = in ? u.r_old[g] - alpha * u.q[g] : 0.0;
         else sb[t] = in ? L.b[g] : 0.0;
-        sd[t] = in ? L.d[g] : 1.0;
+        const double dg = in ? L.d[g] : 1.0;
+        sd[t] = dg;
+        sdi[t] = 1.0 / dg;
         swe[t] = in ? L.we[g] : 0.0;
         swn[t] = in ? L.wn[g] : 0.0;
         sx[t] = 0.0;
     }
     __syncthreads();
-    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
-    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
-    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
-    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 0, sb, sdi, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 1, sb, sdi, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 0, sb, sdi, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 1, sb, sdi, swe, swn, sx);
     // residual of the tile interior (exact: the halo absorbed the four half-sweeps)
     for (int t = threadIdx.x; t < kMgT * kMgT; t += blockDim.x) {
         const int tj = t / kMgT, ti = t - tj * kMgT;
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(MgHier h, int l, double* o
 {
     if (st && st->done) return;
     constexpr int W = kMgT + 2 * kMgHu;
-    __shared__ double sb[W * W], sd[W * W], swe[W * W], swn[W * W], sx[W * W];
+    __shared__ double sb[W * W], sdi[W * W], swe[W * W], swn[W * W], sx[W * W];
     const MgLevel L = h.lv[l], C = h.lv[l + 1];
     const int ntx = (L.nx + kMgT - 1) / kMgT;
     const int gx0 = (blockIdx.x % ntx) * kMgT, gy0 = (blockIdx.x / ntx) * kMgT;
@@ -175,16 +178,16 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(MgHier h, int l, double* o
         const bool in = gi >= 0 && gj >= 0 && gi < L.nx && gj < L.ny;
         const int g = gj * L.nx + gi;
         sb[t] = in ? L.b[g] : 0.0;
-        sd[t] = in ? L.d[g] : 1.0;
+        sdi[t] = in ? 1.0 / L.d[g] : 1.0;
         swe[t] = in ? L.we[g] : 0.0;
         swn[t] = in ? L.wn[g] : 0.0;
         sx[t] = in ? L.x[g] + kMgOver * C.y[(gj >> 1) * C.nx + (gi >> 1)] : 0.0;
     }
     __syncthreads();
-    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
-    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
-    lds_half_sweep(W, ox, oy, 1, sb, sd, swe, swn, sx);
-    lds_half_sweep(W, ox, oy, 0, sb, sd, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 1, sb, sdi, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 0, sb, sdi, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 1, sb, sdi, swe, swn, sx);
+    lds_half_sweep(W, ox, oy, 0, sb, sdi, swe, swn, sx);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int t = threadIdx.x; t < kMgT * kMgT; t += blockDim.x) {
         const int tj = t / kMgT, ti = t - tj * kMgT;
@@ -202,13 +205,18 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(MgHier h, int l, double* o
 }
 
 // The V-cycle from level lc (<= 32 x 32) down to 1 x 1 and back, in one block, in LDS:
-// reads b of level lc, writes y of level lc (and on level 0 the CG partial sums).
+// reads b of level lc, writes y of level lc (and on level 0 the CG partial sums).  The levels
+// are small, so a phase is one pass of a few waves followed by a barrier, and its cost is the
+// latency of one cell: the four neighbour terms read their LDS operands without branches (a
+// neighbour outside the level reads the cell itself with a zero weight) so that the reads issue
+// together, the smoother multiplies by the reciprocal diagonal kept in LDS, and the restriction
+// sums the residuals of each aggregate as it computes them.
 __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgState* st, double* part,
                                                           const double* bb_src)
 {
     if (st && st->done) return;
-    __shared__ double sb[kMgCoarseCells], sx[kMgCoarseCells], sd[kMgCoarseCells], swe[kMgCoarseCells],
-        swn[kMgCoarseCells], sr[1024];
+    __shared__ double sb[kMgCoarseCells], sx[kMgCoarseCells], sd[kMgCoarseCells], sdi[kMgCoarseCells],
+        swe[kMgCoarseCells], swn[kMgCoarseCells];
     const int lc = mg_coarse_first(h);
     const int nl = h.nlev;
     int off[kMgMaxLevels];
@@ -220,7 +228,9 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
         const MgLevel& L = h.lv[l];
         const int n = L.nx * L.ny;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            sd[off[l] + t] = L.d[t];
+            const double d = L.d[t];
+            sd[off[l] + t] = d;
+            sdi[off[l] + t] = 1.0 / d;
             swe[off[l] + t] = L.we[t];
             swn[off[l] + t] = L.wn[t];
             sb[off[l] + t] = (l == lc) ? L.b[t] : 0.0;
@@ -228,6 +238,13 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
         }
     }
     __syncthreads();
+    // A x of cell (i, j) at flat LDS index f of a level nx x ny
+    auto ax = [&](int f, int i, int j, int nx, int ny) -> double {
+        const bool e = i + 1 < nx, w = i > 0, nn = j + 1 < ny, so = j > 0;
+        const int fe = f + (e ? 1 : 0), fw = f - (w ? 1 : 0), fn = f + (nn ? nx : 0), fs = f - (so ? nx : 0);
+        const double ce = e ? swe[f] : 0.0, cw = w ? swe[fw] : 0.0, cn = nn ? swn[f] : 0.0, cs = so ? swn[fs] : 0.0;
+        return sd[f] * sx[f] - ce * sx[fe] - cw * sx[fw] - cn * sx[fn] - cs * sx[fs];
+    };
     auto half = [&](int l, int color) {   // cells of the colour only: row j holds i = (color - j) & 1, +2, ...
         const int nx = h.lv[l].nx, ny = h.lv[l].ny, o = off[l];
         const int hw = (nx + 1) / 2;
@@ -235,37 +252,29 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
             const int j = u / hw;
             const int i = ((color - j) & 1) + 2 * (u - j * hw);
             if (i >= nx) continue;
-            const int t = j * nx + i;
-            double s = sb[o + t];
-            if (i + 1 < nx) s += swe[o + t] * sx[o + t + 1];
-            if (i > 0) s += swe[o + t - 1] * sx[o + t - 1];
-            if (j + 1 < ny) s += swn[o + t] * sx[o + t + nx];
-            if (j > 0) s += swn[o + t - nx] * sx[o + t - nx];
-            sx[o + t] = s / sd[o + t];
+            const int f = o + j * nx + i;
+            const bool e = i + 1 < nx, w = i > 0, nn = j + 1 < ny, so = j > 0;
+            const int fe = f + (e ? 1 : 0), fw = f - (w ? 1 : 0), fn = f + (nn ? nx : 0), fs = f - (so ? nx : 0);
+            const double ce = e ? swe[f] : 0.0, cw = w ? swe[fw] : 0.0, cn = nn ? swn[f] : 0.0, cs = so ? swn[fs] : 0.0;
+            const double s = sb[f] + ce * sx[fe] + cw * sx[fw] + cn * sx[fn] + cs * sx[fs];
+            sx[f] = s * sdi[f];
         }
         __syncthreads();
     };
     for (int l = lc; l < nl - 1; ++l) {
         half(l, 0); half(l, 1); half(l, 0); half(l, 1);
         const int nx = h.lv[l].nx, ny = h.lv[l].ny, o = off[l];
-        for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
-            const int j = t / nx, i = t - j * nx;
-            double Ax = sd[o + t] * sx[o + t];
-            if (i + 1 < nx) Ax -= swe[o + t] * sx[o + t + 1];
-            if (i > 0) Ax -= swe[o + t - 1] * sx[o + t - 1];
-            if (j + 1 < ny) Ax -= swn[o + t] * sx[o + t + nx];
-            if (j > 0) Ax -= swn[o + t - nx] * sx[o + t - nx];
-            sr[t] = sb[o + t] - Ax;
-        }
-        __syncthreads();
         const int cnx = h.lv[l + 1].nx, cny = h.lv[l + 1].ny, co = off[l + 1];
-        for (int t = threadIdx.x; t < cnx * cny; t += blockDim.x) {
+        for (int t = threadIdx.x; t < cnx * cny; t += blockDim.x) {   // b_coarse = P^T (b - A x)
             const int J = t / cnx, I = t - J * cnx;
             double s = 0.0;
             for (int dj = 0; dj < 2; ++dj)
                 for (int di = 0; di < 2; ++di) {
                     const int i = 2 * I + di, j = 2 * J + dj;
-                    if (i < nx && j < ny) s += sr[j * nx + i];
+                    if (i < nx && j < ny) {
+                        const int f = o + j * nx + i;
+                        s += sb[f] - ax(f, i, j, nx, ny);
+                    }
                 }
             sb[co + t] = s;
         }
@@ -273,7 +282,7 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
     }
     {
         const int o = off[nl - 1], n = h.lv[nl - 1].nx * h.lv[nl - 1].ny;   // 1 x 1: exact
-        for (int t = threadIdx.x; t < n; t += blockDim.x) sx[o + t] = sb[o + t] / sd[o + t];
+        for (int t = threadIdx.x; t < n; t += blockDim.x) sx[o + t] = sb[o + t] * sdi[o + t];
         __syncthreads();
     }
     for (int l = nl - 2; l >= lc; --l) {
