@@ -49,14 +49,21 @@ __device__ __forceinline__ void vg_eval(double psi, double Ks, double alpha, dou
         return;
     }
     const double m = (n - 1.0) / n;
-    const double lx = log(alpha * (-psi));
+    const double x = alpha * (-psi);
+    const double lx = log(x);
     const double t = exp(n * lx);
     const double l1t = log1p(t);
     const double Se = exp(-m * l1t);
     theta = (ts - tr) * Se + tr;
-    const double w = (t > 0.0) ? -expm1(-m * log1p(1.0 / t)) : 1.0;
+    // log1p(1/t) = log1p(t) - log(t); the difference loses ~log(t) t ulps of log1p(1/t), so it
+    // is used only for t < 64 (relative error < 1e-13), the direct form beyond
+    const double l1it = (t < 64.0) ? l1t - n * lx : log1p(1.0 / t);
+    const double w = (t > 0.0) ? -expm1(-m * l1it) : 1.0;
     K = Ks * sqrt(Se) * w * w;
-    C = (ts - tr) * (n - 1.0) * alpha * exp((n - 1.0) * lx + (1.0 / n - 2.0) * l1t);
+    // dtheta/dpsi = (ts - tr) (n - 1) alpha x^(n-1) (1 + t)^(1/n - 2), with x^(n-1) = t / x and
+    // (1 + t)^(1/n - 2) = Se / (1 + t) since Se = (1 + t)^(1/n - 1)
+    // (t overflowing to infinity -- |psi| beyond any soil -- gives C = 0, the limit)
+    C = (t < 1e300) ? (ts - tr) * (n - 1.0) * alpha * t * Se / (x * (1.0 + t)) : 0.0;
 }
 
 // vanGenuchtenInverse, src/global.cu:32-52 (SI).
