@@ -146,3 +146,20 @@ def test_column_autotuned_tiles(hip, orc):
     w.run(gm, 3)
     w.run(om, 3)
     assert np.max(np.abs(gm.download("PSI") - om.download("PSI"))) <= 1e-6
+
+
+def test_column_huge_surface_fallbacks(hip):
+    """A surface with more 32 x 8 column tiles than partial slots (1056 x 1000: 4125 > 4096):
+    the SpMV and assembly run untiled, the mixed reliable update falls back to the separate FP64
+    residual and column start-up, the one-kernel Jacobi PCG caps its grid; all three agree."""
+    w = inputs.c1_column(nx=1056, ny=1000, nz=2)
+    col = model(hip, w, _abi.PRECOND_COLUMN)
+    mix = model(hip, w, _abi.PRECOND_COLUMN, precision=_abi.MIXED)
+    jac = model(hip, w, _abi.PRECOND_JACOBI)
+    assert col.get_tile(8) == (0, 0, 0)
+    sc, sm, sj = col.step(1)[0], mix.step(1)[0], jac.step(1)[0]
+    print("lin iters column / mixed / jacobi:", sc["lin_iters"], sm["lin_iters"], sj["lin_iters"], flush=True)
+    assert sc["lin_converged"] and sm["lin_converged"] and sj["lin_converged"]
+    pc, pm, pj = col.download("PSI"), mix.download("PSI"), jac.download("PSI")
+    assert np.max(np.abs(pc - pj)) <= 1e-8
+    assert np.max(np.abs(pm - pc)) <= 1e-6
