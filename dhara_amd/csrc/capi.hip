@@ -796,6 +796,8 @@ int ensure_mg(dhara_ctx* c)
             return rc;
         }
     }
+    c->mg.le = dh::mg_exact_level(c->mg);
+    if ((rc = dalloc(c, &c->mg.inv, (size_t)dh::kMgExactCells * dh::kMgExactCells))) return rc;
     c->mg_ready = true;
     return DHARA_OK;
 }

@@ -5,9 +5,10 @@
 // orders of magnitude between dry faces (K0) and ponded ones; Jacobi-preconditioned CG needs
 // ~1500 iterations per step on a 400 x 400 surface.  This preconditioner is one V-cycle of
 // unsmoothed-aggregation multigrid:
-//   * levels: 2 x 2 aggregates down to 1 x 1, Galerkin coarse operators P^T A P with P
-//     piecewise constant -- on a structured grid these stay 5-point (d, we, wn), so every
-//     level uses the same kernels;
+//   * levels: 2 x 2 aggregates, Galerkin coarse operators P^T A P with P piecewise constant --
+//     on a structured grid these stay 5-point (d, we, wn), so every level uses the same kernels
+//     -- down to the first level of at most 64 cells, which is solved exactly with its dense
+//     inverse (formed once per overland step);
 //   * smoothing: two red-black Gauss-Seidel sweeps before (red first) and after (black first)
 //     the coarse correction, which keeps the V-cycle symmetric, so CG stays valid; the cell
 //     update multiplies by the reciprocal of the diagonal (held in LDS);
@@ -216,9 +217,9 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
 {
     if (st && st->done) return;
     __shared__ double sb[kMgCoarseCells], sx[kMgCoarseCells], sd[kMgCoarseCells], sdi[kMgCoarseCells],
-        swe[kMgCoarseCells], swn[kMgCoarseCells];
+        swe[kMgCoarseCells], swn[kMgCoarseCells], sinv[kMgExactCells * kMgExactCells];
     const int lc = mg_coarse_first(h);
-    const int nl = h.nlev;
+    const int nl = h.le + 1;   // levels lc .. le; le is solved exactly
     int off[kMgMaxLevels];
     {
         int o = 0;
@@ -236,6 +237,10 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
             sb[off[l] + t] = (l == lc) ? L.b[t] : 0.0;
             sx[off[l] + t] = 0.0;
         }
+    }
+    {
+        const int ne = h.lv[h.le].nx * h.lv[h.le].ny;
+        for (int t = threadIdx.x; t < ne * ne; t += blockDim.x) sinv[t] = h.inv[t];
     }
     __syncthreads();
     // A x of cell (i, j) at flat LDS index f of a level nx x ny
@@ -281,8 +286,12 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
         __syncthreads();
     }
     {
-        const int o = off[nl - 1], n = h.lv[nl - 1].nx * h.lv[nl - 1].ny;   // 1 x 1: exact
-        for (int t = threadIdx.x; t < n; t += blockDim.x) sx[o + t] = sb[o + t] * sdi[o + t];
+        const int o = off[nl - 1], n = h.lv[nl - 1].nx * h.lv[nl - 1].ny;   // level le: exact
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            double v = 0.0;
+            for (int j = 0; j < n; ++j) v += sinv[t * n + j] * sb[o + j];
+            sx[o + t] = v;
+        }
         __syncthreads();
     }
     for (int l = nl - 2; l >= lc; --l) {
@@ -306,6 +315,48 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
     }
     acc[3] = acc[1];
     if (part) block_partials<4, false>(acc, part);
+}
+
+// Dense inverse of level le's operator (at most 64 cells) by Gauss-Jordan elimination in LDS;
+// the operator is symmetric positive definite and diagonally dominant, so no pivoting.
+__global__ __launch_bounds__(kBlock) void k_mg_inverse(MgHier h)
+{
+    constexpr int NE = kMgExactCells, PER = NE * NE / kBlock;   // row pitch NE: i = e >> 6, j = e & 63
+    static_assert(NE == 64, "index arithmetic assumes a 64-wide pitch");
+    __shared__ double M[NE * NE];
+    const MgLevel L = h.lv[h.le];
+    const int nx = L.nx, n = L.nx * L.ny;
+    for (int e = threadIdx.x; e < NE * NE; e += blockDim.x) {
+        const int i = e >> 6, j = e & 63;
+        double v = 0.0;
+        if (i < n && j < n) {
+            if (i == j) v = L.d[i];
+            else if (j == i + 1 && i % nx != nx - 1) v = -L.we[i];
+            else if (i == j + 1 && j % nx != nx - 1) v = -L.we[j];
+            else if (j == i + nx) v = -L.wn[i];
+            else if (i == j + nx) v = -L.wn[j];
+        }
+        M[e] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double ip = 1.0 / M[k * NE + k];
+        double nv[PER];
+#pragma unroll
+        for (int c = 0; c < PER; ++c) {
+            const int e = threadIdx.x + c * kBlock, i = e >> 6, j = e & 63;
+            const double rk = (j == k) ? ip : M[k * NE + j] * ip;   // row k, scaled
+            nv[c] = (i == k) ? rk : ((j == k) ? -M[i * NE + k] * ip : M[e] - M[i * NE + k] * rk);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < PER; ++c) M[threadIdx.x + c * kBlock] = nv[c];
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - i * n;
+        h.inv[e] = M[i * NE + j];
+    }
 }
 
 // The update of MgUpd as a kernel of its own (surfaces handled entirely by the coarse kernel).
@@ -337,6 +388,13 @@ int mg_levels(int nx, int ny, int dims[][2])
     return n;
 }
 
+int mg_exact_level(const MgHier& h)
+{
+    int l = mg_coarse_level(h);
+    while (l < h.nlev - 1 && h.lv[l].nx * h.lv[l].ny > kMgExactCells) ++l;
+    return l;
+}
+
 int mg_coarse_level(const MgHier& h)
 {
     int l = 0;
@@ -353,10 +411,11 @@ int mg_partials(const MgHier& h)
 void launch_mg_setup(hipStream_t s, const Geo2& g, const double* ke, const double* kn, const MgHier& h)
 {
     hipLaunchKernelGGL(k_mg_fine, dim3(grid_for(g.n)), dim3(kBlock), 0, s, g, ke, kn, h.lv[0].we, h.lv[0].wn);
-    for (int l = 0; l + 1 < h.nlev; ++l) {
+    for (int l = 0; l < h.le; ++l) {
         const int nc = h.lv[l + 1].nx * h.lv[l + 1].ny;
         hipLaunchKernelGGL(k_mg_coarsen, dim3(grid_for(nc)), dim3(kBlock), 0, s, h.lv[l], h.lv[l + 1]);
     }
+    hipLaunchKernelGGL(k_mg_inverse, dim3(1), dim3(kBlock), 0, s, h);
 }
 
 void launch_mg_vcycle(hipStream_t s, const MgHier& h, const PcgState* st, double* part, const double* bb_src)

@@ -1,9 +1,10 @@
 """Overland multigrid preconditioner on the GPU.
 
 * One V-cycle against a numpy restatement of the same algorithm (2 x 2 aggregation, Galerkin
-  coarse operators down to 1 x 1, two red-black Gauss-Seidel sweeps before and after, coarse
-  correction x 1.5) on odd surface sizes that exercise the tiled large-level kernels, partial
-  tiles, odd aggregates and the single-block coarse kernel.
+  coarse operators down to the first level of at most 64 cells, solved exactly, two red-black
+  Gauss-Seidel sweeps before and after, coarse correction x 1.5) on odd surface sizes that
+  exercise the tiled large-level kernels, partial tiles, odd aggregates and the single-block
+  coarse kernel.
 * Coupled runs with the multigrid and with Jacobi preconditioning give the same ponding depth
   as the oracle, and the multigrid needs far fewer overland iterations.
 """
@@ -59,10 +60,26 @@ def rbgs(d, we, wn, shape, x, b, color):
     return x
 
 
+def exact_level(levels):
+    """The level the V-cycle solves exactly: the first one of at most 64 cells at or below the
+    first level of at most 32 x 32 (the single-block kernel's range)."""
+    l = 0
+    while l < len(levels) - 1 and not (levels[l][3][0] <= 32 and levels[l][3][1] <= 32):
+        l += 1
+    while l < len(levels) - 1 and levels[l][3][0] * levels[l][3][1] > 64:
+        l += 1
+    return l
+
+
+def dense(d, we, wn, shape):
+    n = d.size
+    return np.array([op(d, we, wn, np.eye(n)[k], shape) for k in range(n)]).T
+
+
 def vcycle(levels, l, b):
     d, we, wn, shape = levels[l]
-    if l == len(levels) - 1:
-        return b / d
+    if l == exact_level(levels):
+        return np.linalg.solve(dense(d, we, wn, shape), b)
     x = np.zeros_like(b)
     for col in (0, 1, 0, 1):
         x = rbgs(d, we, wn, shape, x, b, col)
