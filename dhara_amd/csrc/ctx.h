@@ -137,13 +137,14 @@ void free_adjoint(dhara_ctx* c);   // adjoint.hip
 // only the launches that did work (the first ones of the batch) are accumulated.
 template <class SpmvFn, class UpdFn>
 int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
-               SpmvFn spmv, UpdFn upd, int it_start = 0)
+               SpmvFn spmv, UpdFn upd, int it_start = 0, int first_batch = 0)
 {
     int it = it_start;
     // first batch: the previous solve's count + the launch that detects the end (the next
-    // solve is usually within +-2 of it); then small batches, since an over-queued iteration costs two empty launches and an
+    // solve is usually within +-2 of it), or first_batch when the caller knows better; then
+    // small batches, since an over-queued iteration costs two empty launches and an
     // under-queued one a host round trip
-    int batch = std::max(2, std::min(kMaxBatch, hint + 1));
+    int batch = first_batch > 0 ? first_batch : std::max(2, std::min(kMaxBatch, hint + 1));
     for (;;) {
         const int it0 = it;
         for (int b = 0; b < batch; ++b, ++it) {
@@ -178,7 +179,7 @@ int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, 
         if (it > maxit + 1) return set_err(c, DHARA_E_STATE, "Krylov loop did not terminate");
         batch = 3;
     }
-    hint = std::max(1, c->h_st->iter - it_start);
+    if (first_batch <= 0) hint = std::max(1, c->h_st->iter - it_start);
     return DHARA_OK;
 }
 

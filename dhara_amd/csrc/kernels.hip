@@ -229,6 +229,8 @@ __device__ __forceinline__ void b_assemble(AsmArgs a, Blk bk)
     const Geo g = a.g;
     const double dx = g.dx, dy = g.dy, dz = g.dz, dt = a.dt;
     const double dx2inv = -1.0 / (dx * dx), dy2inv = -1.0 / (dy * dy), dz2inv = -1.0 / (dz * dz);
+    // reciprocals of the step constants: one product per use instead of a division per cell
+    const double idt = 1.0 / dt, idz = 1.0 / dz, sst = a.Ss * idt / a.poros;
     const double* __restrict__ K = a.K;
     const double* __restrict__ x = a.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz, rr (solver precision), bb, rr (FP64)
@@ -245,16 +247,16 @@ __device__ __forceinline__ void b_assemble(AsmArgs a, Blk bk)
         const double ke = (i == g.nx - 1) ? 0.0 : 0.5 * (Kc + K[c + 1]);
         const double thm = a.theta_m[c];
         const double Cc = a.C[c];
-        double D = a.Ss / dt * thm / a.poros + Cc / dt - (ke + kw) * dx2inv - (kn + ks) * dy2inv -
+        double D = sst * thm + Cc * idt - (ke + kw) * dx2inv - (kn + ks) * dy2inv -
                    (ku + kd) * dz2inv;
         const int id2d = j * g.nx + i;
         double sink = 0.0;
         if (a.has_transp) {
             const double tr = a.transp[id2d];
-            if (tr > 0.0) sink = -tr * a.root[a.procmap[id2d] * g.nz + k] / dz;
+            if (tr > 0.0) sink = -tr * a.root[a.procmap[id2d] * g.nz + k] * idz;
         }
-        double b = a.Ss / dt * thm / a.poros * a.psi_n[c] + Cc / dt * a.psi_m[c] - (ku - kd) / dz +
-                   (a.theta_n[c] - thm) / dt + sink;
+        double b = sst * thm * a.psi_n[c] + Cc * idt * a.psi_m[c] - (ku - kd) * idz + (a.theta_n[c] - thm) * idt +
+                   sink;
         const int bi = k * g.ny + j, bj = k * g.nx + i, bk = j * g.nx + i;
         bool ident = false;
         // top
@@ -377,6 +379,7 @@ __global__ __launch_bounds__(TX * TY) void k_assemble_tiled(AsmArgs a, int kc, i
     const double* __restrict__ K = a.K;
     const double dx = g.dx, dy = g.dy, dz = g.dz, dt = a.dt;
     const double dx2inv = -1.0 / (dx * dx), dy2inv = -1.0 / (dy * dy), dz2inv = -1.0 / (dz * dz);
+    const double idt = 1.0 / dt, idz = 1.0 / dz, sst = a.Ss * idt / a.poros;
 
     double xm = 0.0, km = 0.0, xc = 0.0, kcur = 0.0;
     if (in) {
@@ -412,12 +415,12 @@ __global__ __launch_bounds__(TX * TY) void k_assemble_tiled(AsmArgs a, int kc, i
             const double ke = (i == g.nx - 1) ? 0.0 : 0.5 * (Kc + sk[buf][ty + 1][tx + 2]);
             const double thm = a.theta_m[c];
             const double Cc = a.C[c];
-            const double D = a.Ss / dt * thm / a.poros + Cc / dt - (ke + kw) * dx2inv - (kno + ks) * dy2inv -
+            const double D = sst * thm + Cc * idt - (ke + kw) * dx2inv - (kno + ks) * dy2inv -
                              (ku + kd) * dz2inv;
             double sink = 0.0;
-            if (tr > 0.0) sink = -tr * a.root[rgrp * g.nz + k] / dz;
-            double b = a.Ss / dt * thm / a.poros * a.psi_n[c] + Cc / dt * a.psi_m[c] - (ku - kd) / dz +
-                       (a.theta_n[c] - thm) / dt + sink;
+            if (tr > 0.0) sink = -tr * a.root[rgrp * g.nz + k] * idz;
+            double b = sst * thm * a.psi_n[c] + Cc * idt * a.psi_m[c] - (ku - kd) * idz +
+                       (a.theta_n[c] - thm) * idt + sink;
             const int bi = k * g.ny + j, bj = k * g.nx + i;
             if (k == 0) b += (-a.bcq[FT][col] * dz) * dz2inv;
             if (k == g.nz - 1) {
