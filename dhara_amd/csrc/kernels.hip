@@ -793,9 +793,19 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
 
     auto xnew = [&](int c) { return x[c]; };
     double xm = 0.0, km = 0.0, xc = 0.0, kcur = 0.0;
+    // the plane's D, b and halo values are loaded one plane ahead, with the next plane's x and K:
+    // a block has a whole column per thread (few waves per SIMD), so each plane would otherwise
+    // wait a full memory latency after its barrier
+    double Dc = 1.0, bc = 0.0, hxc = 0.0, hkc = 0.0;
     if (in) {
         xc = xnew(col);
         kcur = K[col];
+        Dc = a.D[col];
+        bc = a.b[col];
+    }
+    if (has_halo) {
+        hxc = xnew(hcol);
+        hkc = K[hcol];
     }
     float y[NZM], cp[NZM];
     float kprev = 0.0f, yprev = 0.0f, inv_prev = 0.0f;
@@ -803,22 +813,28 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
 #pragma unroll
     for (int k = 0; k < NZM; ++k) {
         if (k < nz) {
-            double xn = 0.0, kn = 0.0;
+            double xn = 0.0, kn = 0.0, Dn = 1.0, bn = 0.0, hxn = 0.0, hkn = 0.0;
             if (in && k + 1 < nz) {
-                xn = xnew((k + 1) * nxy + col);
-                kn = K[(k + 1) * nxy + col];
+                const int cn = (k + 1) * nxy + col;
+                xn = xnew(cn);
+                kn = K[cn];
+                Dn = a.D[cn];
+                bn = a.b[cn];
+            }
+            if (has_halo && k + 1 < nz) {
+                hxn = xnew((k + 1) * nxy + hcol);
+                hkn = K[(k + 1) * nxy + hcol];
             }
             const int buf = k & 1;
             sx[buf][ty + 1][tx + 1] = xc;
             sk[buf][ty + 1][tx + 1] = kcur;
             if (has_halo) {
-                sx[buf][hy + 1][hx + 1] = xnew(k * nxy + hcol);
-                sk[buf][hy + 1][hx + 1] = K[k * nxy + hcol];
+                sx[buf][hy + 1][hx + 1] = hxc;
+                sk[buf][hy + 1][hx + 1] = hkc;
             }
             __syncthreads();
             if (in) {
                 const int c = k * nxy + col;
-                const double Dc = a.D[c];
                 double ax = Dc * xc;
                 if (k > 0) ax -= wz * (half * (kcur + km)) * xm;
                 if (j > 0) ax -= wy * (half * (kcur + sk[buf][ty][tx + 1])) * sx[buf][ty][tx + 1];
@@ -826,7 +842,7 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
                 if (i < g.nx - 1) ax -= wx * (half * (kcur + sk[buf][ty + 1][tx + 2])) * sx[buf][ty + 1][tx + 2];
                 if (j < g.ny - 1) ax -= wy * (half * (kcur + sk[buf][ty + 2][tx + 1])) * sx[buf][ty + 2][tx + 1];
                 if (k < nz - 1) ax -= wz * (half * (kcur + kn)) * xn;
-                const double r = a.b[c] - ax;
+                const double r = bc - ax;
                 const float r32 = (float)r;
                 a.r32[c] = r32;
                 a.d32[c] = 0.0f;
@@ -850,7 +866,7 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
                 yprev = yk;
                 inv_prev = inv;
             }
-            xm = xc; km = kcur; xc = xn; kcur = kn;
+            xm = xc; km = kcur; xc = xn; kcur = kn; Dc = Dn; bc = bn; hxc = hxn; hkc = hkn;
         }
     }
     if (in) {
