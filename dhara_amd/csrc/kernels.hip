@@ -392,16 +392,27 @@ __global__ __launch_bounds__(TX * TY) void k_assemble_tiled(AsmArgs a, int kc, i
         tr = a.transp[col];
         rgrp = a.procmap[col];
     }
+    // the plane's cell fields and halo values are loaded one plane ahead, with the next plane's
+    // x and K, so that a plane does not wait a memory latency after its barrier
+    struct Cell { double thm, C, psi_n, psi_m, theta_n; };
+    auto load_cell = [&](int c) { return Cell{a.theta_m[c], a.C[c], a.psi_n[c], a.psi_m[c], a.theta_n[c]}; };
+    Cell cc{0.0, 0.0, 0.0, 0.0, 0.0};
+    double hxc = 0.0, hkc = 0.0;
+    if (in && k0 < k1) cc = load_cell(k0 * g.nxy + col);
+    if (has_halo && k0 < k1) { hxc = x[k0 * g.nxy + hcol]; hkc = K[k0 * g.nxy + hcol]; }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};   // rz, rr (solver precision), bb, rr (FP64)
     for (int k = k0; k < k1; ++k) {
-        double xn = 0.0, kn = 0.0;
+        double xn = 0.0, kn = 0.0, hxn = 0.0, hkn = 0.0;
+        Cell cn{0.0, 0.0, 0.0, 0.0, 0.0};
         if (in && k + 1 < g.nz) { xn = x[(k + 1) * g.nxy + col]; kn = K[(k + 1) * g.nxy + col]; }
+        if (in && k + 1 < k1) cn = load_cell((k + 1) * g.nxy + col);
+        if (has_halo && k + 1 < k1) { hxn = x[(k + 1) * g.nxy + hcol]; hkn = K[(k + 1) * g.nxy + hcol]; }
         const int buf = k & 1;
         sx[buf][ty + 1][tx + 1] = xc;
         sk[buf][ty + 1][tx + 1] = kcur;
         if (has_halo) {
-            sx[buf][hy + 1][hx + 1] = x[k * g.nxy + hcol];
-            sk[buf][hy + 1][hx + 1] = K[k * g.nxy + hcol];
+            sx[buf][hy + 1][hx + 1] = hxc;
+            sk[buf][hy + 1][hx + 1] = hkc;
         }
         __syncthreads();
         if (in) {
@@ -413,14 +424,13 @@ __global__ __launch_bounds__(TX * TY) void k_assemble_tiled(AsmArgs a, int kc, i
             const double kno = (j == g.ny - 1) ? 0.0 : 0.5 * (Kc + sk[buf][ty + 2][tx + 1]);
             const double kw = (i == 0) ? 0.0 : 0.5 * (Kc + sk[buf][ty + 1][tx]);
             const double ke = (i == g.nx - 1) ? 0.0 : 0.5 * (Kc + sk[buf][ty + 1][tx + 2]);
-            const double thm = a.theta_m[c];
-            const double Cc = a.C[c];
+            const double thm = cc.thm;
+            const double Cc = cc.C;
             const double D = sst * thm + Cc * idt - (ke + kw) * dx2inv - (kno + ks) * dy2inv -
                              (ku + kd) * dz2inv;
             double sink = 0.0;
             if (tr > 0.0) sink = -tr * a.root[rgrp * g.nz + k] * idz;
-            double b = sst * thm * a.psi_n[c] + Cc * idt * a.psi_m[c] - (ku - kd) * idz +
-                       (a.theta_n[c] - thm) * idt + sink;
+            double b = sst * thm * cc.psi_n + Cc * idt * cc.psi_m - (ku - kd) * idz + (cc.theta_n - thm) * idt + sink;
             const int bi = k * g.ny + j, bj = k * g.nx + i;
             if (k == 0) b += (-a.bcq[FT][col] * dz) * dz2inv;
             if (k == g.nz - 1) {
@@ -458,6 +468,7 @@ __global__ __launch_bounds__(TX * TY) void k_assemble_tiled(AsmArgs a, int kc, i
             }
         }
         xm = xc; km = kcur; xc = xn; kcur = kn;
+        cc = cn; hxc = hxn; hkc = hkn;
     }
     block_partials<4, false>(acc, a.part_out);
 }
