@@ -766,6 +766,118 @@ __global__ __launch_bounds__(kBlock) void k_line(Geo g, const T* __restrict__ K,
     b_line<T, NZM, INIT>(g, K, D, x, r, p, q, z, sc, hw_blk());
 }
 
+// SpMV kernel of the column-preconditioned PCG: p = z + beta p_old, q = A p, partial (p, q), on a
+// TX x TY tile of columns marching through kc planes with the p / K planes staged in LDS (one-cell
+// halo, double buffered, one barrier per plane).  Loads run ahead of their use so that a plane
+// does not wait for memory: the own column's raw z, p_old, K, D two planes ahead and the halo's
+// raw z, p_old, K one plane ahead are in flight while the current plane is computed.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_spmv_z(Geo g, const T* __restrict__ K, const T* __restrict__ D,
+                                                    const T* __restrict__ z, const T* __restrict__ pold,
+                                                    T* __restrict__ pnew, T* __restrict__ q, SolveCtl sc, int kc,
+                                                    int ntx, int nty, int nkz)
+{
+    double beta_d;
+    if (!pcg_begin(sc, beta_d, lin_bid())) return;
+    const bool first = sc.it == 0;
+    const T beta = (T)beta_d;
+    __shared__ T sp[2][TY + 2][TX + 2];
+    __shared__ T sk[2][TY + 2][TX + 2];
+    const int nb = ntx * nty * nkz;
+    const int t = xcd_tile(blockIdx.x, nb);
+    const int bz = t % nkz, bxy = t / nkz;
+    const int bx = bxy % ntx, by = bxy / ntx;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * TX + tx, j = by * TY + ty;
+    const int k0 = bz * kc, k1 = min(k0 + kc, g.nz);
+    const bool in = (i < g.nx) && (j < g.ny);
+    const int col = j * g.nx + i;
+    const int lid = ty * TX + tx;
+    int hx = -2, hy = -2;
+    if (lid < TX) { hx = lid; hy = -1; }
+    else if (lid < 2 * TX) { hx = lid - TX; hy = TY; }
+    else if (lid < 2 * TX + TY) { hx = -1; hy = lid - 2 * TX; }
+    else if (lid < 2 * TX + 2 * TY) { hx = TX; hy = lid - 2 * TX - TY; }
+    const int hi = bx * TX + hx, hj = by * TY + hy;
+    const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
+    const int hcol = hj * g.nx + hi;
+    const int nxy = g.nxy, nz = g.nz;
+    const T wx = (T)g.wx, wy = (T)g.wy, wz = (T)g.wz, half = T(0.5);
+    auto pv = [&](T zv, T pov) -> T { return first ? zv : zv + beta * pov; };
+
+    T pm = T(0), km = T(0), pc = T(0), kcur = T(0), dc = T(1), pn = T(0), kn = T(0), dn = T(1);
+    T z2 = T(0), p2 = T(0), k2 = T(0), d2 = T(1);
+    T hzc = T(0), hpc = T(0), hkc = T(0);
+    if (in) {
+        if (k0 > 0) {
+            const int c = (k0 - 1) * nxy + col;
+            pm = pv(z[c], pold[c]);
+            km = K[c];
+        }
+        if (k0 < k1) {
+            const int c = k0 * nxy + col;
+            pc = pv(z[c], pold[c]);
+            kcur = K[c];
+            dc = D[c];
+        }
+        if (k0 + 1 < nz) {
+            const int c = (k0 + 1) * nxy + col;
+            pn = pv(z[c], pold[c]);
+            kn = K[c];
+            dn = D[c];
+        }
+        if (k0 + 2 < nz) {
+            const int c = (k0 + 2) * nxy + col;
+            z2 = z[c]; p2 = pold[c]; k2 = K[c]; d2 = D[c];
+        }
+    }
+    if (has_halo && k0 < k1) {
+        const int c = k0 * nxy + hcol;
+        hzc = z[c]; hpc = pold[c]; hkc = K[c];
+    }
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        T z3 = T(0), p3 = T(0), k3 = T(0), d3 = T(1);
+        if (in && k + 3 < nz) {
+            const int c = (k + 3) * nxy + col;
+            z3 = z[c]; p3 = pold[c]; k3 = K[c]; d3 = D[c];
+        }
+        T hzn = T(0), hpn = T(0), hkn = T(0);
+        if (has_halo && k + 1 < k1) {
+            const int c = (k + 1) * nxy + hcol;
+            hzn = z[c]; hpn = pold[c]; hkn = K[c];
+        }
+        const int buf = k & 1;
+        sp[buf][ty + 1][tx + 1] = pc;
+        sk[buf][ty + 1][tx + 1] = kcur;
+        if (has_halo) {
+            sp[buf][hy + 1][hx + 1] = pv(hzc, hpc);
+            sk[buf][hy + 1][hx + 1] = hkc;
+        }
+        __syncthreads();
+        if (in) {
+            const int c = k * nxy + col;
+            T qv = dc * pc;
+            if (k > 0) qv -= wz * (half * (kcur + km)) * pm;
+            if (j > 0) qv -= wy * (half * (kcur + sk[buf][ty][tx + 1])) * sp[buf][ty][tx + 1];
+            if (i > 0) qv -= wx * (half * (kcur + sk[buf][ty + 1][tx])) * sp[buf][ty + 1][tx];
+            if (i < g.nx - 1) qv -= wx * (half * (kcur + sk[buf][ty + 1][tx + 2])) * sp[buf][ty + 1][tx + 2];
+            if (j < g.ny - 1) qv -= wy * (half * (kcur + sk[buf][ty + 2][tx + 1])) * sp[buf][ty + 2][tx + 1];
+            if (k < nz - 1) qv -= wz * (half * (kcur + kn)) * pn;
+            pnew[c] = pc;
+            q[c] = qv;
+            acc += (double)pc * (double)qv;
+        }
+        pm = pc; km = kcur;
+        pc = pn; kcur = kn; dc = dn;
+        pn = pv(z2, p2); kn = k2; dn = d2;
+        z2 = z3; p2 = p3; k2 = k3; d2 = d3;
+        hzc = hzn; hpc = hpn; hkc = hkn;
+    }
+    double v[1] = {acc};
+    block_partials<1, false>(v, sc.part_out);
+}
+
 // Reliable update of the mixed column-preconditioned PCG after k_refine_add (x += d): the true
 // residual r = b - A x in FP64, r32 = (float) r, d = 0, and the start of the next FP32 segment,
 // z32 = T32^-1 r32 with T32 the column operator of the FP32 solve ((float) K, (float) D, the
@@ -2115,8 +2227,7 @@ static void tiled_launch(hipStream_t s, const Geo& g, int kc, const uint8_t* fla
     const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, nkz = (g.nz + kc - 1) / kc;
     const dim3 gr(ntx * nty * nkz), bl(TX, TY);
     if constexpr (ZIN)
-        launch_ev(k_pcg3_spmv_tiled<T, false, TX, TY, false, true>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q,
-                  sc, kc, ntx, nty, nkz);
+        launch_ev(k_spmv_z<T, TX, TY>, gr, bl, s, ev, g, K, D, r, pold, pnew, q, sc, kc, ntx, nty, nkz);
     else if (flags)
         launch_ev(k_pcg3_spmv_tiled<T, true, TX, TY>, gr, bl, s, ev, g, flags, K, D, r, pold, pnew, q, sc, kc,
                   ntx, nty, nkz);
