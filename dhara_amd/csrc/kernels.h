@@ -379,7 +379,7 @@ struct MgHier {
     double* inv;     // kMgExactCells^2 doubles: (level le's operator)^-1, row-major
 };
 constexpr int kMgExactCells = 64;   // the V-cycle bottoms out on the first coarse level this small
-int mg_exact_level(const MgHier& h);   // that level (at or below the single-block kernel's first)
+int mg_exact_level(const MgHier& h);   // that level
 int mg_levels(int nx, int ny, int dims[][2]);   // level sizes down to 1 x 1; returns the count
 int mg_coarse_level(const MgHier& h);           // first level handled by the single-block kernel
 int mg_partials(const MgHier& h);               // partial sums the V-cycle's last kernel writes

@@ -17,7 +17,8 @@
 //     smooth) per level; a tile of T x T cells works in LDS with a halo of 5 (down) or 4 (up)
 //     cells recomputed redundantly, so four half-sweeps need no grid-wide synchronisation and
 //     the tile interior equals a global sweep exactly;
-//   * levels of at most 32 x 32 cells: the rest of the V-cycle in one single-block kernel.
+//   * the coarsest levels (as many as fit 3400 cells in LDS: 50 x 50 down to 7 x 7 on a 400 x 400
+//     surface): the rest of the V-cycle in one single-block kernel.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -29,16 +30,24 @@ constexpr int kMgT = 28;          // tile edge of the large-level kernels (even)
 constexpr int kMgHd = 5;          // halo: 4 half-sweeps + the residual
 constexpr int kMgHu = 4;          // halo: 4 half-sweeps
 constexpr int kMgThreads = 1024;   // one pass of a 38 x 38 tile's half-sweep per block
-constexpr int kMgCoarseCells = 1366;   // cells of all levels from 32 x 32 down to 1 x 1
+constexpr int kMgCoarseCells = 3400;   // LDS capacity of the single-block kernel, in cells of its levels
+
+// The first level handled by the single-block kernel: the finest whose cells, together with
+// those of every coarser level down to le, fit in its LDS (50 x 50 ... 7 x 7 on a 400 x 400
+// surface).  Host and device use the same rule.
+__host__ __device__ inline int mg_first_single_block(const MgHier& h)
+{
+    int l = h.le, cells = h.lv[l].nx * h.lv[l].ny;
+    while (l > 0 && cells + h.lv[l - 1].nx * h.lv[l - 1].ny <= kMgCoarseCells) {
+        --l;
+        cells += h.lv[l].nx * h.lv[l].ny;
+    }
+    return l;
+}
 
 namespace {
 
-__device__ __forceinline__ int mg_coarse_first(const MgHier& h)
-{
-    int l = 0;
-    while (l < h.nlev - 1 && !(h.lv[l].nx <= 32 && h.lv[l].ny <= 32)) ++l;
-    return l;
-}
+__device__ __forceinline__ int mg_coarse_first(const MgHier& h) { return mg_first_single_block(h); }
 
 // Level-0 operator: the eliminated overland matrix.  Diagonal = the assembled D (ring rows 1);
 // weights only between two interior cells (couplings to the ring moved to the right-hand side).
@@ -216,8 +225,8 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
                                                           const double* bb_src)
 {
     if (st && st->done) return;
-    __shared__ double sb[kMgCoarseCells], sx[kMgCoarseCells], sd[kMgCoarseCells], sdi[kMgCoarseCells],
-        swe[kMgCoarseCells], swn[kMgCoarseCells], sinv[kMgExactCells * kMgExactCells];
+    __shared__ double sb[kMgCoarseCells], sx[kMgCoarseCells], sdi[kMgCoarseCells], swe[kMgCoarseCells],
+        swn[kMgCoarseCells];
     const int lc = mg_coarse_first(h);
     const int nl = h.le + 1;   // levels lc .. le; le is solved exactly
     int off[kMgMaxLevels];
@@ -230,7 +239,6 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
         const int n = L.nx * L.ny;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const double d = L.d[t];
-            sd[off[l] + t] = d;
             sdi[off[l] + t] = 1.0 / d;
             swe[off[l] + t] = L.we[t];
             swn[off[l] + t] = L.wn[t];
@@ -238,17 +246,13 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
             sx[off[l] + t] = 0.0;
         }
     }
-    {
-        const int ne = h.lv[h.le].nx * h.lv[h.le].ny;
-        for (int t = threadIdx.x; t < ne * ne; t += blockDim.x) sinv[t] = h.inv[t];
-    }
     __syncthreads();
     // A x of cell (i, j) at flat LDS index f of a level nx x ny
     auto ax = [&](int f, int i, int j, int nx, int ny) -> double {
         const bool e = i + 1 < nx, w = i > 0, nn = j + 1 < ny, so = j > 0;
         const int fe = f + (e ? 1 : 0), fw = f - (w ? 1 : 0), fn = f + (nn ? nx : 0), fs = f - (so ? nx : 0);
         const double ce = e ? swe[f] : 0.0, cw = w ? swe[fw] : 0.0, cn = nn ? swn[f] : 0.0, cs = so ? swn[fs] : 0.0;
-        return sd[f] * sx[f] - ce * sx[fe] - cw * sx[fw] - cn * sx[fn] - cs * sx[fs];
+        return sx[f] / sdi[f] - ce * sx[fe] - cw * sx[fw] - cn * sx[fn] - cs * sx[fs];
     };
     auto half = [&](int l, int color) {   // cells of the colour only: row j holds i = (color - j) & 1, +2, ...
         const int nx = h.lv[l].nx, ny = h.lv[l].ny, o = off[l];
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_coarse(MgHier h, const PcgSta
         const int o = off[nl - 1], n = h.lv[nl - 1].nx * h.lv[nl - 1].ny;   // level le: exact
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             double v = 0.0;
-            for (int j = 0; j < n; ++j) v += sinv[t * n + j] * sb[o + j];
+            for (int j = 0; j < n; ++j) v += h.inv[t * n + j] * sb[o + j];
             sx[o + t] = v;
         }
         __syncthreads();
@@ -390,17 +394,12 @@ int mg_levels(int nx, int ny, int dims[][2])
 
 int mg_exact_level(const MgHier& h)
 {
-    int l = mg_coarse_level(h);
+    int l = 0;
     while (l < h.nlev - 1 && h.lv[l].nx * h.lv[l].ny > kMgExactCells) ++l;
     return l;
 }
 
-int mg_coarse_level(const MgHier& h)
-{
-    int l = 0;
-    while (l < h.nlev - 1 && !(h.lv[l].nx <= 32 && h.lv[l].ny <= 32)) ++l;
-    return l;
-}
+int mg_coarse_level(const MgHier& h) { return mg_first_single_block(h); }
 
 int mg_partials(const MgHier& h)
 {

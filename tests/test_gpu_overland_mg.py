@@ -61,11 +61,8 @@ def rbgs(d, we, wn, shape, x, b, color):
 
 
 def exact_level(levels):
-    """The level the V-cycle solves exactly: the first one of at most 64 cells at or below the
-    first level of at most 32 x 32 (the single-block kernel's range)."""
+    """The level the V-cycle solves exactly: the first one of at most 64 cells."""
     l = 0
-    while l < len(levels) - 1 and not (levels[l][3][0] <= 32 and levels[l][3][1] <= 32):
-        l += 1
     while l < len(levels) - 1 and levels[l][3][0] * levels[l][3][1] > 64:
         l += 1
     return l
