@@ -699,51 +699,94 @@ __device__ __forceinline__ void b_line(Geo g, const T* __restrict__ K, const T* 
     const int nz = g.nz, nxy = g.nxy;
     double acc[INIT ? 4 : 2] = {};
     constexpr bool kCpInZ = sizeof(T) == 8;
+    // The start-up loads its planes in groups of G: the loads of a group are issued together and
+    // the group is then eliminated plane by plane, so a column pays one memory latency per group
+    // instead of one per plane (one column per thread leaves ~2.4 waves per SIMD on a 400 x 400
+    // surface, too few to hide a latency per plane); its FP64 back substitution reads the stored
+    // factors c'_k in groups the same way.  The update streams twice the arrays and runs at 0.7
+    // of the HBM peak plane by plane; grouped, its 48 plane streams per wave measured slower.
+    constexpr int G = INIT ? (sizeof(T) == 4 ? 8 : 4) : 1;
     for (int col = bk.id * kBlock + threadIdx.x; col < nxy; col += bk.n * kBlock) {
         T y[NZM], cp[kCpInZ ? 1 : NZM];
         T kprev = T(0), yprev = T(0), inv_prev = T(0);
 #pragma unroll
-        for (int k = 0; k < NZM; ++k) {
-            if (k < nz) {
-                const int c = k * nxy + col;
-                T rc = r[c];
-                if constexpr (!INIT) {
-                    rc -= alpha * q[c];
-                    r[c] = rc;
-                    x[c] += alpha * p[c];
+        for (int k0 = 0; k0 < NZM; k0 += G) {
+            if (k0 < nz) {
+                T rb[G], qb[G], pb[G], xb[G], kb[G], db[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const int c = (k0 + u) * nxy + col;
+                    if (G > 1 && k0 + u < nz) {
+                        rb[u] = r[c];
+                        if constexpr (!INIT) {
+                            qb[u] = q[c];
+                            pb[u] = p[c];
+                            xb[u] = x[c];
+                        }
+                        kb[u] = K[c];
+                        db[u] = D[c];
+                    }
                 }
-                const T kc = K[c];
-                T den = D[c], w = rc;
-                if (k > 0) {
-                    const T a = -hz * (kc + kprev);   // T[k][k-1] = T[k-1][k]
-                    const T cpk = a * inv_prev;       // c'_{k-1}
-                    if constexpr (kCpInZ) z[c - nxy] = cpk;
-                    else cp[k - 1] = cpk;
-                    den -= a * cpk;
-                    w -= a * yprev;
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const int k = k0 + u;
+                    if (k < nz) {
+                        const int c = k * nxy + col;
+                        // (G == 1: operands read where they are used, the plane-by-plane order)
+                        T rc = G > 1 ? rb[u] : r[c];
+                        if constexpr (!INIT) {
+                            rc -= alpha * (G > 1 ? qb[u] : q[c]);
+                            r[c] = rc;
+                            x[c] = (G > 1 ? xb[u] : x[c]) + alpha * (G > 1 ? pb[u] : p[c]);
+                        }
+                        const T kc = G > 1 ? kb[u] : K[c];
+                        T den = G > 1 ? db[u] : D[c], w = rc;
+                        if (k > 0) {
+                            const T a = -hz * (kc + kprev);   // T[k][k-1] = T[k-1][k]
+                            const T cpk = a * inv_prev;       // c'_{k-1}
+                            if constexpr (kCpInZ) z[c - nxy] = cpk;
+                            else cp[k - 1] = cpk;
+                            den -= a * cpk;
+                            w -= a * yprev;
+                        }
+                        const T inv = jac_inv(den);
+                        const T yk = w * inv;
+                        y[k] = yk;
+                        acc[0] += (double)w * (double)yk;
+                        acc[1] += (double)rc * (double)rc;
+                        kprev = kc;
+                        yprev = yk;
+                        inv_prev = inv;
+                    }
                 }
-                const T inv = jac_inv(den);
-                const T yk = w * inv;
-                y[k] = yk;
-                acc[0] += (double)w * (double)yk;
-                acc[1] += (double)rc * (double)rc;
-                kprev = kc;
-                yprev = yk;
-                inv_prev = inv;
             }
         }
         T zn = T(0);
 #pragma unroll
-        for (int k = NZM - 1; k >= 0; --k) {
-            if (k < nz) {
-                const int c = k * nxy + col;
-                T zk = y[k];
-                if (k < nz - 1) {
-                    if constexpr (kCpInZ) zk -= z[c] * zn;
-                    else zk -= cp[k] * zn;
+        for (int k1 = NZM - 1; k1 >= 0; k1 -= G) {
+            if (k1 - G + 1 < nz) {
+                T cb[G];
+                if constexpr (kCpInZ && G > 1) {
+#pragma unroll
+                    for (int u = 0; u < G; ++u) {
+                        const int k = k1 - u;
+                        if (k < nz - 1) cb[u] = z[k * nxy + col];
+                    }
                 }
-                z[c] = zk;
-                zn = zk;
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const int k = k1 - u;
+                    if (k < nz) {
+                        const int c = k * nxy + col;
+                        T zk = y[k];
+                        if (k < nz - 1) {
+                            if constexpr (kCpInZ) zk -= (G > 1 ? cb[u] : z[c]) * zn;
+                            else zk -= cp[k] * zn;
+                        }
+                        z[c] = zk;
+                        zn = zk;
+                    }
+                }
             }
         }
     }
@@ -914,20 +957,24 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
     const float hz = (float)(0.5 * g.wz);
     const int nz = g.nz, nxy = g.nxy;
 
-    auto xnew = [&](int c) { return x[c]; };
-    double xm = 0.0, km = 0.0, xc = 0.0, kcur = 0.0;
-    // the plane's D, b and halo values are loaded one plane ahead, with the next plane's x and K:
-    // a block has a whole column per thread (few waves per SIMD), so each plane would otherwise
+    double xm = 0.0, km = 0.0, xc = 0.0, kcur = 0.0, xn = 0.0, kn = 0.0;
+    // loads run ahead of their use: the own column's x and K two planes ahead (plane k + 1 is
+    // already in the stencil of plane k), its D and b and the halo's x and K one plane ahead.  A
+    // block has a whole column per thread (few waves per SIMD), so each plane would otherwise
     // wait a full memory latency after its barrier
     double Dc = 1.0, bc = 0.0, hxc = 0.0, hkc = 0.0;
     if (in) {
-        xc = xnew(col);
+        xc = x[col];
         kcur = K[col];
         Dc = a.D[col];
         bc = a.b[col];
+        if (nz > 1) {
+            xn = x[nxy + col];
+            kn = K[nxy + col];
+        }
     }
     if (has_halo) {
-        hxc = xnew(hcol);
+        hxc = x[hcol];
         hkc = K[hcol];
     }
     float y[NZM], cp[NZM];
@@ -936,16 +983,18 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
 #pragma unroll
     for (int k = 0; k < NZM; ++k) {
         if (k < nz) {
-            double xn = 0.0, kn = 0.0, Dn = 1.0, bn = 0.0, hxn = 0.0, hkn = 0.0;
+            double x2 = 0.0, k2 = 0.0, Dn = 1.0, bn = 0.0, hxn = 0.0, hkn = 0.0;
             if (in && k + 1 < nz) {
                 const int cn = (k + 1) * nxy + col;
-                xn = xnew(cn);
-                kn = K[cn];
                 Dn = a.D[cn];
                 bn = a.b[cn];
+                if (k + 2 < nz) {
+                    x2 = x[cn + nxy];
+                    k2 = K[cn + nxy];
+                }
             }
             if (has_halo && k + 1 < nz) {
-                hxn = xnew((k + 1) * nxy + hcol);
+                hxn = x[(k + 1) * nxy + hcol];
                 hkn = K[(k + 1) * nxy + hcol];
             }
             const int buf = k & 1;
@@ -989,7 +1038,8 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
                 yprev = yk;
                 inv_prev = inv;
             }
-            xm = xc; km = kcur; xc = xn; kcur = kn; Dc = Dn; bc = bn; hxc = hxn; hkc = hkn;
+            xm = xc; km = kcur; xc = xn; kcur = kn; xn = x2; kn = k2;
+            Dc = Dn; bc = bn; hxc = hxn; hkc = hkn;
         }
     }
     if (in) {
