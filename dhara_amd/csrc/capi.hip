@@ -32,7 +32,8 @@ namespace {
 
 struct TuneKey {
     int nx, ny, nz, elem;
-    int kind;   // 0: SpMV kernel of the two-kernel PCG, 1: one-kernel PCG
+    int kind;   // 0: SpMV kernel of the two-kernel PCG, 1: one-kernel PCG, 2: assembly (FP64
+                // tile of the mixed column-preconditioned path, whose only FP64 tiled kernel it is)
     bool operator==(const TuneKey& o) const
     {
         return nx == o.nx && ny == o.ny && nz == o.nz && elem == o.elem && kind == o.kind;
@@ -48,6 +49,13 @@ void remember_tile(const TuneKey& key, const dh::TileCfg& cfg)
     for (TuneEntry& te : g_tune_table)
         if (te.key == key) { te.cfg = cfg; return; }
     g_tune_table.push_back(TuneEntry{key, cfg});
+}
+
+// Mixed precision with the column preconditioner: the FP32 Krylov kernels are the SpMV's only
+// users, and the FP64 tile serves the assembly alone.
+bool mixed_column(const dhara_params* p)
+{
+    return p->precision == DHARA_MIXED && p->precond == DHARA_PRECOND_COLUMN;
 }
 
 // Default tile: 32 x 8 columns (128 B rows in FP32), planes per block chosen for ~2048 blocks.
@@ -973,8 +981,9 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
         c->tile[e] = default_tile(g);
         c->cg1_tile[e] = dh::TileCfg{32, 8, g.nz, 0};
         if (dh::cg1_blocks(g, c->cg1_tile[e]) > dh::kMaxBlocks) c->cg1_tile[e].grid = dh::kMaxBlocks;   // blocks loop
+        const int tkind = (e == 0 && mixed_column(params)) ? 2 : 0;
         for (const TuneEntry& te : g_tune_table) {
-            if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4, 0}) c->tile[e] = te.cfg;
+            if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4, tkind}) c->tile[e] = te.cfg;
             if (te.key == TuneKey{g.nx, g.ny, g.nz, e == 0 ? 8 : 4, 1}) c->cg1_tile[e] = te.cfg;
         }
     }
@@ -1353,6 +1362,7 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
     const uint8_t* fl = has_dirichlet(c) ? c->flags : nullptr;
     // the column-preconditioned PCG runs the SpMV kernel that reads z (= r here)
     const bool zin = c->p.precond == DHARA_PRECOND_COLUMN && !fl && dh::line_supported(c->g);
+    const bool asm_tune = zin && mixed;   // FP64 tile: time the assembly, its one user here
     double best[2] = {0.0, 0.0};
     for (int e = 0; e < 2; ++e) {
         if (e == 1 && !mixed) continue;
@@ -1367,7 +1377,13 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
             sc.st = st;
             float us = 0.f;
             int r2;
-            if (e == 0) {
+            if (e == 0 && asm_tune) {
+                // the assembly rewrites D, b, D32, r32, d32 and the free-drainage flux from the
+                // state, as the next Picard iteration does anyway
+                r2 = time_spmv<double>(c, [&](int) {
+                    dh::launch_assemble_tiled(c->s, asm_args(c, false), true, t);
+                }, &us);
+            } else if (e == 0) {
                 double* const pp[2] = {c->pv[0], c->pv[1]};
                 r2 = time_spmv<double>(c, [&](int it) {
                     dh::SolveCtl k = sc;
@@ -1401,7 +1417,7 @@ int dhara_autotune(dhara_ctx* c, double* best_us)
         if ((rc = consider(dh::TileCfg{0, 0, 0}))) return rc;
         c->tile[e] = best_cfg;
         best[e] = best_t;
-        remember_tile(TuneKey{c->g.nx, c->g.ny, c->g.nz, e == 0 ? 8 : 4, 0}, best_cfg);
+        remember_tile(TuneKey{c->g.nx, c->g.ny, c->g.nz, e == 0 ? 8 : 4, (e == 0 && asm_tune) ? 2 : 0}, best_cfg);
     }
     // one-kernel PCG: tile shape x planes per chunk x grid cap, on scratch output buffers
     if (c->p.krylov == DHARA_KRYLOV_ONE_KERNEL && !fl && !zin) {

@@ -293,10 +293,12 @@ int dhara_get_krylov_tile(const dhara_ctx* ctx, int32_t elem_bytes, int32_t* tx,
 /* Times the candidate tile shapes on the context's current operator (HIP events, a few
  * launches each) for the precisions the context uses -- of the SpMV kernel (its z-input form
  * with DHARA_PRECOND_COLUMN) and, with DHARA_KRYLOV_ONE_KERNEL and Jacobi, of the one-kernel
- * PCG -- keeps the fastest (the earlier candidate unless another is 3 % faster), and records it
- * in a process-wide table keyed by (nx, ny, nz, elem_bytes) that later contexts of the same
- * shape reuse without timing.  best_us (may be NULL) receives the winner's time per launch for
- * [FP64, FP32]. */
+ * PCG; for the FP64 tile of DHARA_MIXED with DHARA_PRECOND_COLUMN, whose one FP64 tiled kernel
+ * is the assembly, of the assembly (which rewrites the system from the current state, as every
+ * Picard iteration does) -- keeps the fastest (the earlier candidate unless another is 3 %
+ * faster), and records it in a process-wide table keyed by (nx, ny, nz, elem_bytes, kernel)
+ * that later contexts of the same shape reuse without timing.  best_us (may be NULL) receives
+ * the winner's time per launch for [FP64, FP32]. */
 int dhara_autotune(dhara_ctx* ctx, double* best_us);
 
 /* ---- kernel-level entry points used by the parity tests ------------------------------ */
