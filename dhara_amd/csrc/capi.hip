@@ -493,6 +493,27 @@ int solve_fp64_line(dhara_ctx* c, dhara_step_stats* st)
 // Mixed precision with the column preconditioner: the reliable-update scheme of solve_mixed,
 // FP32 column solves; every segment starts with the column start-up kernel on the fresh
 // residual, whose partials the segment's first SpMV tests (outer FP64 test included).
+// Launches to queue for the FP32 segment after a reliable update, from the segment that just
+// ended (n iterations from the FP64 residual rr_outer down to the FP32 residual rr): a segment
+// that ran down to its floor (half the outer target), or whose residual is already below the
+// outer target, leaves an FP64 residual that usually passes the outer test at the next
+// segment's first launch -- one launch; otherwise the iterations its stopping test needs at the
+// segment's rate of reduction, plus the launch that detects the end.  0: no estimate (the
+// caller's hint decides).  Queued launches past the end return at once (~4 us each); one too
+// few costs a host round trip and a further batch.
+int next_segment_batch(const dhara_ctx* c, int n)
+{
+    const dh::PcgState& s = *c->h_st;
+    const double target = c->p.lin_tol * c->p.lin_tol * s.bb, floor2 = 0.25 * target;
+    if (s.tol2 <= floor2 * (1.0 + 1e-9) || s.rr <= target) return 1;
+    if (n < 1 || !(s.rr > 0.0) || !(s.rr < s.rr_outer)) return 0;
+    const double rate = std::pow(s.rr / s.rr_outer, 1.0 / n);   // per-iteration factor of ||r||^2
+    const double tol2 = std::max(c->p.inner_tol * c->p.inner_tol * s.rr, floor2);
+    const double need = std::log(s.rr / tol2) / std::log(1.0 / rate);
+    if (!std::isfinite(need)) return 0;
+    return std::max(1, std::min(kMaxBatch, (int)std::ceil(need - 1e-9) + 1));
+}
+
 int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
 {
     int rc;
@@ -555,10 +576,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
         if (c->h_st->nan) { st->nan_seen = 1; break; }
         if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
         const int it_end = c->h_st->iter;
-        // a segment that ran down to its floor (half the outer target) leaves an FP64 residual
-        // that usually passes the outer test at the next segment's first launch: queue one
-        // iteration instead of a full batch of launches that would return at once
-        first_batch = (c->h_st->tol2 <= 0.25 * c->p.lin_tol * c->p.lin_tol * c->h_st->bb * (1.0 + 1e-9)) ? 1 : 0;
+        first_batch = next_segment_batch(c, it_end - seg);
         // x += d, FP64 residual, and the next segment's z = T^-1 r (partials to part_line)
         seg_parts = dh::launch_refine_line(c->s, ra, c->zl32);
         if (!seg_parts) {   // more column tiles than partial slots: residual, then the start-up
