@@ -498,9 +498,11 @@ int solve_fp64_line(dhara_ctx* c, dhara_step_stats* st)
 // that ran down to its floor (half the outer target), or whose residual is already below the
 // outer target, leaves an FP64 residual that usually passes the outer test at the next
 // segment's first launch -- one launch; otherwise the iterations its stopping test needs at the
-// segment's rate of reduction, plus the launch that detects the end.  0: no estimate (the
-// caller's hint decides).  Queued launches past the end return at once (~4 us each); one too
-// few costs a host round trip and a further batch.
+// segment's rate of reduction, one more (the FP64 residual after the update sits above the FP32
+// one the estimate starts from: without it a third of the estimates on the coupled run came
+// out one short), and the launch that detects the end.  0: no estimate (the caller's hint
+// decides).  Queued launches past the end return at once (~4 us a launch); one too few costs a
+// host round trip and a further batch (~50 us).
 int next_segment_batch(const dhara_ctx* c, int n)
 {
     const dh::PcgState& s = *c->h_st;
@@ -511,7 +513,7 @@ int next_segment_batch(const dhara_ctx* c, int n)
     const double tol2 = std::max(c->p.inner_tol * c->p.inner_tol * s.rr, floor2);
     const double need = std::log(s.rr / tol2) / std::log(1.0 / rate);
     if (!std::isfinite(need)) return 0;
-    return std::max(1, std::min(kMaxBatch, (int)std::ceil(need - 1e-9) + 1));
+    return std::max(1, std::min(kMaxBatch, (int)std::ceil(need - 1e-9) + 2));
 }
 
 int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
