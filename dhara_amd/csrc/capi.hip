@@ -160,6 +160,17 @@ int host_reduce(dhara_ctx* c, int nparts, int nv, bool take_max, double* out)
     return DHARA_OK;
 }
 
+// Sum of n per-block partials at dev (a row of a partial buffer), in block order.
+int host_sum(dhara_ctx* c, const double* dev, int n, double* out)
+{
+    HIPCHK(c, hipMemcpyAsync(c->h_part, dev, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    double acc = 0.0;
+    for (int b = 0; b < n; ++b) acc += c->h_part[b];
+    *out = acc;
+    return DHARA_OK;
+}
+
 bool has_dirichlet(const dhara_ctx* c)
 {
     for (int f = 0; f < 6; ++f) {
@@ -592,6 +603,17 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
         }
         LAUNCHCHK(c);
         ++updates;
+        if (first_batch == 1 && it_end < maxit && updates <= c->p.refine_maxiter) {
+            // the outer test that the next segment's first launch would make, made here on the
+            // update's FP64 residual (slot 3 of its partials): when it passes, as it usually does
+            // after a segment that reached its floor, the solve ends without that launch
+            double rr64 = 0.0;
+            if ((rc = host_sum(c, c->part_line + 3 * (size_t)dh::kMaxBlocks, seg_parts, &rr64))) return rc;
+            if (rr64 <= c->p.lin_tol * c->p.lin_tol * c->h_st->bb) {
+                outer_conv = true;
+                break;
+            }
+        }
         if (it_end >= maxit || updates > c->p.refine_maxiter) {
             // final convergence check only: a segment start at the limit ends the solve
             dh::SolveCtl k = sc;
