@@ -590,8 +590,14 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
         if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
         const int it_end = c->h_st->iter;
         first_batch = next_segment_batch(c, it_end - seg);
+        ++updates;
+        // after a segment that reached its floor the update usually passes the outer test: the
+        // test is then made here, on the host, from the update's FP64 residual (slot 3 of its
+        // partials), and the update computes only that residual until the test has failed
+        const bool check = first_batch == 1 && it_end < maxit && updates <= c->p.refine_maxiter;
         // x += d, FP64 residual, and the next segment's z = T^-1 r (partials to part_line)
-        seg_parts = dh::launch_refine_line(c->s, ra, c->zl32);
+        seg_parts = dh::launch_refine_line(c->s, ra, c->zl32, check ? dh::kRefineCheck : dh::kRefineFull);
+        const bool partial = check && seg_parts != 0;   // r32, d and z32 not written yet
         if (!seg_parts) {   // more column tiles than partial slots: residual, then the start-up
             ra.part_out = c->part[0];
             dh::SolveCtl k = sc;
@@ -602,16 +608,16 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
             seg_parts = lparts;
         }
         LAUNCHCHK(c);
-        ++updates;
-        if (first_batch == 1 && it_end < maxit && updates <= c->p.refine_maxiter) {
-            // the outer test that the next segment's first launch would make, made here on the
-            // update's FP64 residual (slot 3 of its partials): when it passes, as it usually does
-            // after a segment that reached its floor, the solve ends without that launch
+        if (check) {
             double rr64 = 0.0;
             if ((rc = host_sum(c, c->part_line + 3 * (size_t)dh::kMaxBlocks, seg_parts, &rr64))) return rc;
             if (rr64 <= c->p.lin_tol * c->p.lin_tol * c->h_st->bb) {
                 outer_conv = true;
                 break;
+            }
+            if (partial) {   // the rest of the update: r32, d = 0, z32 and the segment partials
+                seg_parts = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineRest);
+                LAUNCHCHK(c);
             }
         }
         if (it_end >= maxit || updates > c->p.refine_maxiter) {

@@ -929,7 +929,9 @@ __global__ __launch_bounds__(TX * TY) void k_spmv_z(Geo g, const T* __restrict__
 // updating.)  A block owns TX x TY whole columns (the column solve needs them); x and K of each
 // plane are staged in LDS with a one-cell halo, the vertical neighbours stay in registers.
 // Partials [(r32, z32), (r32, r32), 0, (r, r)] per block, the layout of a segment start.
-template <int NZM, int TX, int TY>
+// FULL = false: the FP64 residual's norm only (partials [0, 0, 0, (r, r)]; r32, d32 and z32 are
+// not written), for the outer test of an update after which the solve usually ends.
+template <int NZM, int TX, int TY, bool FULL>
 __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __restrict__ z32, int ntx, int nty)
 {
     const Geo g = a.g;
@@ -1015,34 +1017,36 @@ __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __
                 if (j < g.ny - 1) ax -= wy * (half * (kcur + sk[buf][ty + 2][tx + 1])) * sx[buf][ty + 2][tx + 1];
                 if (k < nz - 1) ax -= wz * (half * (kcur + kn)) * xn;
                 const double r = bc - ax;
-                const float r32 = (float)r;
-                a.r32[c] = r32;
-                a.d32[c] = 0.0f;
                 acc[3] += r * r;
-                // forward elimination of the FP32 column system (k_line's arithmetic)
-                const float k32 = (float)kcur;
-                float den = (float)Dc, w = r32;
-                if (k > 0) {
-                    const float av = -hz * (k32 + kprev);
-                    const float cpk = av * inv_prev;
-                    cp[k - 1] = cpk;
-                    den -= av * cpk;
-                    w -= av * yprev;
+                if constexpr (FULL) {
+                    const float r32 = (float)r;
+                    a.r32[c] = r32;
+                    a.d32[c] = 0.0f;
+                    // forward elimination of the FP32 column system (k_line's arithmetic)
+                    const float k32 = (float)kcur;
+                    float den = (float)Dc, w = r32;
+                    if (k > 0) {
+                        const float av = -hz * (k32 + kprev);
+                        const float cpk = av * inv_prev;
+                        cp[k - 1] = cpk;
+                        den -= av * cpk;
+                        w -= av * yprev;
+                    }
+                    const float inv = jac_inv(den);
+                    const float yk = w * inv;
+                    y[k] = yk;
+                    acc[0] += (double)w * (double)yk;
+                    acc[1] += (double)r32 * (double)r32;
+                    kprev = k32;
+                    yprev = yk;
+                    inv_prev = inv;
                 }
-                const float inv = jac_inv(den);
-                const float yk = w * inv;
-                y[k] = yk;
-                acc[0] += (double)w * (double)yk;
-                acc[1] += (double)r32 * (double)r32;
-                kprev = k32;
-                yprev = yk;
-                inv_prev = inv;
             }
             xm = xc; km = kcur; xc = xn; kcur = kn; xn = x2; kn = k2;
             Dc = Dn; bc = bn; hxc = hxn; hkc = hkn;
         }
     }
-    if (in) {
+    if (FULL && in) {
         float zn = 0.0f;
 #pragma unroll
         for (int k = NZM - 1; k >= 0; --k) {
@@ -2172,16 +2176,25 @@ static void refine_tiled(hipStream_t s, const RefineArgs& a, int kc)
                        nty, nkz);
 }
 
-int launch_refine_line(hipStream_t s, const RefineArgs& a, float* z32)
+template <bool FULL>
+static void refine_line_dispatch(hipStream_t s, const RefineArgs& a, float* z32, int ntx, int nty)
+{
+    constexpr int TX = 32, TY = 8;
+    const dim3 gr(ntx * nty), bl(TX, TY);
+    if (a.g.nz <= 16) hipLaunchKernelGGL((k_refine_line<16, TX, TY, FULL>), gr, bl, 0, s, a, z32, ntx, nty);
+    else if (a.g.nz <= 32) hipLaunchKernelGGL((k_refine_line<32, TX, TY, FULL>), gr, bl, 0, s, a, z32, ntx, nty);
+    else hipLaunchKernelGGL((k_refine_line<64, TX, TY, FULL>), gr, bl, 0, s, a, z32, ntx, nty);
+}
+
+int launch_refine_line(hipStream_t s, const RefineArgs& a, float* z32, RefineLineMode mode)
 {
     constexpr int TX = 32, TY = 8;
     const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
     if (ntx * nty > kMaxBlocks || a.g.nz > kLineMaxNz) return 0;
-    hipLaunchKernelGGL(k_refine_add, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a.g.n, a.x, (const float*)a.d32);
-    const dim3 gr(ntx * nty), bl(TX, TY);
-    if (a.g.nz <= 16) hipLaunchKernelGGL((k_refine_line<16, TX, TY>), gr, bl, 0, s, a, z32, ntx, nty);
-    else if (a.g.nz <= 32) hipLaunchKernelGGL((k_refine_line<32, TX, TY>), gr, bl, 0, s, a, z32, ntx, nty);
-    else hipLaunchKernelGGL((k_refine_line<64, TX, TY>), gr, bl, 0, s, a, z32, ntx, nty);
+    if (mode != kRefineRest)
+        hipLaunchKernelGGL(k_refine_add, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a.g.n, a.x, (const float*)a.d32);
+    if (mode == kRefineCheck) refine_line_dispatch<false>(s, a, z32, ntx, nty);
+    else refine_line_dispatch<true>(s, a, z32, ntx, nty);
     return ntx * nty;
 }
 
