@@ -763,8 +763,14 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
         if (st->nan_seen) { c->theta_m_current = false; break; }
         dh::launch_vg_picard(c->s, vg_args(c, c->x, c->theta_new));
         LAUNCHCHK(c);
+        // Picard increment and NaN flag: maxima over the blocks, taken on the device
         double pic[2];
-        if ((rc = host_reduce(c, dh::grid_for(c->N), 2, true, pic))) return rc;
+        dh::launch_max_rows2(c->s, c->part[0], dh::grid_for(c->N), c->dred);
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(c->h_part, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        pic[0] = c->h_part[0];
+        pic[1] = c->h_part[1];
         ++niter;
         delta = pic[0];
         if (pic[1] > 0.0) { st->nan_seen = 1; c->theta_m_current = false; break; }
@@ -1061,7 +1067,7 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
     if (grid->nroot_groups > dh::kMaxSums * dh::kMaxBlocks) { dhara_destroy(c); return DHARA_E_ARG; }
     if ((rc = dalloc(c, &c->procmap, S)) || (rc = dalloc(c, &c->root, (size_t)grid->nroot_groups * grid->nz)) ||
         (rc = dalloc(c, &c->flags, N)) || (rc = dalloc(c, &c->part[0], (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
-        (rc = dalloc(c, &c->part[1], (size_t)dh::kMaxSums * dh::kMaxBlocks)) ||
+        (rc = dalloc(c, &c->part[1], (size_t)dh::kMaxSums * dh::kMaxBlocks)) || (rc = dalloc(c, &c->dred, 16)) ||
         (rc = dalloc(c, &c->st_sub, 1)) || (rc = dalloc(c, &c->st_olf, 1))) {
         dhara_destroy(c);
         return rc;

@@ -55,6 +55,7 @@ struct dhara_ctx {
     double* zl = nullptr;
     float* zl32 = nullptr;
     double* part_line = nullptr;
+    double* dred = nullptr;   // a few device-reduced values for the host (launch_max_rows2)
     dh::PcgState* st_sub = nullptr;
     dh::PcgState* st_olf = nullptr;
     // pinned host mirrors

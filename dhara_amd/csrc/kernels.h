@@ -179,6 +179,8 @@ struct LaunchEv {
 
 void launch_vg(hipStream_t s, const VgArgs& a);                 // theta, K, C of psi
 void launch_vg_picard(hipStream_t s, const VgArgs& a);          // + increment + relax
+// out[0], out[1] = maxima of rows 0 and 1 of part (nparts partials each); one block
+void launch_max_rows2(hipStream_t s, const double* part, int nparts, double* out);
 void launch_vg_inverse(hipStream_t s, const Geo& g, const Soil& soil, const double* theta,
                        double* psi);
 void launch_row_flags(hipStream_t s, const Geo& g, const int* const bctype[6], uint8_t* flags);

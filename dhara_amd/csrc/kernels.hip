@@ -131,6 +131,25 @@ __global__ __launch_bounds__(kBlock) void k_vg_picard(VgArgs a)
     b_vg_picard(a, hw_blk());
 }
 
+// Maxima of rows 0 and 1 of a partial buffer (nparts blocks each) into out[0], out[1], in one
+// block: the host then reads two values instead of the rows (Picard increment and NaN flag;
+// a maximum does not depend on the order it is taken in).
+__global__ __launch_bounds__(kBlock) void k_max_rows2(const double* __restrict__ part, int nparts,
+                                                      double* __restrict__ out)
+{
+    __shared__ double lds[2 * 16];
+    double v[2] = {-INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        v[0] = fmax(v[0], part[b]);
+        v[1] = fmax(v[1], part[kMaxBlocks + b]);
+    }
+    block_reduce<2, true>(v, lds);
+    if (threadIdx.x == 0) {
+        out[0] = v[0];
+        out[1] = v[1];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_vg_inverse(Geo g, Soil s, const double* theta, double* psi)
 {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.n; c += gridDim.x * blockDim.x)
@@ -2089,6 +2108,11 @@ void launch_vg(hipStream_t s, const VgArgs& a)
 void launch_vg_picard(hipStream_t s, const VgArgs& a)
 {
     hipLaunchKernelGGL(k_vg_picard, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_max_rows2(hipStream_t s, const double* part, int nparts, double* out)
+{
+    hipLaunchKernelGGL(k_max_rows2, dim3(1), dim3(kBlock), 0, s, part, nparts, out);
 }
 
 void launch_vg_inverse(hipStream_t s, const Geo& g, const Soil& soil, const double* theta, double* psi)
