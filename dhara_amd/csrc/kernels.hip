@@ -1103,9 +1103,9 @@ __device__ __forceinline__ void st_off(T* base, unsigned byte_off, T v)
 
 
 // INIT: start-up of a one-kernel PCG segment instead -- u = M^-1 r, A u, no stores, partials
-// [(r, u), (r, r), (u, A u), (p_old, A u), 0] (p_old: the kept direction, it > 0).
-// ZIN: r holds the preconditioned residual z (column preconditioner): p = z + beta p.
-template <typename T, bool DIR, int TX, int TY, bool INIT = false, bool ZIN = false>
+// [(r, u), (r, r), (u, A u), (p_old, A u), 0] (p_old: the kept direction, it > 0).  (The
+// column-preconditioned form, whose input is z, is k_spmv_z.)
+template <typename T, bool DIR, int TX, int TY, bool INIT = false>
 __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_t* __restrict__ flags,
                                                              const T* __restrict__ K, const T* __restrict__ D,
                                                              const T* __restrict__ r, const T* __restrict__ pold,
@@ -1144,10 +1144,7 @@ __global__ __launch_bounds__(TX * TY) void k_pcg3_spmv_tiled(Geo g, const uint8_
     const bool has_halo = hx > -2 && hi >= 0 && hi < g.nx && hj >= 0 && hj < g.ny;
     const int hcol = hj * g.nx + hi;
 
-    auto jac = [&](T rv, T dv) -> T {
-        if constexpr (ZIN) return rv;
-        else return INIT ? rv * jac_inv(dv) : rv / dv;
-    };
+    auto jac = [&](T rv, T dv) -> T { return INIT ? rv * jac_inv(dv) : rv / dv; };
     auto pval = [&](int c) -> T {
         const T z = jac(r[c], D[c]);
         return first ? z : z + beta * pold[c];
