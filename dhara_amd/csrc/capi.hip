@@ -562,9 +562,20 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
     int seg_parts = lparts;   // partials of the segment start-up in part_line
     ra.part_out = c->part_line;
     int first_batch = 0;      // 0: size the first batch from the previous segment
+    // A segment expected to end at its floor (half the outer target) is followed by an update
+    // that usually ends the solve: its check-only form (gated: it runs only once the segment has
+    // ended unconverged) is then queued behind every batch of the segment, and the update's
+    // partials come back with the segment's state in the same synchronisation.
+    const double target = c->p.lin_tol * c->p.lin_tol;   // x ||b||^2
+    bool spec = false, line_update = true;   // line_update: k_refine_line fits the surface
+    int spec_parts = 0;
+    hipError_t spec_err = hipSuccess;
+    auto expect_floor = [&](double rr0) {   // the next segment's stopping test is its floor
+        return line_update && c->p.inner_tol * c->p.inner_tol * rr0 <= 0.5 * 0.25 * target * c->h_st->bb;
+    };
     for (;;) {
         const int seg = it;
-        rc = run_krylov(
+        rc = run_krylov_then(
             c, c->st_sub, maxit, c->pcg_hint, c->prof, 4.0,
             [&](int j, const dh::LaunchEv& ev) {
                 dh::SolveCtl k = sc;
@@ -583,12 +594,39 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
                 dh::launch_line_update_f32(c->s, c->g, c->K32, c->D32, c->d32, c->r32, pp[(j + 1) & 1], c->q32,
                                            c->zl32, k, ev);
             },
+            [&] {
+                if (!spec) return;
+                spec_parts = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineCheck);
+                if (spec_parts && spec_err == hipSuccess)
+                    spec_err = hipMemcpyAsync(c->h_part, c->part_line + 3 * (size_t)dh::kMaxBlocks,
+                                              (size_t)spec_parts * sizeof(double), hipMemcpyDeviceToHost, c->s);
+            },
             it, first_batch);
         if (rc) return rc;
+        if (spec_err != hipSuccess) return set_err(c, DHARA_E_DEVICE, hipGetErrorString(spec_err));
         c->prof_acc.update_bytes = 9.0 * 4.0 * c->N;
         if (c->h_st->nan) { st->nan_seen = 1; break; }
         if (c->h_st->outer_conv) { outer_conv = true; break; }   // d == 0 here
         const int it_end = c->h_st->iter;
+        const bool spec_ran = spec && spec_parts != 0 && c->h_st->gate;   // the queued update ran
+        spec = false;
+        if (spec_ran) {
+            ++updates;
+            double rr64 = 0.0;
+            for (int b = 0; b < spec_parts; ++b) rr64 += c->h_part[b];
+            if (rr64 <= target * c->h_st->bb) {
+                outer_conv = true;
+                break;
+            }
+            if (it_end >= maxit || updates > c->p.refine_maxiter) break;   // the outer test at the limit failed
+            first_batch = next_segment_batch(c, it_end - seg);
+            seg_parts = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineRest);
+            LAUNCHCHK(c);
+            spec = expect_floor(rr64);
+            it = it_end;
+            sc.first_sweep = 0;
+            continue;
+        }
         first_batch = next_segment_batch(c, it_end - seg);
         ++updates;
         // after a segment that reached its floor the update usually passes the outer test: the
@@ -599,6 +637,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
         seg_parts = dh::launch_refine_line(c->s, ra, c->zl32, check ? dh::kRefineCheck : dh::kRefineFull);
         const bool partial = check && seg_parts != 0;   // r32, d and z32 not written yet
         if (!seg_parts) {   // more column tiles than partial slots: residual, then the start-up
+            line_update = false;
             ra.part_out = c->part[0];
             dh::SolveCtl k = sc;
             k.nparts = dh::launch_refine_tiled(c->s, ra, c->tile[0]);
@@ -608,13 +647,15 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
             seg_parts = lparts;
         }
         LAUNCHCHK(c);
+        double rr_next = c->h_st->rr;   // the next segment's start, as far as the host knows it
         if (check) {
             double rr64 = 0.0;
             if ((rc = host_sum(c, c->part_line + 3 * (size_t)dh::kMaxBlocks, seg_parts, &rr64))) return rc;
-            if (rr64 <= c->p.lin_tol * c->p.lin_tol * c->h_st->bb) {
+            if (rr64 <= target * c->h_st->bb) {
                 outer_conv = true;
                 break;
             }
+            rr_next = rr64;
             if (partial) {   // the rest of the update: r32, d = 0, z32 and the segment partials
                 seg_parts = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineRest);
                 LAUNCHCHK(c);
@@ -635,6 +676,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
             outer_conv = c->h_st->outer_conv != 0;
             break;
         }
+        spec = expect_floor(rr_next);
         it = it_end;
         sc.first_sweep = 0;
     }

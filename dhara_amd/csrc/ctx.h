@@ -136,9 +136,10 @@ void free_adjoint(dhara_ctx* c);   // adjoint.hip
 // kernel of iteration `it` reads partials A and writes B, the update kernel reads B and
 // writes A.  With profiling on, events bracket each SpMV and update launch of the batch and
 // only the launches that did work (the first ones of the batch) are accumulated.
-template <class SpmvFn, class UpdFn>
-int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
-               SpmvFn spmv, UpdFn upd, int it_start = 0, int first_batch = 0)
+// after() queues further work behind every batch, before the state is read back.
+template <class SpmvFn, class UpdFn, class AfterFn>
+int run_krylov_then(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
+                    SpmvFn spmv, UpdFn upd, AfterFn after, int it_start, int first_batch)
 {
     int it = it_start;
     // first batch: the previous solve's count + the launch that detects the end (the next
@@ -157,6 +158,7 @@ int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, 
             spmv(it, e_spmv);
             upd(it, e_upd);
         }
+        after();
         LAUNCHCHK(c);
         int rc = read_state(c, st);
         if (rc) return rc;
@@ -182,6 +184,13 @@ int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, 
     }
     if (first_batch <= 0) hint = std::max(1, c->h_st->iter - it_start);
     return DHARA_OK;
+}
+
+template <class SpmvFn, class UpdFn>
+int run_krylov(dhara_ctx* c, dh::PcgState* st, int maxit, int& hint, bool prof, double elem,
+               SpmvFn spmv, UpdFn upd, int it_start = 0, int first_batch = 0)
+{
+    return run_krylov_then(c, st, maxit, hint, prof, elem, spmv, upd, [] {}, it_start, first_batch);
 }
 
 // One launch per iteration (one-kernel PCG): the same batching as run_krylov; iteration `it`

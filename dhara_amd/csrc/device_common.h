@@ -32,7 +32,8 @@ struct PcgState {
     int32_t conv;     // 1: converged
     int32_t nan;
     int32_t outer_conv;
-    int32_t pad[3];
+    int32_t gate;     // 1: the solve segment ended without convergence or NaN (k_refine_gate)
+    int32_t pad[2];
 };
 
 // ---- Mualem-van Genuchten ------------------------------------------------------------

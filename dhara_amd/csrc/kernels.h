@@ -201,7 +201,10 @@ int launch_refine_tiled(hipStream_t s, const RefineArgs& a, const TileCfg& t);
 // r32 = b - A x, d = 0 and z32 = T32^-1 r32 in one launch; partials [rz, rr, 0, rr64] to
 // a.part_out; returns their number, or 0 without launching anything when the surface has more
 // column tiles than partial slots.  kRefineCheck: x += d and the FP64 residual's norm only
-// (partials [0, 0, 0, rr64]); kRefineRest: r32, d, z32 and all partials of an x already updated.
+// (partials [0, 0, 0, rr64]), both gated on the end of the segment (a.st->gate, set by a gate
+// launch first: nothing runs if the segment has not ended, or ended converged or with NaN), so
+// that it may be queued behind a batch; kRefineRest: r32, d, z32 and all partials of an x
+// already updated.
 enum RefineLineMode { kRefineFull, kRefineCheck, kRefineRest };
 int launch_refine_line(hipStream_t s, const RefineArgs& a, float* z32, RefineLineMode mode = kRefineFull);
 void launch_end_step(hipStream_t s, const EndArgs& a);

@@ -505,6 +505,21 @@ __global__ __launch_bounds__(kBlock) void k_refine_add(int n, double* x, const f
     b_refine_add(n, x, d, hw_blk());
 }
 
+// The gate of a reliable update queued behind a batch of Krylov launches before the host has
+// seen whether the segment ended: open (1) when the segment ended without outer convergence or
+// NaN.  The gated kernels read it; the kernel that starts the next segment (k_refine_line,
+// block 0) resets `done` only after every gated kernel before it has read the gate.
+__global__ void k_refine_gate(PcgState* st)
+{
+    if (threadIdx.x == 0) st->gate = (st->done && !st->outer_conv && !st->nan) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refine_add_gated(int n, double* x, const float* d, const PcgState* st)
+{
+    if (!st->gate) return;
+    b_refine_add(n, x, d, hw_blk());
+}
+
 template <bool DIR>
 __device__ __forceinline__ void b_refine_residual(RefineArgs a, Blk bk)
 {
@@ -949,12 +964,17 @@ __global__ __launch_bounds__(TX * TY) void k_spmv_z(Geo g, const T* __restrict__
 // plane are staged in LDS with a one-cell halo, the vertical neighbours stay in registers.
 // Partials [(r32, z32), (r32, r32), 0, (r, r)] per block, the layout of a segment start.
 // FULL = false: the FP64 residual's norm only (partials [0, 0, 0, (r, r)]; r32, d32 and z32 are
-// not written), for the outer test of an update after which the solve usually ends.
+// not written), for the outer test of an update after which the solve usually ends; this form
+// runs only when the gate (k_refine_gate) is open, so that it can be queued before the host has
+// seen the segment end.
 template <int NZM, int TX, int TY, bool FULL>
 __global__ __launch_bounds__(TX * TY) void k_refine_line(RefineArgs a, float* __restrict__ z32, int ntx, int nty)
 {
+    if (!FULL && !a.st->gate) return;
     const Geo g = a.g;
-    if (lin_bid() == 0 && lin_tid() == 0) a.st->done = 0;   // the next inner solve starts
+    // the next inner solve starts (the check-only form leaves the ended segment's state as it is:
+    // the host reads it after the check, and the rest of the update resets it)
+    if (FULL && lin_bid() == 0 && lin_tid() == 0) a.st->done = 0;
     __shared__ double sx[2][TY + 2][TX + 2];
     __shared__ double sk[2][TY + 2][TX + 2];
     const int t = xcd_tile(blockIdx.x, ntx * nty);
@@ -2212,10 +2232,17 @@ int launch_refine_line(hipStream_t s, const RefineArgs& a, float* z32, RefineLin
     constexpr int TX = 32, TY = 8;
     const int ntx = (a.g.nx + TX - 1) / TX, nty = (a.g.ny + TY - 1) / TY;
     if (ntx * nty > kMaxBlocks || a.g.nz > kLineMaxNz) return 0;
-    if (mode != kRefineRest)
-        hipLaunchKernelGGL(k_refine_add, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a.g.n, a.x, (const float*)a.d32);
-    if (mode == kRefineCheck) refine_line_dispatch<false>(s, a, z32, ntx, nty);
-    else refine_line_dispatch<true>(s, a, z32, ntx, nty);
+    if (mode == kRefineCheck) {
+        hipLaunchKernelGGL(k_refine_gate, dim3(1), dim3(kWave), 0, s, a.st);
+        hipLaunchKernelGGL(k_refine_add_gated, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a.g.n, a.x,
+                           (const float*)a.d32, (const PcgState*)a.st);
+        refine_line_dispatch<false>(s, a, z32, ntx, nty);
+    } else {
+        if (mode == kRefineFull)
+            hipLaunchKernelGGL(k_refine_add, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a.g.n, a.x,
+                               (const float*)a.d32);
+        refine_line_dispatch<true>(s, a, z32, ntx, nty);
+    }
     return ntx * nty;
 }
 
