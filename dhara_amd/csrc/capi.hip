@@ -597,9 +597,14 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
             [&] {
                 if (!spec) return;
                 spec_parts = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineCheck);
-                if (spec_parts && spec_err == hipSuccess)
-                    spec_err = hipMemcpyAsync(c->h_part, c->part_line + 3 * (size_t)dh::kMaxBlocks,
-                                              (size_t)spec_parts * sizeof(double), hipMemcpyDeviceToHost, c->s);
+                if (!spec_parts) return;
+                // the outer test on the device and, if it passes, the Picard pass that follows the
+                // solve (theta, K, C, increment): its maxima arrive with the state
+                dh::launch_refine_conv(c->s, c->part_line, spec_parts, c->p.lin_tol, c->st_sub);
+                dh::launch_vg_picard_gated(c->s, vg_args(c, c->x, c->theta_new), c->st_sub);
+                dh::launch_max_rows2(c->s, c->part[0], dh::grid_for(c->N), c->dred);
+                if (spec_err == hipSuccess)
+                    spec_err = hipMemcpyAsync(c->h_red, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s);
             },
             it, first_batch);
         if (rc) return rc;
@@ -612,10 +617,10 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
         spec = false;
         if (spec_ran) {
             ++updates;
-            double rr64 = 0.0;
-            for (int b = 0; b < spec_parts; ++b) rr64 += c->h_part[b];
-            if (rr64 <= target * c->h_st->bb) {
+            const double rr64 = c->h_st->rr_update;
+            if (c->h_st->gate == 2) {   // the device's outer test passed; k_vg_picard has run
                 outer_conv = true;
+                c->vg_ready = true;
                 break;
             }
             if (it_end >= maxit || updates > c->p.refine_maxiter) break;   // the outer test at the limit failed
@@ -803,16 +808,20 @@ int do_subsurface_step(dhara_ctx* c, dhara_step_stats* st)
                               : (one ? solve_fp64_cg1(c, st) : solve_fp64(c, st));
         if (rc) return rc;
         if (st->nan_seen) { c->theta_m_current = false; break; }
-        dh::launch_vg_picard(c->s, vg_args(c, c->x, c->theta_new));
-        LAUNCHCHK(c);
-        // Picard increment and NaN flag: maxima over the blocks, taken on the device
+        // Picard increment and NaN flag: maxima over the blocks, taken on the device (by the
+        // solve itself when it queued the Picard pass behind its last batch)
         double pic[2];
-        dh::launch_max_rows2(c->s, c->part[0], dh::grid_for(c->N), c->dred);
-        LAUNCHCHK(c);
-        HIPCHK(c, hipMemcpyAsync(c->h_part, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(c, hipStreamSynchronize(c->s));
-        pic[0] = c->h_part[0];
-        pic[1] = c->h_part[1];
+        if (c->vg_ready) {
+            c->vg_ready = false;
+        } else {
+            dh::launch_vg_picard(c->s, vg_args(c, c->x, c->theta_new));
+            dh::launch_max_rows2(c->s, c->part[0], dh::grid_for(c->N), c->dred);
+            LAUNCHCHK(c);
+            HIPCHK(c, hipMemcpyAsync(c->h_red, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+            HIPCHK(c, hipStreamSynchronize(c->s));
+        }
+        pic[0] = c->h_red[0];
+        pic[1] = c->h_red[1];
         ++niter;
         delta = pic[0];
         if (pic[1] > 0.0) { st->nan_seen = 1; c->theta_m_current = false; break; }
@@ -1122,6 +1131,7 @@ int dhara_create(const dhara_grid* grid, const dhara_params* params, dhara_ctx**
         }
     }
     if (hipHostMalloc((void**)&c->h_st, sizeof(dh::PcgState)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_red, 16 * sizeof(double)) != hipSuccess ||
         hipHostMalloc((void**)&c->h_part, (size_t)dh::kMaxSums * dh::kMaxBlocks * sizeof(double)) != hipSuccess) {
         dhara_destroy(c);
         return DHARA_E_ALLOC;
@@ -1141,6 +1151,7 @@ int dhara_destroy(dhara_ctx* c)
     for (void* v : c->allocs) (void)hipFree(v);
     if (c->h_st) (void)hipHostFree(c->h_st);
     if (c->h_part) (void)hipHostFree(c->h_part);
+    if (c->h_red) (void)hipHostFree(c->h_red);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
     return DHARA_OK;

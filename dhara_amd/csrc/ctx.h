@@ -56,6 +56,9 @@ struct dhara_ctx {
     float* zl32 = nullptr;
     double* part_line = nullptr;
     double* dred = nullptr;   // a few device-reduced values for the host (launch_max_rows2)
+    double* h_red = nullptr;  // pinned: the Picard increment and NaN flag of a k_vg_picard queued
+                              // by the linear solve (vg_ready)
+    bool vg_ready = false;    // the solve ended with k_vg_picard already run, h_red its maxima
     dh::PcgState* st_sub = nullptr;
     dh::PcgState* st_olf = nullptr;
     // pinned host mirrors

@@ -26,13 +26,15 @@ struct PcgState {
     double tol2;      // stop when ||r||^2 <= tol2
     double rr;        // ||r||^2 when the solve ended
     double rr_outer;  // ||r_64||^2 of the last FP64 residual (mixed precision)
+    double rr_update; // ||r_64||^2 after the last gated reliable update (k_refine_conv)
     double pi[2];     // one-kernel PCG: (p_i, A p_i), slot i & 1, written at the start of iteration i
     int32_t iter;     // iterations done when the solve ended
     int32_t done;     // 1: solve finished (converged, limit, or breakdown)
     int32_t conv;     // 1: converged
     int32_t nan;
     int32_t outer_conv;
-    int32_t gate;     // 1: the solve segment ended without convergence or NaN (k_refine_gate)
+    int32_t gate;     // 1: the solve segment ended without convergence or NaN (k_refine_gate);
+                      // 2: and the update after it passed the outer test (k_refine_conv)
     int32_t pad[2];
 };
 

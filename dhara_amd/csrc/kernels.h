@@ -181,6 +181,11 @@ void launch_vg(hipStream_t s, const VgArgs& a);                 // theta, K, C o
 void launch_vg_picard(hipStream_t s, const VgArgs& a);          // + increment + relax
 // out[0], out[1] = maxima of rows 0 and 1 of part (nparts partials each); one block
 void launch_max_rows2(hipStream_t s, const double* part, int nparts, double* out);
+// after a gated check-only update (launch_refine_line, kRefineCheck): the outer test on the
+// device (st->gate = 2 when the sum of part's slot-3 partials passes lin_tol^2 ||b||^2, the sum
+// in st->rr_update), and k_vg_picard run only in that case
+void launch_refine_conv(hipStream_t s, const double* part, int nparts, double lin_tol, PcgState* st);
+void launch_vg_picard_gated(hipStream_t s, const VgArgs& a, const PcgState* st);
 void launch_vg_inverse(hipStream_t s, const Geo& g, const Soil& soil, const double* theta,
                        double* psi);
 void launch_row_flags(hipStream_t s, const Geo& g, const int* const bctype[6], uint8_t* flags);

@@ -131,6 +131,14 @@ __global__ __launch_bounds__(kBlock) void k_vg_picard(VgArgs a)
     b_vg_picard(a, hw_blk());
 }
 
+// k_vg_picard queued before the host knows that the linear solve has converged: runs only when
+// the device-side outer test has passed (gate 2, k_refine_conv).
+__global__ __launch_bounds__(kBlock) void k_vg_picard_gated(VgArgs a, const PcgState* st)
+{
+    if (st->gate != 2) return;
+    b_vg_picard(a, hw_blk());
+}
+
 // Maxima of rows 0 and 1 of a partial buffer (nparts blocks each) into out[0], out[1], in one
 // block: the host then reads two values instead of the rows (Picard increment and NaN flag;
 // a maximum does not depend on the order it is taken in).
@@ -518,6 +526,23 @@ __global__ __launch_bounds__(kBlock) void k_refine_add_gated(int n, double* x, c
 {
     if (!st->gate) return;
     b_refine_add(n, x, d, hw_blk());
+}
+
+// Outer test of a gated check-only update, on the device: the sum of the update's FP64 residual
+// partials (slot 3, summed in a fixed order) into rr_update and, when it passes
+// lin_tol^2 ||b||^2, gate = 2 (the solve has converged).  The host adopts this decision.
+__global__ __launch_bounds__(kBlock) void k_refine_conv(const double* __restrict__ part, int nparts, double lin_tol,
+                                                        PcgState* st)
+{
+    if (st->gate != 1) return;
+    __shared__ double lds[16];
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) v[0] += part[3 * kMaxBlocks + b];
+    block_reduce<1, false>(v, lds);
+    if (threadIdx.x == 0) {
+        st->rr_update = v[0];
+        if (v[0] <= lin_tol * lin_tol * st->bb) st->gate = 2;
+    }
 }
 
 template <bool DIR>
@@ -2125,6 +2150,16 @@ void launch_vg(hipStream_t s, const VgArgs& a)
 void launch_vg_picard(hipStream_t s, const VgArgs& a)
 {
     hipLaunchKernelGGL(k_vg_picard, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a);
+}
+
+void launch_vg_picard_gated(hipStream_t s, const VgArgs& a, const PcgState* st)
+{
+    hipLaunchKernelGGL(k_vg_picard_gated, dim3(grid_for(a.g.n)), dim3(kBlock), 0, s, a, st);
+}
+
+void launch_refine_conv(hipStream_t s, const double* part, int nparts, double lin_tol, PcgState* st)
+{
+    hipLaunchKernelGGL(k_refine_conv, dim3(1), dim3(kBlock), 0, s, part, nparts, lin_tol, st);
 }
 
 void launch_max_rows2(hipStream_t s, const double* part, int nparts, double* out)
