@@ -530,6 +530,7 @@ int next_segment_batch(const dhara_ctx* c, int n)
 int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
 {
     int rc;
+    c->vg_ready = false;
     if ((rc = alloc_line(c, true))) return rc;
     const int aparts = dh::launch_assemble_tiled(c->s, asm_args(c, false), true, c->tile[0]);
     LAUNCHCHK(c);
