@@ -567,7 +567,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
     // that usually ends the solve: its check-only form (gated: it runs only once the segment has
     // ended unconverged) is then queued behind every batch of the segment, and the update's
     // partials come back with the segment's state in the same synchronisation.
-    const double target = c->p.lin_tol * c->p.lin_tol;   // x ||b||^2
+    const double target = c->p.lin_tol * c->p.lin_tol;   // outer target of ||r||^2 / ||b||^2
     bool spec = false, line_update = true;   // line_update: k_refine_line fits the surface
     int spec_parts = 0;
     hipError_t spec_err = hipSuccess;
