@@ -574,6 +574,19 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
     auto expect_floor = [&](double rr0) {   // the next segment's stopping test is its floor
         return line_update && c->p.inner_tol * c->p.inner_tol * rr0 <= 0.5 * 0.25 * target * c->h_st->bb;
     };
+    // the gated check-only update, the outer test on the device and, if it passes, the Picard
+    // pass that follows the solve (theta, K, C, increment), whose maxima arrive with the state;
+    // returns the update's partial count (0: the surface is too large for k_refine_line)
+    auto queue_update = [&]() -> int {
+        const int n = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineCheck);
+        if (!n) return 0;
+        dh::launch_refine_conv(c->s, c->part_line, n, c->p.lin_tol, c->st_sub);
+        dh::launch_vg_picard_gated(c->s, vg_args(c, c->x, c->theta_new), c->st_sub);
+        dh::launch_max_rows2(c->s, c->part[0], dh::grid_for(c->N), c->dred);
+        if (spec_err == hipSuccess)
+            spec_err = hipMemcpyAsync(c->h_red, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s);
+        return n;
+    };
     for (;;) {
         const int seg = it;
         rc = run_krylov_then(
@@ -596,16 +609,7 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
                                            c->zl32, k, ev);
             },
             [&] {
-                if (!spec) return;
-                spec_parts = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineCheck);
-                if (!spec_parts) return;
-                // the outer test on the device and, if it passes, the Picard pass that follows the
-                // solve (theta, K, C, increment): its maxima arrive with the state
-                dh::launch_refine_conv(c->s, c->part_line, spec_parts, c->p.lin_tol, c->st_sub);
-                dh::launch_vg_picard_gated(c->s, vg_args(c, c->x, c->theta_new), c->st_sub);
-                dh::launch_max_rows2(c->s, c->part[0], dh::grid_for(c->N), c->dred);
-                if (spec_err == hipSuccess)
-                    spec_err = hipMemcpyAsync(c->h_red, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s);
+                if (spec) spec_parts = queue_update();
             },
             it, first_batch);
         if (rc) return rc;
@@ -639,6 +643,28 @@ int solve_mixed_line(dhara_ctx* c, dhara_step_stats* st)
         // test is then made here, on the host, from the update's FP64 residual (slot 3 of its
         // partials), and the update computes only that residual until the test has failed
         const bool check = first_batch == 1 && it_end < maxit && updates <= c->p.refine_maxiter;
+        if (check && line_update) {
+            // the same queue as behind a batch (the gate opens: the segment has ended
+            // unconverged), with one synchronisation for the update, its test and the Picard pass
+            if (queue_update()) {
+                LAUNCHCHK(c);
+                if ((rc = read_state(c, c->st_sub))) return rc;
+                if (spec_err != hipSuccess) return set_err(c, DHARA_E_DEVICE, hipGetErrorString(spec_err));
+                if (c->h_st->gate == 2) {
+                    outer_conv = true;
+                    c->vg_ready = true;
+                    break;
+                }
+                const double rr64 = c->h_st->rr_update;
+                seg_parts = dh::launch_refine_line(c->s, ra, c->zl32, dh::kRefineRest);
+                LAUNCHCHK(c);
+                spec = expect_floor(rr64);
+                it = it_end;
+                sc.first_sweep = 0;
+                continue;
+            }
+            line_update = false;
+        }
         // x += d, FP64 residual, and the next segment's z = T^-1 r (partials to part_line)
         seg_parts = dh::launch_refine_line(c->s, ra, c->zl32, check ? dh::kRefineCheck : dh::kRefineFull);
         const bool partial = check && seg_parts != 0;   // r32, d and z32 not written yet
