@@ -148,6 +148,30 @@ def test_column_autotuned_tiles(hip, orc):
     assert np.max(np.abs(gm.download("PSI") - om.download("PSI"))) <= 1e-6
 
 
+@pytest.mark.parametrize("inner_tol", [1e-5, 1e-7])
+def test_mixed_column_tight_tolerance_matches_fp64(hip, inner_tol):
+    """Mixed column PCG at ||r|| <= 1e-14 ||b|| (the FP64 attainable end): several FP32 segments
+    and reliable updates per solve, through the queued update with its outer test on the device,
+    the rate-sized batches and the rest of an update after a failed test.  Every solve converges,
+    with the Picard counts of the FP64 column run, and the heads are that run's to within the
+    solve tolerance."""
+    w = inputs.c2_coupled(nx=48, ny=40, nz=20)
+    wm = inputs.c2_coupled(nx=48, ny=40, nz=20, precision=_abi.MIXED)
+    f64 = model(hip, w, _abi.PRECOND_COLUMN, lin_tol=1e-14)
+    mix = model(hip, wm, _abi.PRECOND_COLUMN, lin_tol=1e-14, inner_tol=inner_tol)
+    steps = 4
+    sf, sm = w.run(f64, steps), wm.run(mix, steps)
+    picard = sum(s["picard_iters"] for s in sm)
+    updates = sum(s["refine_sweeps"] for s in sm)
+    err = np.max(np.abs(mix.download("PSI") - f64.download("PSI")))
+    print(f"inner_tol {inner_tol}: picard {picard} (fp64 {sum(s['picard_iters'] for s in sf)}), "
+          f"reliable updates {updates}, max |dpsi| {err:.2e} m", flush=True)
+    assert all(s["lin_converged"] for s in sm) and all(s["lin_converged"] for s in sf)
+    assert [s["picard_iters"] for s in sm] == [s["picard_iters"] for s in sf]
+    assert updates >= picard
+    assert err <= 1e-8
+
+
 def test_column_huge_surface_fallbacks(hip):
     """A surface with more 32 x 8 column tiles than partial slots (1056 x 1000: 4125 > 4096):
     the SpMV and assembly run untiled, the mixed reliable update falls back to the separate FP64
